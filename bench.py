@@ -1,0 +1,140 @@
+"""Benchmark of the MI355X sieve worker: primes/sec sieved + BSGS'd at p ~ 1e12 on one GPU.
+
+Workload (one step): the item Factor=10,111,P0=1e12,P1=1e12+2^28,N0=0,N1=1e7,mod=6,res=4 with
+per-prime residue classes on, run end to end: device sieve, residue filters, BSGS, and the exact
+host re-check of every flagged number, chunked as the reference chunks it with MemoryAllowed=64
+and CPU_count=0 (2^26 integers per chunk).  value = primes in [P0, P1) / seconds per step.
+
+Prints one JSON line.  The CPU baseline is the oracle (a C restatement of the reference's CPU
+path) timed on a bounded slice of the same item on this machine's cores.
+"""
+import argparse
+import ctypes
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+
+WORKLOAD = dict(b=10, c=111, P0=10**12, P1=10**12 + 2**28, N0=0, N1=10_000_000, mod=6, res=4)
+CHUNK = 1 << 26
+# fp64 operations per modular product of the FP64 field: mul, fma, fma + add (quotient), fma + add
+FLOP_PER_PRODUCT_F64 = 9
+# 32-bit multiplies per 64-bit Montgomery product (a*b: 4, lo*ninv: 3, high half of m*p: 4)
+MUL_PER_PRODUCT_MONT = 11
+PEAK_FP64_TFLOPS = 78.6          # MI355X published FP64 vector/matrix peak
+PEAK_INT_MUL_TOPS = 9.8          # 256 CU x 16 lanes/clk x 2.4 GHz (quarter-rate 32-bit multiply)
+
+
+def load_mps():
+    spec = importlib.util.spec_from_file_location("mps", os.path.join(ROOT, "mass-prime-siever_amd", "mps.py"))
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mps"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def prime_count(P0, P1):
+    with open(os.path.join(ROOT, "tests", "golden", "prime_counts.json")) as f:
+        counts = json.load(f)
+    return counts[f"{P0}:{P1}"]
+
+
+def cpu_baseline(threads, slice_ints):
+    """The oracle's CPU path (segmented sieve + run_bsgs per prime) on the first slice_ints
+    integers of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    w = WORKLOAD
+    P1 = w["P0"] + slice_ints
+    item = O.make_item(w["b"], w["c"], w["P0"], P1, w["N0"], w["N1"], w["mod"], w["res"])
+    chunk = max(1 << 20, slice_ints // (4 * threads))
+    t0 = time.perf_counter()
+    rels, primes = O.process_range(item, w["P0"], P1, chunk, threads)
+    dt = time.perf_counter() - t0
+    return {"value": primes / dt, "unit": "primes/s", "cores": threads, "kind": "port",
+            "sample": f"first {slice_ints} integers of the workload's P range ({primes} primes, "
+                      f"{len(rels)} relations) in {dt:.2f} s on {threads} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-slice", type=int, default=1 << 26)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this worker runs on one GPU; --gpus must be 1 "
+                                   "(P ranges shard across GPUs as independent processes)"}))
+        sys.exit(2)
+
+    mps = load_mps()
+    w = WORKLOAD
+    g = mps.GpuSiever()
+    if not g.init(0):
+        raise SystemExit(g.banner)
+    item = mps.work_item(w["b"], w["c"], w["P0"], w["P1"], w["N0"], w["N1"], w["mod"], w["res"])
+    nprimes = prime_count(w["P0"], w["P1"])
+
+    for _ in range(args.warmup):
+        g.run_item(item, chunk_ints=CHUNK)
+    g.reset_stats()
+    times, rels = [], None
+    t_all = time.perf_counter()
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        rels, st = g.run_item(item, chunk_ints=CHUNK)
+        times.append(time.perf_counter() - t0)
+    total = time.perf_counter() - t_all
+    ms_per_step = 1000.0 * total / args.steps
+    s = g.stats()
+
+    # roofline of the dominant kernel (k_bsgs), from HIP events on the worker's stream
+    launches = max(1, s["n_bsgs_launches"])
+    ms_launch = s["ms_bsgs"] / launches
+    products_launch = s["w_bsgs"] / launches
+    if s["field"] == 1:
+        achieved = products_launch * FLOP_PER_PRODUCT_F64 / (ms_launch * 1e-3) / 1e12
+        roof = {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
+    else:
+        achieved = products_launch * MUL_PER_PRODUCT_MONT / (ms_launch * 1e-3) / 1e12
+        roof = {"bound": "valu-int-mul", "achieved": achieved, "peak": PEAK_INT_MUL_TOPS, "unit": "TOP/s",
+                "frac": achieved / PEAK_INT_MUL_TOPS, "traffic": None}
+    roof.update({"kernel": "k_bsgs", "ms_per_launch": ms_launch, "products_per_launch": products_launch})
+
+    line = {
+        "metric": "primes/sec sieved+BSGS'd at p~1e12 (1 GPU)",
+        "value": nprimes / (ms_per_step / 1000.0),
+        "unit": "primes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64" if s["field"] == 0 else "f64-exact",
+        "data": "synthetic (deterministic work item)",
+        "config": {"workload": "Factor=10,111,1000000000000,1000268435456,0,10000000,6,4 GPU_classes=1",
+                   "chunk_ints": CHUNK, "primes": nprimes, "relations": len(rels),
+                   "p50_step_ms": sorted(times)[len(times) // 2] * 1000,
+                   "stage_ms_per_step": {k: s[k] / args.steps for k in ("ms_sieve", "ms_filter", "ms_setup",
+                                                                          "ms_bsgs")},
+                   "entries_per_step": s["entries"] // args.steps,
+                   "flagged_per_step": s["flagged"] // args.steps},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(threads, args.cpu_slice)
+    print(json.dumps(line), flush=True)
+    g.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,110 @@
+/*
+ * mps_gpu.h -- C-ABI of the MI355X GPU worker for Mass Prime Siever.
+ *
+ * This is the drop-in boundary: it replaces the reference's GpuSiever chunk interface
+ * (Gpu.hpp:1040-1074) and the OpenCL/CUDA host workers behind it (Gpu.hpp:654-1036,
+ * Cuda.hpp:195-455).  Plain C types only: no exceptions, no C++ or torch types cross it.
+ * A handle may be used from several threads; calls on one handle are serialised, as the
+ * reference serialises its feeder threads with a mutex (Gpu.hpp:1059, :1065).
+ */
+#ifndef MPS_GPU_H
+#define MPS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* The fields of a work item that the device needs (Types.hpp:14-26), converted as
+ * Gpu.hpp:862-864 and :896-901 convert them: N0 < 0 is searched from 0, cls_mod 0 means 1. */
+typedef struct mps_item {
+    int64_t b;        /* base b of b^n + c */
+    int64_t c;        /* constant c */
+    int64_t N0;       /* exponents n in [N0, N1) */
+    int64_t N1;
+    uint64_t cls_mod; /* 0 or 1: every n; else only n == cls_res (mod cls_mod) */
+    uint64_t cls_res;
+} mps_item;
+
+/* Status codes.  DECLINED: this chunk must take the caller's CPU path (reference: sieve_filter
+ * returning false for a range, table or buffer limit, Gpu.hpp:858, :875, :925).  ERROR: a device
+ * error; the GPU sieve stays disabled for the rest of the process (sv_fail, Gpu.hpp:982-987).
+ * CAPACITY: the output buffer is too small; *out_n holds the count needed. */
+enum { MPS_OK = 0, MPS_DECLINED = 1, MPS_ERROR = 2, MPS_CAPACITY = 3 };
+
+typedef struct mps_gpu mps_gpu;
+
+/* GpuSiever::init (Gpu.hpp:1042-1056).  Opens HIP device `device_index` (gfx950 only) and writes
+ * a banner "GPU enabled (HIP): <name> ..." into banner (may be NULL).  Returns NULL on failure;
+ * then banner holds the reason ("GPU disabled: ..."). */
+mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap);
+
+/* GpuSiever::ok (Gpu.hpp:1057). */
+int mps_gpu_ok(const mps_gpu* g);
+
+/* GpuSiever::set_classes (Gpu.hpp:1063): per-prime residue classes on the sieve path (default on). */
+void mps_gpu_set_classes(mps_gpu* g, int on);
+
+/* GpuSiever::sieve_filter (Gpu.hpp:1064-1067, OpenClSiever::sieve_filter Gpu.hpp:857-979).
+ * Generates the primes of [lo|1, hi) on the device and writes to out[] every number it cannot
+ * rule out: each prime p of the range for which some in-class n in [N0, N1) has p | b^n + c,
+ * plus the numbers left to the host (p < 1000, p | b) and rare fingerprint false positives or
+ * base-2 pseudoprimes.  Order is unspecified; the caller sorts, dedups, range-checks and runs
+ * the exact check.  Precondition of the GPU path: hi < 2^62 and lo > 3 (Worker.hpp:209). */
+int mps_gpu_sieve_filter(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, uint64_t* out,
+                         size_t out_cap, size_t* out_n);
+
+/* GpuSiever::filter (Gpu.hpp:1058-1061, OpenClSiever::filter Gpu.hpp:726-798): the primes are
+ * given by the caller.  Writes the ascending indices i with primes[i] possibly dividing an
+ * in-class b^n + c (or left to the host) to todo[]. */
+int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item* item, uint64_t* todo,
+                   size_t todo_cap, size_t* n_todo);
+
+/* Text of the last error or decline reason on this handle (never NULL). */
+const char* mps_gpu_last_error(const mps_gpu* g);
+
+/* Releases the device buffers, stream and events of the handle. */
+void mps_gpu_destroy(mps_gpu* g);
+
+/* Counters and device times accumulated over sieve_filter calls.  Times come from HIP events
+ * recorded on the worker's stream around each kernel launch. */
+typedef struct mps_gpu_stats {
+    uint64_t passes;
+    uint64_t odds;          /* odd numbers sieved */
+    uint64_t survivors;     /* numbers with no prime factor <= 65521 other than themselves */
+    uint64_t candidates;    /* survivors passing the residue filters */
+    uint64_t entries;       /* BSGS entries (prime passed Fermat and the power test) */
+    uint64_t flagged;       /* numbers written to out[] */
+    uint64_t flag2;         /* of which left to the host (p < 1000, p | b, entry overflow) */
+    uint64_t fermat_fail;
+    uint64_t w_fermat;      /* modular products, by stage */
+    uint64_t w_setup;
+    uint64_t w_bsgs;        /* baby + giant steps: sum over entries of m + s * giant steps */
+    double ms_sieve, ms_filter, ms_setup, ms_bsgs;  /* summed kernel durations */
+    uint64_t n_bsgs_launches;
+    int field;              /* field of the last pass: 0 = Montgomery, 1 = FP64 */
+    int tbits;              /* log2 of the BSGS table size of the last pass */
+} mps_gpu_stats;
+
+int mps_gpu_get_stats(const mps_gpu* g, mps_gpu_stats* s);
+void mps_gpu_reset_stats(mps_gpu* g);
+
+/* Tuning and test hooks (no counterpart in the reference).
+ * field: -1 = automatic (FP64 when hi <= 2^50, else Montgomery), 0 = Montgomery, 1 = FP64.
+ * pass_odds: odd numbers per device pass (buffers are sized from it; 0 keeps the current). */
+int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds);
+
+/* Stage outputs of one pass over [lo|1, hi) for stage-level parity tests:
+ *   stage 0: survivors (values), stage 1: candidates (value, info) pairs,
+ *   stage 2: entries (value, m, giant steps, s | info << 32) quadruples.
+ * out receives the records as u64 words; *out_n = number of records. */
+int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const mps_item* item, uint64_t* out,
+                        size_t out_cap_words, size_t* out_n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

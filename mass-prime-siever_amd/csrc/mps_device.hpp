@@ -1,0 +1,209 @@
+// Device arithmetic for the MI355X (gfx950) sieve worker.
+//
+// Two exact prime fields share one interface so that every stage is a template over the field:
+//   MontField : 64-bit Montgomery arithmetic, odd p < 2^62, R = 2^64.  The REDC carry out of the
+//               low word is (lo != 0), because lo + (lo * ninv mod 2^64) * p == 0 mod 2^64, so the
+//               low half of m*p is never formed: 11 32-bit multiplies per product.
+//   F64Field  : exact modular products in IEEE double for p < 2^50, on the full-rate FP64 FMA
+//               pipes of CDNA4.  x = a*b is split exactly as h + l (h = fl(a*b), l = fma(a,b,-h));
+//               the quotient q = rint(h/p) comes from one FMA against 1.5*2^52; r = fma(-q,p,h) + l
+//               is exact because |h - q*p| < 2^53.  |q - x/p| < 0.76, so r lies in (-p, p): values
+//               are kept in that signed range and made canonical only where they are compared.
+// Both give the canonical residue through key(), which every hash and equality test uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mps {
+
+__device__ __forceinline__ uint64_t umulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
+
+// x mod d for d < 2^16, from 32-bit remainders only; r32 = 2^32 mod d.
+__device__ __forceinline__ uint32_t mod_small(uint64_t x, uint32_t d, uint32_t r32) {
+    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    const uint32_t r = (hi % d) * r32 + (lo % d);
+    return r % d;
+}
+
+__device__ __forceinline__ uint32_t gcd32(uint32_t a, uint32_t b) {
+    while (b) {
+        const uint32_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+// Jacobi symbol (a/n) for odd n, both below 2^32.
+__device__ __forceinline__ int jacobi32(uint32_t a, uint32_t n) {
+    int s = 1;
+    a %= n;
+    while (a != 0) {
+        const uint32_t e = __builtin_ctz(a);
+        a >>= e;
+        if ((e & 1) && ((n & 7) == 3 || (n & 7) == 5)) s = -s;
+        if ((a & 3) == 3 && (n & 3) == 3) s = -s;
+        const uint32_t r = n % a;
+        n = a;
+        a = r;
+    }
+    return n == 1 ? s : 0;
+}
+
+// Jacobi symbol (a/n) for odd n < 2^63, binary method (no 64-bit division).
+__device__ __forceinline__ int jacobi64(uint64_t a, uint64_t n) {
+    int s = 1;
+    while (a != 0) {
+        const uint32_t e = (uint32_t)__builtin_ctzll(a);
+        a >>= e;
+        if ((e & 1) && ((n & 7) == 3 || (n & 7) == 5)) s = -s;
+        if (a < n) {
+            const uint64_t x = a;
+            a = n;
+            n = x;
+            if ((a & 3) == 3 && (n & 3) == 3) s = -s;
+        }
+        a -= n;
+    }
+    return n == 1 ? s : 0;
+}
+
+// (a/p) for a < 2^16, p odd < 2^63: strip the twos, then one reciprocity step brings the
+// problem into 32 bits.
+__device__ __forceinline__ int jacobi_small(uint32_t a, uint64_t p) {
+    if (a == 0) return 0;
+    int s = 1;
+    const uint32_t e = __builtin_ctz(a);
+    a >>= e;
+    const uint32_t p8 = (uint32_t)p & 7;
+    if ((e & 1) && (p8 == 3 || p8 == 5)) s = -s;
+    if (a == 1) return s;
+    if ((a & 3) == 3 && (p & 3) == 3) s = -s;
+    const uint32_t r32 = (uint32_t)((0x100000000ULL) % a);
+    return s * jacobi32(mod_small(p, a, r32), a);
+}
+
+// Jacobi symbol (x mod p / p) for x given as a signed 64-bit value (b or -c of an item).
+__device__ __forceinline__ int jacobi_signed(int64_t x, uint64_t p) {
+    const uint64_t ax = x < 0 ? (uint64_t)(-x) : (uint64_t)x;
+    int s = 1;
+    if (x < 0 && (p & 3) == 3) s = -1;  // (-1/p)
+    if (ax < 65536u) return s * jacobi_small((uint32_t)ax, p);
+    return s * jacobi64(ax % p, p);
+}
+
+// ---------------------------------------------------------------------------------------------
+struct MontField {
+    typedef uint64_t T;
+    uint64_t p, ninv, one_;
+
+    __device__ __forceinline__ void init(uint64_t mod) {
+        p = mod;
+        uint64_t inv = (3 * mod) ^ 2;  // correct to 5 bits for odd mod
+        for (int i = 0; i < 4; ++i) inv *= 2 - mod * inv;
+        ninv = 0 - inv;
+        one_ = (0 - mod) % mod;
+    }
+    __device__ __forceinline__ void set(uint64_t mod, uint64_t ninv_, uint64_t one) {
+        p = mod;
+        ninv = ninv_;
+        one_ = one;
+    }
+    __device__ __forceinline__ T one() const { return one_; }
+    __device__ __forceinline__ T mul(T a, T b) const {
+        const uint64_t lo = a * b;
+        const uint64_t hi = umulhi64(a, b);
+        const uint64_t m = lo * ninv;
+        const uint64_t t = hi + umulhi64(m, p) + (lo != 0 ? 1 : 0);
+        return t >= p ? t - p : t;
+    }
+    __device__ __forceinline__ T dbl(T a) const {
+        const uint64_t r = a + a;
+        return r >= p ? r - p : r;
+    }
+    // R^2 mod p as the Montgomery form of 2^64: square mont(2) six times.
+    __device__ __forceinline__ T r2() const {
+        T x = dbl(one_);
+        for (int i = 0; i < 6; ++i) x = mul(x, x);
+        return x;
+    }
+    __device__ __forceinline__ T from(uint64_t x, T rr) const { return mul(x, rr); }  // x < p
+    __device__ __forceinline__ uint64_t key(T a) const { return a; }
+    __device__ __forceinline__ uint64_t aux() const { return ninv; }
+    __device__ __forceinline__ static uint64_t to_bits(T a) { return a; }
+    __device__ __forceinline__ static T from_bits(uint64_t a) { return a; }
+};
+
+struct F64Field {
+    typedef double T;
+    double p, pinv;
+
+    __device__ __forceinline__ void init(uint64_t mod) {
+        p = (double)mod;
+        pinv = 1.0 / p;
+    }
+    __device__ __forceinline__ void set(uint64_t mod, uint64_t pinv_bits, uint64_t) {
+        p = (double)mod;
+        pinv = __longlong_as_double((long long)pinv_bits);
+    }
+    __device__ __forceinline__ T one() const { return 1.0; }
+    __device__ __forceinline__ T mul(T a, T b) const {
+        const double h = a * b;
+        const double l = __fma_rn(a, b, -h);
+        const double q = __fma_rn(h, pinv, 6755399441055744.0) - 6755399441055744.0;
+        return __fma_rn(-q, p, h) + l;
+    }
+    __device__ __forceinline__ T dbl(T a) const {
+        double r = a + a;
+        r = r >= p ? r - p : r;
+        return r <= -p ? r + p : r;
+    }
+    __device__ __forceinline__ T r2() const { return 1.0; }
+    __device__ __forceinline__ T from(uint64_t x, T) const { return (double)x; }
+    // canonical residue as an integer: adding 2^52 puts the integer in the mantissa bits
+    __device__ __forceinline__ uint64_t key(T a) const {
+        const double c = a < 0.0 ? a + p : a;
+        return (uint64_t)__double_as_longlong(c + 4503599627370496.0) & 0x000FFFFFFFFFFFFFULL;
+    }
+    __device__ __forceinline__ uint64_t aux() const { return (uint64_t)__double_as_longlong(pinv); }
+    __device__ __forceinline__ static uint64_t to_bits(T a) { return (uint64_t)__double_as_longlong(a); }
+    __device__ __forceinline__ static T from_bits(uint64_t a) { return __longlong_as_double((long long)a); }
+};
+
+template <class F>
+__device__ __forceinline__ typename F::T fpow(const F& f, typename F::T base, uint64_t e) {
+    typename F::T r = f.one();
+    while (e) {
+        if (e & 1) r = f.mul(r, base);
+        e >>= 1;
+        if (e) base = f.mul(base, base);
+    }
+    return r;
+}
+
+template <class F>
+__device__ __forceinline__ bool fequal(const F& f, typename F::T a, typename F::T b) {
+    return f.key(a) == f.key(b);
+}
+
+// 2^(p-1) == 1 (mod p) by left-to-right square-and-double.
+template <class F>
+__device__ __forceinline__ bool fermat2(const F& f, uint64_t p) {
+    const uint64_t e = p - 1;
+    int k = 63 - __builtin_clzll(e);
+    typename F::T x = f.one();
+    for (; k >= 0; --k) {
+        x = f.mul(x, x);
+        if ((e >> k) & 1) x = f.dbl(x);
+    }
+    return f.key(x) == f.key(f.one());
+}
+
+// 32-bit table word of a canonical residue: the two halves folded; 0xFFFFFFFF marks an empty slot.
+__device__ __forceinline__ uint32_t table_word(uint64_t key) {
+    const uint32_t w = (uint32_t)key ^ (uint32_t)(key >> 32) * 0x9E3779B1u;
+    return w == 0xFFFFFFFFu ? 0xFFFFFFFEu : w;
+}
+
+}  // namespace mps

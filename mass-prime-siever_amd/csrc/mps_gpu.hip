@@ -1,0 +1,528 @@
+// Host side of the MI355X GPU worker: the C-ABI of include/mps_gpu.h.
+//
+// One handle owns one HIP stream and one set of pass buffers.  sieve_filter cuts [lo, hi) into
+// passes of up to pass_odds odd numbers; each pass runs reset -> sieve -> filter -> setup -> bsgs
+// and reads the stage counts back to size the next launch (this is the plain per-chunk worker;
+// the batched worker in mps_batch.hip keeps the counts on the device).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "mps_gpu.h"
+#include "mps_kernels.h"
+#include "mps_worker_internal.h"
+
+using namespace mps;
+
+namespace {
+
+uint64_t gcd_u64(uint64_t a, uint64_t b) {
+    while (b) {
+        const uint64_t t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+void write_str(char* dst, size_t cap, const std::string& s) {
+    if (dst == nullptr || cap == 0) return;
+    const size_t n = std::min(cap - 1, s.size());
+    std::memcpy(dst, s.data(), n);
+    dst[n] = 0;
+}
+
+}  // namespace
+
+namespace mps {
+
+bool item_params(const mps_item* it, bool classes_on, ItemParams& ip, uint32_t& tbits, std::string& why) {
+    std::memset(&ip, 0, sizeof(ip));
+    ip.b = it->b;
+    ip.c = it->c;
+    const uint64_t N0 = it->N0 < 0 ? 0 : (uint64_t)it->N0;
+    const uint64_t N1 = it->N1 < 0 ? 0 : (uint64_t)it->N1;
+    const uint64_t cm = it->cls_mod == 0 ? 1 : it->cls_mod;
+    ip.N0 = N0;
+    ip.N1 = N1;
+    ip.cm = cm;
+    ip.cres = it->cls_res;
+    ip.n_start = N0 + ((it->cls_res % cm) + cm - (N0 % cm)) % cm;
+    ip.kcount = ip.n_start >= N1 ? 0 : (N1 - ip.n_start + cm - 1) / cm;
+    ip.cm_r32 = cm < 65536 ? (uint32_t)((1ull << 32) % cm) : 0;
+    // gpu_classes (Math.hpp:324-337): only for an even cm with lcm(cm, 24) <= 48 and a long range
+    bool qmode = false;
+    if (classes_on && it->N0 >= 0 && it->N1 > it->N0) {
+        const uint64_t span = (uint64_t)(it->N1 - it->N0);
+        const uint64_t Q = cm / gcd_u64(cm, 24) * 24;
+        if ((cm & 1) == 0 && Q <= 48 && span / Q >= 4096) {
+            qmode = true;
+            ip.Qmax = Q;
+            ip.n0modQ = (uint32_t)(N0 % Q);
+            ip.Q_r32 = (uint32_t)((1ull << 32) % Q);
+        }
+    }
+    ip.qmode = qmode ? 1 : 0;
+    // LDS table: 2^tbits 32-bit words at <= 80% load for the largest m of the item
+    uint64_t m;
+    if (qmode) {
+        m = kClassMaxM;
+    } else {
+        const uint64_t kmax = (N1 > N0 ? (N1 - N0) / cm : 0) + 2;
+        m = (uint64_t)std::sqrt((double)kmax) + 2;
+    }
+    tbits = 4;
+    while ((1ull << tbits) < m + m / 4) ++tbits;
+    if (tbits > 14) {
+        why = "BSGS table would exceed 2^14 entries";
+        return false;
+    }
+    return true;
+}
+
+}  // namespace mps
+
+struct mps_gpu {
+    int device = 0;
+    bool ok = false;
+    bool classes_on = true;
+    bool broken = false;
+    int field_mode = -1;
+    uint64_t pass_odds = 1ull << 25;
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    uint32_t* d_sp = nullptr;
+    uint32_t* d_sp_r32 = nullptr;
+    uint32_t nsp = 0, nsmall = 0;
+    PassBuffers pb{};
+    uint64_t buf_odds = 0;
+    uint64_t* d_vals = nullptr;
+    size_t vals_cap = 0;
+    Counters* h_cnt = nullptr;
+    uint64_t* h_out = nullptr;
+    hipEvent_t ev[6] = {};
+    mps_gpu_stats st{};
+    std::string err;
+    std::string name;
+    int num_cu = 0;
+};
+
+namespace {
+
+bool hip_fail(mps_gpu* g, hipError_t e, const char* what) {
+    g->err = std::string(what) + ": " + hipGetErrorString(e);
+    g->broken = true;  // sticky, like sv_fail (Gpu.hpp:982-987)
+    return false;
+}
+
+#define HIPCHK(expr, what)                                  \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(g, e_, what); \
+    } while (0)
+
+void free_buffers(mps_gpu* g) {
+    hipFree(g->pb.surv);
+    hipFree(g->pb.cand);
+    hipFree(g->pb.ent);
+    hipFree(g->pb.out);
+    hipHostFree(g->h_out);
+    g->pb.surv = nullptr;
+    g->pb.cand = nullptr;
+    g->pb.ent = nullptr;
+    g->pb.out = nullptr;
+    g->h_out = nullptr;
+    g->buf_odds = 0;
+}
+
+bool ensure_buffers(mps_gpu* g, uint64_t odds) {
+    if (g->buf_odds >= odds) return true;
+    free_buffers(g);
+    const uint64_t ent_cap = std::max<uint64_t>(odds / 8, 4096);
+    const uint64_t out_cap = std::max<uint64_t>(odds / 16, 4096);
+    HIPCHK(hipMalloc(&g->pb.surv, odds * sizeof(uint32_t)), "surv alloc");
+    HIPCHK(hipMalloc(&g->pb.cand, odds * sizeof(uint64_t)), "cand alloc");
+    HIPCHK(hipMalloc(&g->pb.ent, ent_cap * kEntWords * sizeof(uint64_t)), "entry alloc");
+    HIPCHK(hipMalloc(&g->pb.out, out_cap * sizeof(uint64_t)), "out alloc");
+    HIPCHK(hipHostMalloc((void**)&g->h_out, out_cap * sizeof(uint64_t), hipHostMallocDefault), "pinned out");
+    g->pb.surv_cap = (uint32_t)odds;
+    g->pb.cand_cap = (uint32_t)odds;
+    g->pb.ent_cap = (uint32_t)ent_cap;
+    g->pb.out_cap = (uint32_t)out_cap;
+    g->buf_odds = odds;
+    return true;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+bool read_counts(mps_gpu* g) {
+    HIPCHK(hipMemcpyAsync(g->h_cnt, g->pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, g->stream), "count read");
+    HIPCHK(hipStreamSynchronize(g->stream), "stream sync");
+    return true;
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t per_block) {
+    const uint64_t b = (n + per_block - 1) / per_block;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 30));
+}
+
+// One pass through the stage chain.  stop_stage: 0 after the sieve, 1 after the filter,
+// 2 after setup, 3 = full.  Leaves the final counts in g->h_cnt.
+bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, FieldKind fk, uint32_t tbits,
+              int stop_stage) {
+    hipStream_t s = g->stream;
+    launch_reset(s, g->pb.cnt, g->pb.vals == nullptr ? 0u : (uint32_t)pc);
+    HIPCHK(hipEventRecord(g->ev[0], s), "event");
+    if (g->pb.vals == nullptr) launch_sieve(s, plo, pc, g->d_sp, g->d_sp_r32, g->nsp, g->nsmall, g->pb, 0);
+    HIPCHK(hipGetLastError(), "sieve launch");
+    HIPCHK(hipEventRecord(g->ev[1], s), "event");
+    if (!read_counts(g)) return false;
+    const uint32_t n_surv = std::min(g->h_cnt->n_surv, g->pb.surv_cap);
+    g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
+    if (stop_stage == 0 || n_surv == 0) return true;
+
+    HIPCHK(hipEventRecord(g->ev[1], s), "event");
+    launch_filter(s, plo, ip, g->pb, blocks_for(n_surv, 256));
+    HIPCHK(hipGetLastError(), "filter launch");
+    HIPCHK(hipEventRecord(g->ev[2], s), "event");
+    if (!read_counts(g)) return false;
+    g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
+    const uint32_t n_cand = std::min(g->h_cnt->n_cand, g->pb.cand_cap);
+    if (stop_stage == 1 || n_cand == 0) return true;
+
+    HIPCHK(hipEventRecord(g->ev[2], s), "event");
+    launch_setup(s, fk, plo, ip, g->pb, blocks_for(n_cand, 256));
+    HIPCHK(hipGetLastError(), "setup launch");
+    HIPCHK(hipEventRecord(g->ev[3], s), "event");
+    if (!read_counts(g)) return false;
+    g->st.ms_setup += elapsed(g->ev[2], g->ev[3]);
+    const uint32_t n_ent = std::min(g->h_cnt->n_ent, g->pb.ent_cap);
+    if (stop_stage == 2 || n_ent == 0) return true;
+
+    HIPCHK(hipEventRecord(g->ev[3], s), "event");
+    launch_bsgs(s, fk, tbits, g->pb, n_ent, 0);
+    HIPCHK(hipGetLastError(), "bsgs launch");
+    HIPCHK(hipEventRecord(g->ev[4], s), "event");
+    if (!read_counts(g)) return false;
+    g->st.ms_bsgs += elapsed(g->ev[3], g->ev[4]);
+    g->st.n_bsgs_launches++;
+    return true;
+}
+
+void accumulate(mps_gpu* g, uint64_t pc) {
+    const Counters& c = *g->h_cnt;
+    g->st.passes++;
+    g->st.odds += pc;
+    g->st.survivors += c.n_surv;
+    g->st.candidates += c.n_cand;
+    g->st.entries += c.n_ent;
+    g->st.flagged += c.n_out;
+    g->st.flag2 += c.n_flag2 + (c.n_ent > g->pb.ent_cap ? c.n_ent - g->pb.ent_cap : 0);
+    g->st.fermat_fail += c.n_fermat_fail;
+    g->st.w_fermat += c.w_fermat;
+    g->st.w_setup += c.w_setup;
+    g->st.w_bsgs += c.w_bsgs;
+}
+
+FieldKind pick_field(const mps_gpu* g, uint64_t hi) {
+    if (g->field_mode == 0) return FIELD_MONT;
+    if (g->field_mode == 1 && hi <= (1ull << 50)) return FIELD_F64;
+    return hi <= (1ull << 50) ? FIELD_F64 : FIELD_MONT;
+}
+
+}  // namespace
+
+extern "C" {
+
+mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        write_str(banner, banner_cap, "GPU disabled: no HIP device");
+        return nullptr;
+    }
+    if (device_index < 0 || device_index >= ndev) device_index = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device_index) != hipSuccess) {
+        write_str(banner, banner_cap, "GPU disabled: hipGetDeviceProperties failed");
+        return nullptr;
+    }
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+        write_str(banner, banner_cap, std::string("GPU disabled: kernels are built for gfx950, device is ") +
+                                          prop.gcnArchName);
+        return nullptr;
+    }
+    mps_gpu* g = new mps_gpu();
+    g->device = device_index;
+    g->name = prop.name;
+    g->num_cu = prop.multiProcessorCount;
+    bool ok = hipSetDevice(device_index) == hipSuccess && hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess;
+    for (int i = 0; ok && i < 6; ++i) ok = hipEventCreate(&g->ev[i]) == hipSuccess;
+    // odd sieving primes 3 .. 65521 and 2^32 mod q
+    std::vector<uint32_t> sp, r32;
+    {
+        std::vector<uint8_t> comp(65536, 0);
+        for (uint32_t i = 3; i < 65536; i += 2) {
+            if (comp[i]) continue;
+            sp.push_back(i);
+            r32.push_back((uint32_t)((1ull << 32) % i));
+            for (uint32_t j = i * i; j < 65536; j += 2 * i) comp[j] = 1;
+        }
+    }
+    g->nsp = (uint32_t)sp.size();
+    g->nsmall = (uint32_t)(std::lower_bound(sp.begin(), sp.end(), kSmallPrimeBound) - sp.begin());
+    ok = ok && hipMalloc(&g->d_sp, sp.size() * 4) == hipSuccess && hipMalloc(&g->d_sp_r32, sp.size() * 4) == hipSuccess &&
+         hipMemcpy(g->d_sp, sp.data(), sp.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(g->d_sp_r32, r32.data(), r32.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+         hipMalloc(&g->pb.cnt, sizeof(Counters)) == hipSuccess &&
+         hipHostMalloc((void**)&g->h_cnt, sizeof(Counters), hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+        write_str(banner, banner_cap, "GPU disabled: HIP setup failed");
+        mps_gpu_destroy(g);
+        return nullptr;
+    }
+    g->ok = true;
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "GPU enabled (HIP): %s (%s), %d CUs, global mem %zu MB", prop.name,
+                  prop.gcnArchName, prop.multiProcessorCount, (size_t)(prop.totalGlobalMem >> 20));
+    write_str(banner, banner_cap, buf);
+    return g;
+}
+
+int mps_gpu_ok(const mps_gpu* g) { return g != nullptr && g->ok ? 1 : 0; }
+
+void mps_gpu_set_classes(mps_gpu* g, int on) {
+    if (g) g->classes_on = on != 0;
+}
+
+int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds) {
+    if (g == nullptr) return MPS_ERROR;
+    std::lock_guard<std::mutex> lock(g->mu);
+    g->field_mode = field;
+    if (pass_odds) {
+        if (pass_odds < 4096 || pass_odds > (1ull << 31)) return MPS_ERROR;
+        g->pass_odds = pass_odds;
+    }
+    return MPS_OK;
+}
+
+int mps_gpu_sieve_filter(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, uint64_t* out,
+                         size_t out_cap, size_t* out_n) {
+    if (out_n) *out_n = 0;
+    if (g == nullptr || item == nullptr || out_n == nullptr) return MPS_ERROR;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if (!g->ok) return MPS_ERROR;
+    if (g->broken) return MPS_ERROR;
+    if (hi >= (1ull << 62) || lo <= 3) {
+        g->err = "range outside (3, 2^62)";
+        return MPS_DECLINED;
+    }
+    lo |= 1;
+    if (lo >= hi) return MPS_OK;
+    ItemParams ip;
+    uint32_t tbits = 0;
+    if (!item_params(item, g->classes_on, ip, tbits, g->err)) return MPS_DECLINED;
+    const FieldKind fk = pick_field(g, hi);
+    const uint64_t count = (hi - lo + 1) / 2;
+    if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
+    if (!ensure_buffers(g, std::min(count, g->pass_odds))) return MPS_ERROR;
+    g->pb.vals = nullptr;
+    g->pb.need_prime = 1;
+    g->pb.out_index = 0;
+    size_t n = 0;
+    for (uint64_t pb0 = 0; pb0 < count; pb0 += g->pass_odds) {
+        const uint64_t pc = std::min(g->pass_odds, count - pb0);
+        const uint64_t plo = lo + 2 * pb0;
+        if (!run_pass(g, plo, pc, ip, fk, tbits, 3)) return MPS_ERROR;
+        accumulate(g, pc);
+        const uint32_t n_out = g->h_cnt->n_out;
+        if (n_out > g->pb.out_cap) {
+            g->err = "flagged numbers exceed the device buffer";
+            return MPS_DECLINED;
+        }
+        if (n_out) {
+            if (hipMemcpyAsync(g->h_out, g->pb.out, n_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                hipStreamSynchronize(g->stream) != hipSuccess) {
+                hip_fail(g, hipErrorUnknown, "flag read");
+                return MPS_ERROR;
+            }
+            for (uint32_t i = 0; i < n_out; ++i) {
+                if (n < out_cap && out) out[n] = g->h_out[i];
+                ++n;
+            }
+        }
+    }
+    g->st.field = (int)fk;
+    g->st.tbits = (int)tbits;
+    *out_n = n;
+    return n > out_cap ? MPS_CAPACITY : MPS_OK;
+}
+
+int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item* item, uint64_t* todo,
+                   size_t todo_cap, size_t* n_todo) {
+    if (n_todo) *n_todo = 0;
+    if (g == nullptr || item == nullptr || n_todo == nullptr) return MPS_ERROR;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if (!g->ok || g->broken) return MPS_ERROR;
+    if (n == 0) return MPS_OK;
+    ItemParams ip;
+    uint32_t tbits = 0;
+    if (!item_params(item, false, ip, tbits, g->err)) return MPS_DECLINED;  // filter(): plain prep path
+    uint64_t pmax = 0;
+    for (size_t i = 0; i < n; ++i) pmax = std::max(pmax, primes[i]);
+    const FieldKind fk = pick_field(g, pmax + 1);
+    if (pmax >= (1ull << 62)) {
+        g->err = "prime above 2^62";
+        return MPS_DECLINED;
+    }
+    const size_t batch = (size_t)std::min<uint64_t>(g->pass_odds, 1ull << 22);
+    if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
+    if (!ensure_buffers(g, batch)) return MPS_ERROR;
+    if (g->vals_cap < batch) {
+        hipFree(g->d_vals);
+        g->d_vals = nullptr;
+        if (hipMalloc(&g->d_vals, batch * sizeof(uint64_t)) != hipSuccess) {
+            hip_fail(g, hipErrorOutOfMemory, "values alloc");
+            return MPS_ERROR;
+        }
+        g->vals_cap = batch;
+    }
+    std::vector<uint64_t> idx;
+    for (size_t off = 0; off < n; off += batch) {
+        const size_t cnt = std::min(batch, n - off);
+        if (hipMemcpyAsync(g->d_vals, primes + off, cnt * sizeof(uint64_t), hipMemcpyHostToDevice, g->stream) != hipSuccess) {
+            hip_fail(g, hipErrorUnknown, "values upload");
+            return MPS_ERROR;
+        }
+        g->pb.vals = g->d_vals;
+        g->pb.need_prime = 0;
+        g->pb.out_index = 1;
+        const bool ok = run_pass(g, 0, cnt, ip, fk, tbits, 3);
+        g->pb.vals = nullptr;
+        g->pb.need_prime = 1;
+        g->pb.out_index = 0;
+        if (!ok) return MPS_ERROR;
+        accumulate(g, 0);
+        const uint32_t n_out = g->h_cnt->n_out;
+        if (n_out > g->pb.out_cap) {
+            g->err = "flagged numbers exceed the device buffer";
+            return MPS_DECLINED;
+        }
+        if (n_out) {
+            if (hipMemcpyAsync(g->h_out, g->pb.out, n_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+                hipStreamSynchronize(g->stream) != hipSuccess) {
+                hip_fail(g, hipErrorUnknown, "flag read");
+                return MPS_ERROR;
+            }
+            for (uint32_t i = 0; i < n_out; ++i) idx.push_back(off + g->h_out[i]);
+        }
+    }
+    std::sort(idx.begin(), idx.end());
+    idx.erase(std::unique(idx.begin(), idx.end()), idx.end());
+    for (size_t i = 0; i < idx.size() && i < todo_cap; ++i) todo[i] = idx[i];
+    *n_todo = idx.size();
+    return idx.size() > todo_cap ? MPS_CAPACITY : MPS_OK;
+}
+
+int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const mps_item* item, uint64_t* out,
+                        size_t out_cap_words, size_t* out_n) {
+    if (out_n) *out_n = 0;
+    if (g == nullptr || item == nullptr || out_n == nullptr || stage < 0 || stage > 2) return MPS_ERROR;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if (!g->ok || g->broken) return MPS_ERROR;
+    lo |= 1;
+    if (lo <= 3 || hi >= (1ull << 62) || lo >= hi) return MPS_DECLINED;
+    ItemParams ip;
+    uint32_t tbits = 0;
+    if (!item_params(item, g->classes_on, ip, tbits, g->err)) return MPS_DECLINED;
+    const FieldKind fk = pick_field(g, hi);
+    const uint64_t count = (hi - lo + 1) / 2;
+    if (count > g->pass_odds) {
+        g->err = "debug stage covers one pass only";
+        return MPS_DECLINED;
+    }
+    if (hipSetDevice(g->device) != hipSuccess || !ensure_buffers(g, count)) return MPS_ERROR;
+    g->pb.vals = nullptr;
+    g->pb.need_prime = 1;
+    g->pb.out_index = 0;
+    if (!run_pass(g, lo, count, ip, fk, tbits, stage)) return MPS_ERROR;
+    const Counters c = *g->h_cnt;
+    size_t words = 0, recs = 0;
+    if (stage == 0) {
+        const uint32_t n = std::min(c.n_surv, g->pb.surv_cap);
+        std::vector<uint32_t> v(n);
+        if (n && hipMemcpy(v.data(), g->pb.surv, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return MPS_ERROR;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (words < out_cap_words) out[words] = lo + 2 * (uint64_t)v[i];
+            ++words;
+        }
+        recs = n;
+    } else if (stage == 1) {
+        const uint32_t n = std::min(c.n_cand, g->pb.cand_cap);
+        std::vector<uint64_t> v(n);
+        if (n && hipMemcpy(v.data(), g->pb.cand, n * 8, hipMemcpyDeviceToHost) != hipSuccess) return MPS_ERROR;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (words + 1 < out_cap_words) {
+                out[words] = lo + 2 * (v[i] & 0xFFFFFFFFull);
+                out[words + 1] = v[i] >> 32;
+            }
+            words += 2;
+        }
+        recs = n;
+    } else {
+        const uint32_t n = std::min(c.n_ent, g->pb.ent_cap);
+        std::vector<uint64_t> v((size_t)n * kEntWords);
+        if (n && hipMemcpy(v.data(), g->pb.ent, v.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return MPS_ERROR;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t* E = &v[(size_t)i * kEntWords];
+            if (words + 3 < out_cap_words) {
+                out[words] = E[E_P];
+                out[words + 1] = E[E_MG] & 0xFFFFFFFFull;
+                out[words + 2] = E[E_MG] >> 32;
+                out[words + 3] = E[E_SSRC] & 0xFFFFFFFFull;
+            }
+            words += 4;
+        }
+        recs = n;
+    }
+    *out_n = recs;
+    return words > out_cap_words ? MPS_CAPACITY : MPS_OK;
+}
+
+const char* mps_gpu_last_error(const mps_gpu* g) { return g ? g->err.c_str() : "null handle"; }
+
+int mps_gpu_get_stats(const mps_gpu* g, mps_gpu_stats* s) {
+    if (g == nullptr || s == nullptr) return MPS_ERROR;
+    *s = g->st;
+    return MPS_OK;
+}
+
+void mps_gpu_reset_stats(mps_gpu* g) {
+    if (g) g->st = mps_gpu_stats{};
+}
+
+void mps_gpu_destroy(mps_gpu* g) {
+    if (g == nullptr) return;
+    if (g->ok) hipSetDevice(g->device);
+    free_buffers(g);
+    hipFree(g->d_vals);
+    hipFree(g->d_sp);
+    hipFree(g->d_sp_r32);
+    hipFree(g->pb.cnt);
+    hipHostFree(g->h_cnt);
+    for (int i = 0; i < 6; ++i)
+        if (g->ev[i]) hipEventDestroy(g->ev[i]);
+    if (g->stream) hipStreamDestroy(g->stream);
+    delete g;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Kernel launch interface of the MI355X sieve worker (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mps {
+
+// Odd numbers per sieve segment: one workgroup marks one segment in an LDS bitmap.
+constexpr uint32_t kSegOdds = 1u << 17;           // 16 KiB bitmap
+constexpr uint32_t kSieveThreads = 256;
+constexpr uint32_t kSmallPrimeBound = 1024;       // primes below this are shared by all lanes
+constexpr uint32_t kEntWords = 64;                // u64 words per BSGS entry (512 B)
+constexpr uint32_t kMaxClasses = 24;              // classes per prime in class mode
+constexpr uint32_t kClassMaxM = 1024;             // largest baby-step count in class mode
+
+// Entry layout (u64 words).  B is the baby-step base, Bm = B^m the giant-step base.
+enum EntryWord : uint32_t {
+    E_P = 0,      // prime (or Fermat pseudoprime)
+    E_AUX = 1,    // Montgomery -p^-1 mod 2^64, or the bits of 1/p (FP64 field)
+    E_ONE = 2,    // Montgomery form of 1 (unused by the FP64 field)
+    E_T = 3,      // target t = -c (field form): baby steps are t * B^i
+    E_MG = 4,     // m (low 32) | giant steps per class (high 32)
+    E_SSRC = 5,   // classes s (bits 0..7) | info/class mask (bits 8..31) | number index (bits 32..63)
+    E_B64 = 6,    // B^64
+    E_BM64 = 7,   // Bm^64
+    E_BLO = 8,    // B^k, k = 0..7
+    E_BHI = 16,   // B^(8k), k = 0..7
+    E_MLO = 24,   // Bm^k, k = 0..7
+    E_MHI = 32,   // Bm^(8k), k = 0..7
+    E_G = 40,     // giant start per class G_j = b^(n_j), j < s
+};
+
+struct Counters {
+    uint32_t n_surv, n_cand, n_ent, n_out;
+    uint32_t n_flag2, n_fermat_fail, pad0, pad1;
+    unsigned long long w_fermat;  // Montgomery/FP64 products spent, by stage
+    unsigned long long w_setup;
+    unsigned long long w_bsgs;
+    unsigned long long pad2;
+};
+
+// Per-item constants for the filter and setup stages.
+struct ItemParams {
+    int64_t b, c;
+    uint64_t N0, N1;        // N0 clamped at 0
+    uint64_t cm, cres;      // class modulus (>= 1) and residue
+    uint64_t n_start;       // first in-class n >= N0 (plain mode)
+    uint64_t kcount;        // in-class n in [n_start, N1) (plain mode)
+    uint64_t Qmax;          // class mode: lcm(cm, 24)
+    uint32_t n0modQ;        // N0 mod Qmax
+    uint32_t qmode;         // 1 = per-prime classes
+    uint32_t cm_r32;        // 2^32 mod cm when cm < 2^16
+    uint32_t Q_r32;         // 2^32 mod Qmax
+};
+
+struct PassBuffers {
+    const uint64_t* vals;   // filter path: the numbers themselves (then surv is unused), else null
+    uint32_t* surv;         // survivor odd indices relative to the pass start
+    uint64_t* cand;         // candidate: index (low 32) | info (high 32)
+    uint64_t* ent;          // entries, kEntWords u64 each
+    uint64_t* out;          // flagged numbers
+    Counters* cnt;
+    uint32_t surv_cap, cand_cap, ent_cap, out_cap;
+    uint32_t need_prime;    // 1: base-2 Fermat test first (sieve survivors may be composite)
+    uint32_t out_index;     // 1: flagged records are indices into vals, else the numbers
+};
+
+enum FieldKind : int { FIELD_MONT = 0, FIELD_F64 = 1 };
+
+void launch_reset(hipStream_t s, Counters* cnt, uint32_t n_surv0);
+void launch_sieve(hipStream_t s, uint64_t lo, uint64_t nodd, const uint32_t* sp, const uint32_t* sp_r32,
+                  uint32_t nsp, uint32_t nsmall, const PassBuffers& pb, uint32_t grid_override);
+void launch_filter(hipStream_t s, uint64_t lo, const ItemParams& ip, const PassBuffers& pb, uint32_t grid);
+void launch_setup(hipStream_t s, FieldKind fk, uint64_t lo, const ItemParams& ip, const PassBuffers& pb,
+                  uint32_t grid);
+void launch_bsgs(hipStream_t s, FieldKind fk, uint32_t tbits, const PassBuffers& pb, uint32_t n_ent,
+                 uint32_t grid);
+
+}  // namespace mps

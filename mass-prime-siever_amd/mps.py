@@ -1,0 +1,256 @@
+"""Python view of libmps.so, the MI355X GPU worker (include/mps_gpu.h, include/mps_host.h).
+
+GpuSiever mirrors the reference's chunk interface (Gpu.hpp:1040-1074): init/ok/filter/
+set_classes/sieve_filter with the same argument meaning; a declined chunk returns False
+(Gpu.hpp:857-979) and a device error disables the GPU sieve for good.  run_item is the GPU
+branch of the reference's chunk loop plus the exact host re-check.  There is no CPU fallback:
+loading fails loudly when libmps.so is missing.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmps.so")
+
+MPS_OK, MPS_DECLINED, MPS_ERROR, MPS_CAPACITY = 0, 1, 2, 3
+
+
+class Item(ctypes.Structure):
+    """mps_item: the fields of a WorkItem the device needs (Types.hpp:14-26)."""
+    _fields_ = [("b", ctypes.c_int64), ("c", ctypes.c_int64), ("N0", ctypes.c_int64), ("N1", ctypes.c_int64),
+                ("cls_mod", ctypes.c_uint64), ("cls_res", ctypes.c_uint64)]
+
+
+class WorkItem(ctypes.Structure):
+    """mps_work_item (Types.hpp:14-26)."""
+    _fields_ = [("b", ctypes.c_int64), ("c", ctypes.c_int64), ("P0", ctypes.c_uint64), ("P1", ctypes.c_uint64),
+                ("N0", ctypes.c_int64), ("N1", ctypes.c_int64), ("cls_mod", ctypes.c_uint64),
+                ("cls_res", ctypes.c_uint64), ("P0_original", ctypes.c_uint64)]
+
+
+class Relation(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_uint64), ("M", ctypes.c_uint64), ("q", ctypes.c_uint64)]
+
+
+class RunOpts(ctypes.Structure):
+    _fields_ = [("chunk_ints", ctypes.c_uint64), ("feeder_threads", ctypes.c_int)]
+
+
+class RunStats(ctypes.Structure):
+    _fields_ = [("wall_s", ctypes.c_double), ("device_call_s", ctypes.c_double), ("check_s", ctypes.c_double),
+                ("chunks", ctypes.c_uint64), ("declined", ctypes.c_uint64), ("flagged", ctypes.c_uint64),
+                ("prime_flagged", ctypes.c_uint64), ("relations", ctypes.c_uint64)]
+
+
+class GpuStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("passes", "odds", "survivors", "candidates", "entries", "flagged",
+                                               "flag2", "fermat_fail", "w_fermat", "w_setup", "w_bsgs")] + \
+               [(n, ctypes.c_double) for n in ("ms_sieve", "ms_filter", "ms_setup", "ms_bsgs")] + \
+               [("n_bsgs_launches", ctypes.c_uint64), ("field", ctypes.c_int), ("tbits", ctypes.c_int)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def build():
+    """Compiles libmps.so for gfx950 (hipcc cross-compiles without a GPU)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+EXPORTS = ["mps_gpu_create", "mps_gpu_ok", "mps_gpu_set_classes", "mps_gpu_sieve_filter", "mps_gpu_filter",
+           "mps_gpu_last_error", "mps_gpu_destroy", "mps_gpu_get_stats", "mps_gpu_reset_stats",
+           "mps_gpu_configure", "mps_gpu_debug_stage", "mps_run_item", "mps_results_json", "mps_is_prime64",
+           "mps_multiplicative_order", "mps_run_bsgs"]
+
+_lib = None
+
+
+def lib():
+    """Loads libmps.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it first (mps.build() or make -C {HERE})")
+    L = ctypes.CDLL(LIB_PATH)
+    u64, i32, sz, vp = ctypes.c_uint64, ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p
+    P = ctypes.POINTER
+    L.mps_gpu_create.restype = vp
+    L.mps_gpu_create.argtypes = [i32, ctypes.c_char_p, sz]
+    L.mps_gpu_ok.restype = i32
+    L.mps_gpu_ok.argtypes = [vp]
+    L.mps_gpu_set_classes.restype = None
+    L.mps_gpu_set_classes.argtypes = [vp, i32]
+    L.mps_gpu_sieve_filter.restype = i32
+    L.mps_gpu_sieve_filter.argtypes = [vp, u64, u64, P(Item), vp, sz, P(sz)]
+    L.mps_gpu_filter.restype = i32
+    L.mps_gpu_filter.argtypes = [vp, vp, sz, P(Item), vp, sz, P(sz)]
+    L.mps_gpu_last_error.restype = ctypes.c_char_p
+    L.mps_gpu_last_error.argtypes = [vp]
+    L.mps_gpu_destroy.restype = None
+    L.mps_gpu_destroy.argtypes = [vp]
+    L.mps_gpu_get_stats.restype = i32
+    L.mps_gpu_get_stats.argtypes = [vp, P(GpuStats)]
+    L.mps_gpu_reset_stats.restype = None
+    L.mps_gpu_reset_stats.argtypes = [vp]
+    L.mps_gpu_configure.restype = i32
+    L.mps_gpu_configure.argtypes = [vp, i32, u64]
+    L.mps_gpu_debug_stage.restype = i32
+    L.mps_gpu_debug_stage.argtypes = [vp, i32, u64, u64, P(Item), vp, sz, P(sz)]
+    L.mps_run_item.restype = i32
+    L.mps_run_item.argtypes = [vp, P(WorkItem), P(RunOpts), vp, sz, P(sz), P(RunStats)]
+    L.mps_results_json.restype = sz
+    L.mps_results_json.argtypes = [P(WorkItem), vp, sz, ctypes.c_char_p, ctypes.c_char_p, sz]
+    L.mps_is_prime64.restype = i32
+    L.mps_is_prime64.argtypes = [u64]
+    L.mps_multiplicative_order.restype = u64
+    L.mps_multiplicative_order.argtypes = [u64, u64]
+    L.mps_run_bsgs.restype = i32
+    L.mps_run_bsgs.argtypes = [u64, P(WorkItem), P(Relation)]
+    _lib = L
+    return L
+
+
+def work_item(b, c, P0, P1, N0, N1, mod=1, res=0):
+    return WorkItem(b, c, P0, P1, N0, N1, mod, res, P0)
+
+
+def device_item(w):
+    return Item(w.b, w.c, w.N0, w.N1, w.cls_mod, w.cls_res)
+
+
+class GpuSiever:
+    """The reference's GpuSiever (Gpu.hpp:1040-1074) over the C-ABI."""
+
+    def __init__(self):
+        self.h = None
+        self.banner = ""
+
+    def init(self, device_index=0, backend="auto"):
+        if backend not in ("auto", "hip"):
+            self.banner = f"GPU disabled: GPU_backend={backend} is not available in this build"
+            return False
+        buf = ctypes.create_string_buffer(512)
+        h = lib().mps_gpu_create(device_index, buf, 512)
+        self.banner = buf.value.decode()
+        if not h:
+            return False
+        self.h = h
+        return True
+
+    def ok(self):
+        return bool(self.h) and bool(lib().mps_gpu_ok(self.h))
+
+    def set_classes(self, on):
+        lib().mps_gpu_set_classes(self.h, 1 if on else 0)
+
+    def configure(self, field=-1, pass_odds=0):
+        return lib().mps_gpu_configure(self.h, field, pass_odds)
+
+    def sieve_filter(self, lo, hi, item, flagged):
+        """Appends what the device could not rule out in [lo|1, hi) to `flagged`; False = declined."""
+        di = device_item(item) if isinstance(item, WorkItem) else item
+        cap = 1 << 16
+        while True:
+            out = np.empty(cap, dtype=np.uint64)
+            n = ctypes.c_size_t()
+            st = lib().mps_gpu_sieve_filter(self.h, lo, hi, ctypes.byref(di), out.ctypes.data, cap, ctypes.byref(n))
+            if st == MPS_CAPACITY:
+                cap = n.value + 1024
+                continue
+            if st != MPS_OK:
+                return False
+            flagged.extend(int(x) for x in out[: n.value])
+            return True
+
+    def filter(self, primes, item, todo):
+        """Appends the ascending indices of the primes that may divide an in-class candidate."""
+        di = device_item(item) if isinstance(item, WorkItem) else item
+        arr = np.ascontiguousarray(np.asarray(primes, dtype=np.uint64))
+        cap = max(1024, len(arr) // 16)
+        while True:
+            out = np.empty(cap, dtype=np.uint64)
+            n = ctypes.c_size_t()
+            st = lib().mps_gpu_filter(self.h, arr.ctypes.data, len(arr), ctypes.byref(di), out.ctypes.data, cap,
+                                      ctypes.byref(n))
+            if st == MPS_CAPACITY:
+                cap = n.value + 1024
+                continue
+            if st != MPS_OK:
+                return False
+            todo.extend(int(x) for x in out[: n.value])
+            return True
+
+    def debug_stage(self, stage, lo, hi, item):
+        di = device_item(item) if isinstance(item, WorkItem) else item
+        width = (1, 2, 4)[stage]
+        cap = ((hi - lo) // 2 + 16) * width
+        out = np.empty(cap, dtype=np.uint64)
+        n = ctypes.c_size_t()
+        st = lib().mps_gpu_debug_stage(self.h, stage, lo, hi, ctypes.byref(di), out.ctypes.data, cap,
+                                       ctypes.byref(n))
+        if st != MPS_OK:
+            raise RuntimeError(f"debug_stage {stage}: status {st}: {self.last_error()}")
+        return out[: n.value * width].reshape(-1, width) if width > 1 else out[: n.value]
+
+    def run_item(self, item, chunk_ints=0, feeder_threads=0):
+        """Relations (sorted by p) of one item, plus the run statistics."""
+        cap = 1 << 16
+        while True:
+            rels = (Relation * cap)()
+            n = ctypes.c_size_t()
+            st = RunStats()
+            opts = RunOpts(chunk_ints, feeder_threads)
+            rc = lib().mps_run_item(self.h, ctypes.byref(item), ctypes.byref(opts), rels, cap, ctypes.byref(n),
+                                    ctypes.byref(st))
+            if rc == MPS_CAPACITY:
+                cap = n.value + 1024
+                continue
+            if rc != MPS_OK:
+                raise RuntimeError(f"run_item status {rc}: {self.last_error()}")
+            return [(rels[i].p, rels[i].M, rels[i].q) for i in range(n.value)], st
+
+    def stats(self):
+        s = GpuStats()
+        lib().mps_gpu_get_stats(self.h, ctypes.byref(s))
+        return s.as_dict()
+
+    def reset_stats(self):
+        lib().mps_gpu_reset_stats(self.h)
+
+    def last_error(self):
+        return lib().mps_gpu_last_error(self.h).decode() if self.h else "no handle"
+
+    def close(self):
+        if self.h:
+            lib().mps_gpu_destroy(self.h)
+            self.h = None
+
+
+def results_json(item, rels, timestamp):
+    arr = (Relation * max(1, len(rels)))()
+    for i, (p, M, q) in enumerate(rels):
+        arr[i] = Relation(p, M, q)
+    ts = timestamp.encode()
+    need = lib().mps_results_json(ctypes.byref(item), arr, len(rels), ts, None, 0)
+    buf = ctypes.create_string_buffer(need + 1)
+    lib().mps_results_json(ctypes.byref(item), arr, len(rels), ts, buf, need + 1)
+    return buf.value.decode()
+
+
+def is_prime64(n):
+    return bool(lib().mps_is_prime64(n))
+
+
+def multiplicative_order(b, p):
+    return lib().mps_multiplicative_order(b, p)
+
+
+def run_bsgs(p, item):
+    r = Relation()
+    if lib().mps_run_bsgs(p, ctypes.byref(item), ctypes.byref(r)):
+        return (r.p, r.M, r.q)
+    return None

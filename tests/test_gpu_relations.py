@@ -1,0 +1,74 @@
+"""End-to-end parity: relations of the GPU worker (device sieve + filter + BSGS, exact host
+re-check) equal the oracle's CPU path (segmented sieve + run_bsgs on every prime) on the same
+items, compared as sorted (p, M, q), as the reference's own self-test compares them
+(kaggle/kaggle_run.py:64-95)."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ITEMS = [
+    # README item families (README.md:33-36): 2^n-25 with classes 2/1, 6/1 and none; 10^n+111 with
+    # 6/4 and none; p from 10^6 where the sieve primes reach past sqrt(p).
+    (2, -25, 10**9, 10**9 + 2**22, 3_600_000, 5_000_000, 2, 1),
+    (2, -25, 10**9, 10**9 + 2**21, 3_600_000, 5_000_000, 6, 1),
+    (2, -25, 10**9, 10**9 + 2**21, 3_600_000, 5_000_000, 1, 0),
+    (10, 111, 10**12, 10**12 + 2**22, 0, 10_000_000, 6, 4),
+    (10, 111, 10**12, 10**12 + 2**21, 0, 10_000_000, 1, 0),
+    (2, -25, 10**6, 10**6 + 2**21, 3_600_000, 5_000_000, 2, 1),
+    (3, -2, 4 * 10**18, 4 * 10**18 + 2**20, 0, 10_000_000, 1, 0),
+]
+
+
+@pytest.mark.parametrize("classes", [True, False])
+@pytest.mark.parametrize("it", ITEMS)
+def test_relations_match_oracle(gpu, mps, oracle, it, classes):
+    b, c, P0, P1, N0, N1, mod, res = it
+    print(f"relations {it} classes={classes}", flush=True)
+    gpu.set_classes(classes)
+    try:
+        rels, st = gpu.run_item(mps.work_item(b, c, P0, P1, N0, N1, mod, res), chunk_ints=1 << 20)
+    finally:
+        gpu.set_classes(True)
+    want, _ = oracle.process_range(oracle.make_item(b, c, P0, P1, N0, N1, mod, res), P0, P1, 1 << 20, 8)
+    assert rels == want
+    assert st.declined == 0
+
+
+def test_kaggle_self_test_item(gpu, mps, oracle):
+    """kaggle/kaggle_run.py:66: GPU-only relations equal the CPU ones and there is at least one."""
+    P0, P1 = 10_000_000_000, 10_020_000_000
+    item = mps.work_item(2, -25, P0, P1, 3_600_000, 5_000_000, 2, 1)
+    rels, _ = gpu.run_item(item, chunk_ints=1 << 24)
+    want, _ = oracle.process_range(oracle.make_item(2, -25, P0, P1, 3_600_000, 5_000_000, 2, 1), P0, P1, 1 << 22, 8)
+    assert rels == want
+    assert len(rels) > 1
+
+
+@pytest.mark.parametrize("field", [0, 1])
+def test_fields_agree(gpu, mps, oracle, field):
+    """The Montgomery and FP64 fields give the same relations below 2^50."""
+    P0, P1 = 10**12, 10**12 + 2**21
+    gpu.configure(field, 0)
+    try:
+        rels, _ = gpu.run_item(mps.work_item(10, 111, P0, P1, 0, 10_000_000, 6, 4), chunk_ints=1 << 21)
+    finally:
+        gpu.configure(-1, 0)
+    want, _ = oracle.process_range(oracle.make_item(10, 111, P0, P1, 0, 10_000_000, 6, 4), P0, P1, 1 << 20, 8)
+    assert rels == want
+
+
+def test_filter_path_matches_oracle(gpu, mps, oracle):
+    """GpuSiever::filter (Gpu.hpp:726-798): CPU-supplied primes, flagged indices."""
+    P0, P1 = 10**9, 10**9 + 2**21
+    item = mps.work_item(2, -25, P0, P1, 3_600_000, 5_000_000, 2, 1)
+    primes = oracle.sieve_primes(P0, P1)
+    todo = []
+    assert gpu.filter(primes, item, todo)
+    assert todo == sorted(todo)
+    rels = []
+    for i in todo:
+        r = mps.run_bsgs(int(primes[i]), item)
+        if r:
+            rels.append(r)
+    want, _ = oracle.process_range(oracle.make_item(2, -25, P0, P1, 3_600_000, 5_000_000, 2, 1), P0, P1, 1 << 20, 8)
+    assert rels == want
