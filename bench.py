@@ -126,6 +126,9 @@ def main():
                    "stage_ms_per_step": {k: s[k] / args.steps for k in ("ms_sieve", "ms_filter", "ms_setup",
                                                                           "ms_bsgs")},
                    "entries_per_step": s["entries"] // args.steps,
+                   "survivors_per_step": s["survivors"] // args.steps,
+                   "candidates_per_step": s["candidates"] // args.steps,
+                   "products_per_step": {k: s[k] // args.steps for k in ("w_fermat", "w_setup", "w_bsgs")},
                    "flagged_per_step": s["flagged"] // args.steps},
         "roofline": roof,
     }

@@ -56,6 +56,22 @@ void mps_gpu_set_classes(mps_gpu* g, int on);
 int mps_gpu_sieve_filter(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, uint64_t* out,
                          size_t out_cap, size_t* out_n);
 
+/* Exact device relations of [lo|1, hi): one record per prime p of the range for which some
+ * in-class n in [N0, N1) has p | b^n + c, with M = the smallest such n (the relation of
+ * run_bsgs, Math.hpp:377-486, without q = ord_p(b)).  Primality is decided on the device by
+ * the deterministic 12-base Miller-Rabin test.  M == MPS_HOST_DECIDES marks a number the device
+ * leaves to the caller's exact check (p < 1000, p | b, an order of b^cm below the baby-step
+ * count, or no room for its BSGS entry).  This extends the chunk interface (no reference
+ * counterpart); sieve_filter is the same pass returning the numbers only. */
+#define MPS_HOST_DECIDES 0xFFFFFFFFFFFFFFFFull
+typedef struct mps_dev_relation {
+    uint64_t p;
+    uint64_t M;
+} mps_dev_relation;
+
+int mps_gpu_sieve_relations(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, mps_dev_relation* out,
+                            size_t out_cap, size_t* out_n);
+
 /* GpuSiever::filter (Gpu.hpp:1058-1061, OpenClSiever::filter Gpu.hpp:726-798): the primes are
  * given by the caller.  Writes the ascending indices i with primes[i] possibly dividing an
  * in-class b^n + c (or left to the host) to todo[]. */
@@ -85,7 +101,7 @@ typedef struct mps_gpu_stats {
     double ms_sieve, ms_filter, ms_setup, ms_bsgs;  /* summed kernel durations */
     uint64_t n_bsgs_launches;
     int field;              /* field of the last pass: 0 = Montgomery, 1 = FP64 */
-    int tbits;              /* log2 of the BSGS table size of the last pass */
+    int tbits;              /* log2 of the BSGS table's bucket count in the last pass */
 } mps_gpu_stats;
 
 int mps_gpu_get_stats(const mps_gpu* g, mps_gpu_stats* s);
@@ -93,8 +109,9 @@ void mps_gpu_reset_stats(mps_gpu* g);
 
 /* Tuning and test hooks (no counterpart in the reference).
  * field: -1 = automatic (FP64 when hi <= 2^50, else Montgomery), 0 = Montgomery, 1 = FP64.
- * pass_odds: odd numbers per device pass (buffers are sized from it; 0 keeps the current). */
-int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds);
+ * pass_odds: odd numbers per device pass (buffers are sized from it; 0 keeps the current).
+ * mcap: largest baby-step count without classes, 64..4096 (0 keeps the current; default 1024). */
+int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap);
 
 /* Stage outputs of one pass over [lo|1, hi) for stage-level parity tests:
  *   stage 0: survivors (values), stage 1: candidates (value, info) pairs,

@@ -38,17 +38,59 @@ void write_str(char* dst, size_t cap, const std::string& s) {
     dst[n] = 0;
 }
 
+uint64_t ceil_div_u(uint64_t a, uint64_t b) { return (a + b - 1) / b; }
+
+// Plain mode: the number of baby steps m (a multiple of the workgroup's lanes, at most mcap) and
+// giant steps ng = ceil(jc / m) + 1 that need the fewest workgroup rounds.
+void plain_size(uint64_t jc, uint32_t mcap, uint32_t L, uint32_t& m, uint32_t& ng) {
+    uint64_t best_r = ~0ull, best_steps = ~0ull;
+    for (uint64_t k = 1;; ++k) {
+        const uint64_t mm = std::min<uint64_t>(k * L, mcap);
+        const uint64_t g = ceil_div_u(jc, mm) + 1;
+        const uint64_t r = ceil_div_u(mm, L) + ceil_div_u(g, L);
+        if (r < best_r || (r == best_r && mm + g < best_steps)) {
+            best_r = r;
+            best_steps = mm + g;
+            m = (uint32_t)mm;
+            ng = (uint32_t)g;
+        }
+        if (mm >= mcap || mm >= jc) break;
+    }
+}
+
+// Class mode: Gc = L * cr giant steps per class and m <= maxm baby steps with
+// (Gc - 1) * m >= K - 1 (every k in [0, K) is g*m - i for some g < Gc, i < m), chosen for the
+// fewest workgroup rounds ceil(m / L) + s * cr.
+void class_size(uint64_t K, uint32_t s, uint32_t L, uint32_t maxm, uint32_t& m, uint32_t& gc) {
+    uint64_t best = ~0ull, best_steps = ~0ull;
+    for (uint64_t cr = 1;; ++cr) {
+        if (best != ~0ull && s * cr > best) break;
+        const uint64_t G = (uint64_t)L * cr;
+        const uint64_t mm = K <= 1 ? 1 : std::max<uint64_t>(1, ceil_div_u(K - 1, G - 1));
+        if (mm > maxm) continue;
+        const uint64_t r = ceil_div_u(mm, L) + s * cr;
+        if (r < best || (r == best && mm + s * G < best_steps)) {
+            best = r;
+            best_steps = mm + s * G;
+            m = (uint32_t)mm;
+            gc = (uint32_t)G;
+        }
+    }
+}
+
 }  // namespace
 
 namespace mps {
 
-bool item_params(const mps_item* it, bool classes_on, ItemParams& ip, uint32_t& tbits, std::string& why) {
+bool item_params(const mps_item* it, bool classes_on, uint32_t mcap, ItemParams& ip, uint64_t& m_max,
+                 std::vector<uint2>& csz, std::string& why) {
     std::memset(&ip, 0, sizeof(ip));
     ip.b = it->b;
     ip.c = it->c;
     const uint64_t N0 = it->N0 < 0 ? 0 : (uint64_t)it->N0;
     const uint64_t N1 = it->N1 < 0 ? 0 : (uint64_t)it->N1;
     const uint64_t cm = it->cls_mod == 0 ? 1 : it->cls_mod;
+    const uint32_t L = 64 * kBsgsWaves;
     ip.N0 = N0;
     ip.N1 = N1;
     ip.cm = cm;
@@ -56,6 +98,8 @@ bool item_params(const mps_item* it, bool classes_on, ItemParams& ip, uint32_t& 
     ip.n_start = N0 + ((it->cls_res % cm) + cm - (N0 % cm)) % cm;
     ip.kcount = ip.n_start >= N1 ? 0 : (N1 - ip.n_start + cm - 1) / cm;
     ip.cm_r32 = cm < 65536 ? (uint32_t)((1ull << 32) % cm) : 0;
+    ip.mcap = mcap;
+    ip.lanes = L;
     // gpu_classes (Math.hpp:324-337): only for an even cm with lcm(cm, 24) <= 48 and a long range
     bool qmode = false;
     if (classes_on && it->N0 >= 0 && it->N1 > it->N0) {
@@ -69,18 +113,38 @@ bool item_params(const mps_item* it, bool classes_on, ItemParams& ip, uint32_t& 
         }
     }
     ip.qmode = qmode ? 1 : 0;
-    // LDS table: 2^tbits 32-bit words at <= 80% load for the largest m of the item
-    uint64_t m;
+    m_max = 1;
+    csz.assign(49 * 32, make_uint2(1, L));
     if (qmode) {
-        m = kClassMaxM;
-    } else {
+        const uint64_t span = N1 - N0;
+        for (uint32_t qp = 1; qp <= 48; ++qp) {
+            if (ip.Qmax % qp != 0 || qp % cm != 0) continue;
+            const uint64_t K = ceil_div_u(span, qp);
+            for (uint32_t sc = 1; sc <= qp / cm; ++sc) {
+                uint32_t m = 1, gc = L;
+                class_size(K, sc, L, kClassMaxM, m, gc);
+                csz[qp * 32 + sc] = make_uint2(m, gc);
+                m_max = std::max<uint64_t>(m_max, m);
+            }
+        }
+    } else if (ip.kcount > 0) {
+        for (uint32_t combo = 0; combo < 4; ++combo) {
+            const uint64_t stride = combo / 2 + 1, parity = combo & 1;
+            if (ip.kcount <= parity) continue;
+            const uint64_t jc = (ip.kcount - parity + stride - 1) / stride;
+            uint32_t m = 1, ng = 2;
+            plain_size(jc, mcap, L, m, ng);
+            ip.pm[combo] = m;
+            ip.png[combo] = ng;
+            ip.pjc[combo] = jc;
+            m_max = std::max<uint64_t>(m_max, m);
+        }
+        // primes below the exponent count cap it at p (run_bsgs, Math.hpp:398-400): m <= sqrt bound
         const uint64_t kmax = (N1 > N0 ? (N1 - N0) / cm : 0) + 2;
-        m = (uint64_t)std::sqrt((double)kmax) + 2;
+        m_max = std::max<uint64_t>(m_max, std::min<uint64_t>((uint64_t)std::sqrt((double)kmax) + 2, mcap));
     }
-    tbits = 4;
-    while ((1ull << tbits) < m + m / 4) ++tbits;
-    if (tbits > 14) {
-        why = "BSGS table would exceed 2^14 entries";
+    if (m_max > 4096) {
+        why = "BSGS table would exceed the LDS budget";
         return false;
     }
     return true;
@@ -95,7 +159,10 @@ struct mps_gpu {
     bool broken = false;
     int field_mode = -1;
     uint64_t pass_odds = 1ull << 25;
+    uint32_t mcap = 2048;
     std::mutex mu;
+    uint2* d_csz = nullptr;
+    uint2* h_csz = nullptr;
     hipStream_t stream = nullptr;
     uint32_t* d_sp = nullptr;
     uint32_t* d_sp_r32 = nullptr;
@@ -145,12 +212,12 @@ bool ensure_buffers(mps_gpu* g, uint64_t odds) {
     if (g->buf_odds >= odds) return true;
     free_buffers(g);
     const uint64_t ent_cap = std::max<uint64_t>(odds / 8, 4096);
-    const uint64_t out_cap = std::max<uint64_t>(odds / 16, 4096);
+    const uint64_t out_cap = std::max<uint64_t>(odds / 16, 4096);  // records of 2 words
     HIPCHK(hipMalloc(&g->pb.surv, odds * sizeof(uint32_t)), "surv alloc");
     HIPCHK(hipMalloc(&g->pb.cand, odds * sizeof(uint64_t)), "cand alloc");
     HIPCHK(hipMalloc(&g->pb.ent, ent_cap * kEntWords * sizeof(uint64_t)), "entry alloc");
-    HIPCHK(hipMalloc(&g->pb.out, out_cap * sizeof(uint64_t)), "out alloc");
-    HIPCHK(hipHostMalloc((void**)&g->h_out, out_cap * sizeof(uint64_t), hipHostMallocDefault), "pinned out");
+    HIPCHK(hipMalloc(&g->pb.out, 2 * out_cap * sizeof(uint64_t)), "out alloc");
+    HIPCHK(hipHostMalloc((void**)&g->h_out, 2 * out_cap * sizeof(uint64_t), hipHostMallocDefault), "pinned out");
     g->pb.surv_cap = (uint32_t)odds;
     g->pb.cand_cap = (uint32_t)odds;
     g->pb.ent_cap = (uint32_t)ent_cap;
@@ -178,9 +245,10 @@ uint32_t blocks_for(uint64_t n, uint32_t per_block) {
 
 // One pass through the stage chain.  stop_stage: 0 after the sieve, 1 after the filter,
 // 2 after setup, 3 = full.  Leaves the final counts in g->h_cnt.
-bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, FieldKind fk, uint32_t tbits,
+bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, FieldKind fk, const TableGeom& tg,
               int stop_stage) {
     hipStream_t s = g->stream;
+    HIPCHK(hipMemcpyAsync(g->d_csz, g->h_csz, 49 * 32 * sizeof(uint2), hipMemcpyHostToDevice, s), "size table");
     launch_reset(s, g->pb.cnt, g->pb.vals == nullptr ? 0u : (uint32_t)pc);
     HIPCHK(hipEventRecord(g->ev[0], s), "event");
     if (g->pb.vals == nullptr) launch_sieve(s, plo, pc, g->d_sp, g->d_sp_r32, g->nsp, g->nsmall, g->pb, 0);
@@ -210,7 +278,7 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (stop_stage == 2 || n_ent == 0) return true;
 
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
-    launch_bsgs(s, fk, tbits, g->pb, n_ent, 0);
+    launch_bsgs(s, fk, tg, ip, g->pb, n_ent, 0);
     HIPCHK(hipGetLastError(), "bsgs launch");
     HIPCHK(hipEventRecord(g->ev[4], s), "event");
     if (!read_counts(g)) return false;
@@ -232,6 +300,15 @@ void accumulate(mps_gpu* g, uint64_t pc) {
     g->st.w_fermat += c.w_fermat;
     g->st.w_setup += c.w_setup;
     g->st.w_bsgs += c.w_bsgs;
+}
+
+// Item constants plus the class-size table, staged in pinned memory for run_pass to upload.
+bool prepare_item(mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max) {
+    std::vector<uint2> csz;
+    if (!item_params(item, classes_on, g->mcap, ip, m_max, csz, g->err)) return false;
+    std::memcpy(g->h_csz, csz.data(), csz.size() * sizeof(uint2));
+    ip.csz = g->d_csz;
+    return true;
 }
 
 FieldKind pick_field(const mps_gpu* g, uint64_t hi) {
@@ -284,6 +361,8 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
          hipMemcpy(g->d_sp, sp.data(), sp.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(g->d_sp_r32, r32.data(), r32.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMalloc(&g->pb.cnt, sizeof(Counters)) == hipSuccess &&
+         hipMalloc(&g->d_csz, 49 * 32 * sizeof(uint2)) == hipSuccess &&
+         hipHostMalloc((void**)&g->h_csz, 49 * 32 * sizeof(uint2), hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc((void**)&g->h_cnt, sizeof(Counters), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
         write_str(banner, banner_cap, "GPU disabled: HIP setup failed");
@@ -304,10 +383,14 @@ void mps_gpu_set_classes(mps_gpu* g, int on) {
     if (g) g->classes_on = on != 0;
 }
 
-int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds) {
+int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap) {
     if (g == nullptr) return MPS_ERROR;
     std::lock_guard<std::mutex> lock(g->mu);
     g->field_mode = field;
+    if (mcap) {
+        if (mcap < 256 || mcap > 4096) return MPS_ERROR;
+        g->mcap = mcap;
+    }
     if (pass_odds) {
         if (pass_odds < 4096 || pass_odds > (1ull << 31)) return MPS_ERROR;
         g->pass_odds = pass_odds;
@@ -315,13 +398,14 @@ int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds) {
     return MPS_OK;
 }
 
-int mps_gpu_sieve_filter(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, uint64_t* out,
-                         size_t out_cap, size_t* out_n) {
-    if (out_n) *out_n = 0;
-    if (g == nullptr || item == nullptr || out_n == nullptr) return MPS_ERROR;
-    std::lock_guard<std::mutex> lock(g->mu);
-    if (!g->ok) return MPS_ERROR;
-    if (g->broken) return MPS_ERROR;
+}  // extern "C"
+
+namespace {
+
+// Device records (number, M) of [lo|1, hi): the plain per-chunk pipeline, pass by pass.
+int sieve_records(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, std::vector<uint64_t>& rec) {
+    rec.clear();
+    if (!g->ok || g->broken) return MPS_ERROR;
     if (hi >= (1ull << 62) || lo <= 3) {
         g->err = "range outside (3, 2^62)";
         return MPS_DECLINED;
@@ -329,40 +413,72 @@ int mps_gpu_sieve_filter(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* i
     lo |= 1;
     if (lo >= hi) return MPS_OK;
     ItemParams ip;
-    uint32_t tbits = 0;
-    if (!item_params(item, g->classes_on, ip, tbits, g->err)) return MPS_DECLINED;
+    uint64_t m_max = 0;
+    if (!prepare_item(g, item, g->classes_on, ip, m_max)) return MPS_DECLINED;
     const FieldKind fk = pick_field(g, hi);
+    const TableGeom tg = table_geom(m_max, lo);
     const uint64_t count = (hi - lo + 1) / 2;
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
     if (!ensure_buffers(g, std::min(count, g->pass_odds))) return MPS_ERROR;
     g->pb.vals = nullptr;
     g->pb.need_prime = 1;
     g->pb.out_index = 0;
-    size_t n = 0;
     for (uint64_t pb0 = 0; pb0 < count; pb0 += g->pass_odds) {
         const uint64_t pc = std::min(g->pass_odds, count - pb0);
         const uint64_t plo = lo + 2 * pb0;
-        if (!run_pass(g, plo, pc, ip, fk, tbits, 3)) return MPS_ERROR;
+        if (!run_pass(g, plo, pc, ip, fk, tg, 3)) return MPS_ERROR;
         accumulate(g, pc);
         const uint32_t n_out = g->h_cnt->n_out;
         if (n_out > g->pb.out_cap) {
-            g->err = "flagged numbers exceed the device buffer";
+            g->err = "device records exceed the output buffer";
             return MPS_DECLINED;
         }
         if (n_out) {
-            if (hipMemcpyAsync(g->h_out, g->pb.out, n_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+            if (hipMemcpyAsync(g->h_out, g->pb.out, 2 * n_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream) !=
+                    hipSuccess ||
                 hipStreamSynchronize(g->stream) != hipSuccess) {
-                hip_fail(g, hipErrorUnknown, "flag read");
+                hip_fail(g, hipErrorUnknown, "record read");
                 return MPS_ERROR;
             }
-            for (uint32_t i = 0; i < n_out; ++i) {
-                if (n < out_cap && out) out[n] = g->h_out[i];
-                ++n;
-            }
+            rec.insert(rec.end(), g->h_out, g->h_out + 2 * n_out);
         }
     }
     g->st.field = (int)fk;
-    g->st.tbits = (int)tbits;
+    g->st.tbits = (int)tg.nbbits;
+    return MPS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mps_gpu_sieve_filter(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, uint64_t* out,
+                         size_t out_cap, size_t* out_n) {
+    if (out_n) *out_n = 0;
+    if (g == nullptr || item == nullptr || out_n == nullptr) return MPS_ERROR;
+    std::lock_guard<std::mutex> lock(g->mu);
+    std::vector<uint64_t> rec;
+    const int st = sieve_records(g, lo, hi, item, rec);
+    if (st != MPS_OK) return st;
+    const size_t n = rec.size() / 2;
+    for (size_t i = 0; i < n && i < out_cap && out; ++i) out[i] = rec[2 * i];
+    *out_n = n;
+    return n > out_cap ? MPS_CAPACITY : MPS_OK;
+}
+
+int mps_gpu_sieve_relations(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, mps_dev_relation* out,
+                            size_t out_cap, size_t* out_n) {
+    if (out_n) *out_n = 0;
+    if (g == nullptr || item == nullptr || out_n == nullptr) return MPS_ERROR;
+    std::lock_guard<std::mutex> lock(g->mu);
+    std::vector<uint64_t> rec;
+    const int st = sieve_records(g, lo, hi, item, rec);
+    if (st != MPS_OK) return st;
+    const size_t n = rec.size() / 2;
+    for (size_t i = 0; i < n && i < out_cap && out; ++i) {
+        out[i].p = rec[2 * i];
+        out[i].M = rec[2 * i + 1];
+    }
     *out_n = n;
     return n > out_cap ? MPS_CAPACITY : MPS_OK;
 }
@@ -375,15 +491,19 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
     if (!g->ok || g->broken) return MPS_ERROR;
     if (n == 0) return MPS_OK;
     ItemParams ip;
-    uint32_t tbits = 0;
-    if (!item_params(item, false, ip, tbits, g->err)) return MPS_DECLINED;  // filter(): plain prep path
-    uint64_t pmax = 0;
-    for (size_t i = 0; i < n; ++i) pmax = std::max(pmax, primes[i]);
-    const FieldKind fk = pick_field(g, pmax + 1);
+    uint64_t m_max = 0;
+    if (!prepare_item(g, item, false, ip, m_max)) return MPS_DECLINED;  // filter(): plain prep path
+    uint64_t pmax = 0, pmin = ~0ull;
+    for (size_t i = 0; i < n; ++i) {
+        pmax = std::max(pmax, primes[i]);
+        pmin = std::min(pmin, primes[i]);
+    }
     if (pmax >= (1ull << 62)) {
         g->err = "prime above 2^62";
         return MPS_DECLINED;
     }
+    const FieldKind fk = pick_field(g, pmax + 1);
+    const TableGeom tg = table_geom(m_max, std::max<uint64_t>(pmin, 3));
     const size_t batch = (size_t)std::min<uint64_t>(g->pass_odds, 1ull << 22);
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
     if (!ensure_buffers(g, batch)) return MPS_ERROR;
@@ -406,7 +526,7 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
         g->pb.vals = g->d_vals;
         g->pb.need_prime = 0;
         g->pb.out_index = 1;
-        const bool ok = run_pass(g, 0, cnt, ip, fk, tbits, 3);
+        const bool ok = run_pass(g, 0, cnt, ip, fk, tg, 3);
         g->pb.vals = nullptr;
         g->pb.need_prime = 1;
         g->pb.out_index = 0;
@@ -414,16 +534,17 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
         accumulate(g, 0);
         const uint32_t n_out = g->h_cnt->n_out;
         if (n_out > g->pb.out_cap) {
-            g->err = "flagged numbers exceed the device buffer";
+            g->err = "device records exceed the output buffer";
             return MPS_DECLINED;
         }
         if (n_out) {
-            if (hipMemcpyAsync(g->h_out, g->pb.out, n_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+            if (hipMemcpyAsync(g->h_out, g->pb.out, 2 * n_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream) !=
+                    hipSuccess ||
                 hipStreamSynchronize(g->stream) != hipSuccess) {
-                hip_fail(g, hipErrorUnknown, "flag read");
+                hip_fail(g, hipErrorUnknown, "record read");
                 return MPS_ERROR;
             }
-            for (uint32_t i = 0; i < n_out; ++i) idx.push_back(off + g->h_out[i]);
+            for (uint32_t i = 0; i < n_out; ++i) idx.push_back(off + g->h_out[2 * i]);
         }
     }
     std::sort(idx.begin(), idx.end());
@@ -442,9 +563,10 @@ int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const m
     lo |= 1;
     if (lo <= 3 || hi >= (1ull << 62) || lo >= hi) return MPS_DECLINED;
     ItemParams ip;
-    uint32_t tbits = 0;
-    if (!item_params(item, g->classes_on, ip, tbits, g->err)) return MPS_DECLINED;
+    uint64_t m_max = 0;
+    if (!prepare_item(g, item, g->classes_on, ip, m_max)) return MPS_DECLINED;
     const FieldKind fk = pick_field(g, hi);
+    const TableGeom tg = table_geom(m_max, lo);
     const uint64_t count = (hi - lo + 1) / 2;
     if (count > g->pass_odds) {
         g->err = "debug stage covers one pass only";
@@ -454,7 +576,7 @@ int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const m
     g->pb.vals = nullptr;
     g->pb.need_prime = 1;
     g->pb.out_index = 0;
-    if (!run_pass(g, lo, count, ip, fk, tbits, stage)) return MPS_ERROR;
+    if (!run_pass(g, lo, count, ip, fk, tg, stage)) return MPS_ERROR;
     const Counters c = *g->h_cnt;
     size_t words = 0, recs = 0;
     if (stage == 0) {
@@ -519,6 +641,8 @@ void mps_gpu_destroy(mps_gpu* g) {
     hipFree(g->d_sp_r32);
     hipFree(g->pb.cnt);
     hipHostFree(g->h_cnt);
+    hipFree(g->d_csz);
+    hipHostFree(g->h_csz);
     for (int i = 0; i < 6; ++i)
         if (g->ev[i]) hipEventDestroy(g->ev[i]);
     if (g->stream) hipStreamDestroy(g->stream);

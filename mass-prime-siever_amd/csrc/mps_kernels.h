@@ -10,13 +10,14 @@ namespace mps {
 constexpr uint32_t kSegOdds = 1u << 17;           // 16 KiB bitmap
 constexpr uint32_t kSieveThreads = 256;
 constexpr uint32_t kSmallPrimeBound = 1024;       // primes below this are shared by all lanes
-constexpr uint32_t kEntWords = 64;                // u64 words per BSGS entry (512 B)
+constexpr uint32_t kEntWords = 72;                // u64 words per BSGS entry (576 B)
 constexpr uint32_t kMaxClasses = 24;              // classes per prime in class mode
 constexpr uint32_t kClassMaxM = 1024;             // largest baby-step count in class mode
+constexpr uint64_t kHostDecides = ~0ull;          // M of a record the host must decide exactly
 
 // Entry layout (u64 words).  B is the baby-step base, Bm = B^m the giant-step base.
 enum EntryWord : uint32_t {
-    E_P = 0,      // prime (or Fermat pseudoprime)
+    E_P = 0,      // prime (or base-2 Fermat pseudoprime)
     E_AUX = 1,    // Montgomery -p^-1 mod 2^64, or the bits of 1/p (FP64 field)
     E_ONE = 2,    // Montgomery form of 1 (unused by the FP64 field)
     E_T = 3,      // target t = -c (field form): baby steps are t * B^i
@@ -28,7 +29,11 @@ enum EntryWord : uint32_t {
     E_BHI = 16,   // B^(8k), k = 0..7
     E_MLO = 24,   // Bm^k, k = 0..7
     E_MHI = 32,   // Bm^(8k), k = 0..7
-    E_G = 40,     // giant start per class G_j = b^(n_j), j < s
+    E_NB = 40,    // plain: n_start; class mode: N0
+    E_META = 41,  // qp (bits 0..7) | stride 2 (bit 8) | odd k (bit 9) | class mode (bit 10)
+    E_KLIM = 42,  // plain: jcount (solutions k = g*m - i must lie in [0, jcount))
+    E_DJ = 43,    // class mode: dj of the kept classes, 8 bits each, 3 words (n_j = N0 + dj)
+    E_G = 48,     // giant start per class G_j = b^(n_j), j < s
 };
 
 struct Counters {
@@ -52,6 +57,13 @@ struct ItemParams {
     uint32_t qmode;         // 1 = per-prime classes
     uint32_t cm_r32;        // 2^32 mod cm when cm < 2^16
     uint32_t Q_r32;         // 2^32 mod Qmax
+    uint32_t mcap;          // plain mode: largest baby-step count (bounds the LDS table)
+    uint32_t lanes;         // lanes of one BSGS workgroup
+    // plain mode sizes by (stride - 1) * 2 + parity, valid when kcount <= p
+    uint32_t pm[4], png[4];
+    uint64_t pjc[4];
+    // class mode sizes: csz[qp * 32 + s] = (m, giant steps per class)
+    const uint2* csz;
 };
 
 struct PassBuffers {
@@ -59,7 +71,7 @@ struct PassBuffers {
     uint32_t* surv;         // survivor odd indices relative to the pass start
     uint64_t* cand;         // candidate: index (low 32) | info (high 32)
     uint64_t* ent;          // entries, kEntWords u64 each
-    uint64_t* out;          // flagged numbers
+    uint64_t* out;          // records (number, M): M = smallest solution n, or kHostDecides
     Counters* cnt;
     uint32_t surv_cap, cand_cap, ent_cap, out_cap;
     uint32_t need_prime;    // 1: base-2 Fermat test first (sieve survivors may be composite)
@@ -74,7 +86,16 @@ void launch_sieve(hipStream_t s, uint64_t lo, uint64_t nodd, const uint32_t* sp,
 void launch_filter(hipStream_t s, uint64_t lo, const ItemParams& ip, const PassBuffers& pb, uint32_t grid);
 void launch_setup(hipStream_t s, FieldKind fk, uint64_t lo, const ItemParams& ip, const PassBuffers& pb,
                   uint32_t grid);
-void launch_bsgs(hipStream_t s, FieldKind fk, uint32_t tbits, const PassBuffers& pb, uint32_t n_ent,
-                 uint32_t grid);
+// BSGS table geometry of a pass: 2^nbbits buckets of 8 slots; a slot holds a fingerprint of
+// fpbits bits over a baby-step index of ibits bits.
+struct TableGeom {
+    uint32_t nbbits, fpbits, ibits, waves;
+};
+TableGeom table_geom(uint64_t m_max, uint64_t p_min);
+constexpr uint32_t kBsgsWaves = 4;                // wavefronts of one BSGS workgroup (one entry)
+size_t bsgs_lds_bytes(const TableGeom& tg);
+
+void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const ItemParams& ip, const PassBuffers& pb,
+                 uint32_t n_ent, uint32_t grid);
 
 }  // namespace mps

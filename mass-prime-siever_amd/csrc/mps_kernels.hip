@@ -156,6 +156,13 @@ __device__ __forceinline__ uint64_t gcd64(uint64_t a, uint64_t b) {
     return a;
 }
 
+__device__ __forceinline__ void put_record(const PassBuffers& pb, uint32_t slot, uint64_t num, uint64_t M) {
+    if (slot < pb.out_cap) {
+        pb.out[2 * (uint64_t)slot] = num;
+        pb.out[2 * (uint64_t)slot + 1] = M;
+    }
+}
+
 // Plain mode (prep_core decision, Gpu.hpp:231-263).  info: bit0 = stride 2, bit1 = odd k.
 __device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, bool& keep, bool& flag2,
                                              uint32_t& info) {
@@ -181,20 +188,21 @@ __device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, b
     }
 }
 
+// Per-prime class split (prep_q_core, Gpu.hpp:491-498): g = gcd(Qmax, p-1), qp = lcm(cm, g),
+// ncls = qp / cm classes; class j starts at N0 + dj(j), dj(j) = (cres + j*cm - N0) mod qp.
 struct ClassSplit {
-    uint32_t g, qp, ncls;
-    uint32_t dj[kMaxClasses];
+    uint32_t g, qp, ncls, c0, cm, n0q;
+    __device__ __forceinline__ uint32_t dj(uint32_t j) const { return (c0 + j * cm + qp - n0q) % qp; }
 };
 
 __device__ __forceinline__ void class_split(uint64_t p, const ItemParams& ip, ClassSplit& cs) {
     const uint32_t Q = (uint32_t)ip.Qmax;
-    const uint32_t cm = (uint32_t)ip.cm;
+    cs.cm = (uint32_t)ip.cm;
     cs.g = gcd32(Q, mod_small(p - 1, Q, ip.Q_r32));
-    cs.qp = cs.g / gcd32(cs.g, cm) * cm;
-    cs.ncls = cs.qp / cm;
-    const uint32_t n0q = ip.n0modQ % cs.qp;
-    const uint32_t c0 = (uint32_t)(ip.cres % cm);
-    for (uint32_t j = 0; j < cs.ncls; ++j) cs.dj[j] = (c0 + j * cm + cs.qp - n0q) % cs.qp;
+    cs.qp = cs.g / gcd32(cs.g, cs.cm) * cs.cm;
+    cs.ncls = cs.qp / cs.cm;
+    cs.n0q = ip.n0modQ % cs.qp;
+    cs.c0 = (uint32_t)(ip.cres % ip.cm);
 }
 
 // Class mode (prep_q_core decision, Gpu.hpp:478-509).  info = class mask.
@@ -211,8 +219,9 @@ __device__ __forceinline__ void filter_class(uint64_t p, const ItemParams& ip, b
     const uint32_t n0odd = (uint32_t)(ip.N0 & 1);
     uint32_t mask = 0;
     for (uint32_t j = 0; j < cs.ncls; ++j) {
-        if ((uint64_t)cs.dj[j] >= span) continue;
-        const int l = jt * (((n0odd ^ (cs.dj[j] & 1u)) != 0) ? jb : 1);
+        const uint32_t d = cs.dj(j);
+        if ((uint64_t)d >= span) continue;
+        const int l = jt * (((n0odd ^ (d & 1u)) != 0) ? jb : 1);
         if (l == 1) mask |= 1u << j;
     }
     keep = mask != 0;
@@ -240,7 +249,7 @@ __global__ __launch_bounds__(256) void k_filter(uint64_t lo, ItemParams ip, Pass
         const uint32_t slot = wave_append(keep, &pb.cnt->n_cand);
         if (keep && slot < pb.cand_cap) pb.cand[slot] = (uint64_t)idx | ((uint64_t)info << 32);
         const uint32_t oslot = wave_append(flag2, &pb.cnt->n_out);
-        if (flag2 && oslot < pb.out_cap) pb.out[oslot] = pb.out_index ? idx : p;
+        if (flag2) put_record(pb, oslot, pb.out_index ? idx : p, kHostDecides);
         const uint64_t fb = __ballot(flag2);
         if (lane == 0 && fb != 0) atomicAdd(&pb.cnt->n_flag2, (uint32_t)__popcll(fb));
     }
@@ -254,41 +263,6 @@ __device__ __forceinline__ uint64_t isqrt_ceil64(uint64_t x) {
     while (r > 0 && r * r > x) --r;
     while ((r + 1) * (r + 1) <= x) ++r;
     return r * r < x ? r + 1 : r;
-}
-
-// Class mode sizes: Gc = 64*cr giant steps per class and m <= kClassMaxM baby steps with
-// (Gc - 1) * m >= K - 1, chosen to minimise the wavefront rounds ceil(m/64) + s*cr.
-__device__ __forceinline__ void class_sizes(uint64_t K, uint32_t s, uint64_t& m, uint64_t& Gc) {
-    uint64_t crmin = (K + 64ull * (kClassMaxM - 1) - 1) / (64ull * (kClassMaxM - 1));
-    if (crmin < 2) crmin = 2;
-    const uint64_t crc = isqrt_ceil64(K / (4096ull * s));
-    uint64_t best = ~0ull;
-    m = kClassMaxM;
-    Gc = 64 * crmin;
-    const uint64_t c0 = crc > 2 ? crc - 2 : 1;
-    for (uint64_t cr = c0; cr <= crc + 3; ++cr) {
-        const uint64_t cx = cr < crmin ? crmin : cr;
-        const uint64_t gc = 64 * cx;
-        const uint64_t mm = (K + gc - 3) / (gc - 1) + 1;
-        if (mm > kClassMaxM) continue;
-        const uint64_t rounds = (mm + 63) / 64 + (uint64_t)s * cx;
-        if (rounds < best) {
-            best = rounds;
-            m = mm;
-            Gc = gc;
-        }
-    }
-    if (best == ~0ull) {
-        for (uint64_t cx = crmin;; ++cx) {
-            const uint64_t gc = 64 * cx;
-            const uint64_t mm = (K + gc - 3) / (gc - 1) + 1;
-            if (mm <= kClassMaxM) {
-                m = mm;
-                Gc = gc;
-                break;
-            }
-        }
-    }
 }
 
 // Writes the powers tables of base x: tab[k] = x^k and tab[8 + k] = x^(8k), k < 8; returns x^64.
@@ -311,6 +285,8 @@ __device__ __forceinline__ typename F::T write_pow_tables(const F& f, typename F
     return c;  // x^64
 }
 
+__device__ __forceinline__ uint32_t bitlen64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
 template <class F, bool QMODE>
 __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassBuffers pb) {
     typedef typename F::T T;
@@ -323,7 +299,7 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
     for (uint32_t i0 = wglobal * 64; i0 < n; i0 += nwaves * 64) {
         const uint32_t i = i0 + lane;
         bool keep = false;
-        uint64_t p = 0, m = 0, ng = 0;
+        uint64_t p = 0, m = 0, ng = 0, jc = 0;
         uint32_t idx = 0, info = 0, s = 1, mask = 0, dmin = 0;
         ClassSplit cs;
         F f;
@@ -333,7 +309,7 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
             info = (uint32_t)(cd >> 32);
             p = pb.vals ? pb.vals[idx] : lo + 2 * (uint64_t)idx;
             f.init(p);
-            if (pb.need_prime) w_fermat += 64 - __builtin_clzll(p - 1);
+            if (pb.need_prime) w_fermat += bitlen64(p - 1);
             if (pb.need_prime && !fermat2(f, p)) {
                 ++fermat_fail;
             } else {
@@ -347,7 +323,7 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
                         // class j needs t^e == (b^e)^(n_j mod g), e = (p-1)/g
                         const uint64_t e = (p - 1) / cs.g;
                         const T u = fpow(f, tm, e), w = fpow(f, bm, e);
-                        w_setup += 2 * (64 - __builtin_clzll(e)) + cs.g;
+                        w_setup += 2 * bitlen64(e) + cs.g;
                         const uint64_t ku = f.key(u);
                         const uint32_t n0g = ip.n0modQ % cs.g;
                         uint32_t keepm = 0;
@@ -355,7 +331,7 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
                         for (uint32_t k = 0; k < cs.g; ++k) {
                             if (f.key(wk) == ku)
                                 for (uint32_t j = 0; j < cs.ncls; ++j)
-                                    if ((n0g + cs.dj[j]) % cs.g == k) keepm |= 1u << j;
+                                    if ((n0g + cs.dj(j)) % cs.g == k) keepm |= 1u << j;
                             wk = f.mul(wk, w);
                         }
                         mask &= keepm;
@@ -364,9 +340,10 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
                         s = __popc(mask);
                         dmin = cs.qp;
                         for (uint32_t j = 0; j < cs.ncls; ++j)
-                            if (((mask >> j) & 1u) && cs.dj[j] < dmin) dmin = cs.dj[j];
-                        const uint64_t K = (ip.N1 - ip.N0 + cs.qp - 1) / cs.qp;
-                        class_sizes(K, s, m, ng);
+                            if (((mask >> j) & 1u) && cs.dj(j) < dmin) dmin = cs.dj(j);
+                        const uint2 sz = ip.csz[cs.qp * 32 + s];
+                        m = sz.x;
+                        ng = sz.y;
                         keep = true;
                     }
                 } else {
@@ -377,15 +354,23 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
                         const uint64_t e = (p - 1) / g;
                         const T G0 = fpow(f, bm, ip.n_start);
                         ok = f.key(fpow(f, tm, e)) == f.key(fpow(f, G0, e));
-                        w_setup += 2 * (64 - __builtin_clzll(e)) + (64 - __builtin_clzll(ip.n_start | 1));
+                        w_setup += 2 * bitlen64(e) + bitlen64(ip.n_start);
                     }
                     const uint32_t stride = (info & 1u) ? 2 : 1, parity = (info >> 1) & 1u;
-                    uint64_t kc = ip.kcount < p ? ip.kcount : p;
+                    const uint64_t kc = ip.kcount < p ? ip.kcount : p;
                     if (ok && kc > parity) {
-                        const uint64_t jcount = (kc - parity + stride - 1) / stride;
-                        m = isqrt_ceil64(jcount);
-                        if (m == 0) m = 1;
-                        ng = (jcount + m - 1) / m + 1;  // g = 0 .. ceil(jcount/m)
+                        const uint32_t combo = (stride - 1) * 2 + parity;
+                        if (kc == ip.kcount) {
+                            jc = ip.pjc[combo];
+                            m = ip.pm[combo];
+                            ng = ip.png[combo];
+                        } else {  // kcount capped at p (p below the exponent count)
+                            jc = (kc - parity + stride - 1) / stride;
+                            m = isqrt_ceil64(jc);
+                            if (m > ip.mcap) m = ip.mcap;
+                            if (m == 0) m = 1;
+                            ng = (jc + m - 1) / m + 1;  // g = 0 .. ceil(jcount/m)
+                        }
                         keep = true;
                     }
                 }
@@ -394,7 +379,7 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
         const uint32_t slot = wave_append(keep, &pb.cnt->n_ent);
         if (keep && slot >= pb.ent_cap) {
             const uint32_t o = atomicAdd(&pb.cnt->n_out, 1u);  // no room: the host decides this one
-            if (o < pb.out_cap) pb.out[o] = pb.out_index ? idx : p;
+            put_record(pb, o, pb.out_index ? idx : p, kHostDecides);
             keep = false;
         }
         if (keep) {
@@ -411,28 +396,45 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
             E[E_SSRC] = (uint64_t)s | ((uint64_t)((QMODE ? mask : info) & 0xFFFFFFu) << 8) | ((uint64_t)idx << 32);
             if (QMODE) {
                 B = fpow(f, bm, cs.qp);
+                E[E_NB] = ip.N0;
+                E[E_META] = cs.qp | (1u << 10);
+                E[E_KLIM] = 0;
                 // G_j = b^(N0 + dj) = b^(N0 + dmin) * b^(dj - dmin)
                 const T base = fpow(f, bm, ip.N0 + dmin);
-                w_setup += 2 * (64 - __builtin_clzll(ip.N0 + dmin + 1)) + 12;
+                w_setup += 2 * bitlen64(ip.N0 + dmin) + 12;
                 uint32_t o = 0;
+                uint64_t djw[3] = {0, 0, 0};
                 for (uint32_t j = 0; j < cs.ncls; ++j) {
                     if (!((mask >> j) & 1u)) continue;
-                    E[E_G + o] = F::to_bits(f.mul(base, fpow(f, bm, cs.dj[j] - dmin)));
+                    const uint32_t d = cs.dj(j);
+                    E[E_G + o] = F::to_bits(f.mul(base, fpow(f, bm, d - dmin)));
+                    if (o < 8)
+                        djw[0] |= (uint64_t)d << (8 * o);
+                    else if (o < 16)
+                        djw[1] |= (uint64_t)d << (8 * (o - 8));
+                    else
+                        djw[2] |= (uint64_t)d << (8 * (o - 16));
                     w_setup += 12;
                     ++o;
                 }
+                E[E_DJ] = djw[0];
+                E[E_DJ + 1] = djw[1];
+                E[E_DJ + 2] = djw[2];
             } else {
                 B = fpow(f, bm, ip.cm);
                 T G0 = fpow(f, bm, ip.n_start);
-                w_setup += 2 * (64 - __builtin_clzll(ip.n_start | 1)) + 2 * (64 - __builtin_clzll(ip.cm));
+                w_setup += 2 * bitlen64(ip.n_start) + 2 * bitlen64(ip.cm);
                 if (info & 1u) {  // stride 2: k = 2j + parity
                     if (info & 2u) G0 = f.mul(G0, B);
                     B = f.mul(B, B);
                 }
                 E[E_G] = F::to_bits(G0);
+                E[E_NB] = ip.n_start;
+                E[E_META] = ((info & 1u) ? (1u << 8) : 0u) | ((info & 2u) ? (1u << 9) : 0u);
+                E[E_KLIM] = jc;
             }
             const T Bm = fpow(f, B, m);
-            w_setup += 2 * (64 - __builtin_clzll(m));
+            w_setup += 2 * bitlen64(m);
             E[E_B64] = F::to_bits(write_pow_tables(f, B, E + E_BLO, w_setup));
             E[E_BM64] = F::to_bits(write_pow_tables(f, Bm, E + E_MLO, w_setup));
             w_bsgs += m + (uint64_t)s * ng;
@@ -451,77 +453,275 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
 }
 
 // ------------------------------------------------------------------------------------- bsgs
+//
+// Table (LDS, one per entry): 2^nbbits buckets of 8 slots.  A slot is a 16-bit fingerprint in the
+// fingerprint array (one 16-byte bucket = one 128-bit LDS read) and a 16-bit baby index in a
+// parallel array read only on a fingerprint match; one u32 fill counter per bucket.  A baby step
+// takes the next slot of its home bucket from the counter (slots fill in order, so a bucket is full
+// exactly when slot 7 is written) and spills to the next bucket when full.  A lookup reads its
+// home bucket and continues only while buckets are full.  Every fingerprint match is verified
+// exactly (t * B^i against the giant value), so the device verdicts are exact.
+//
+// Hash of a canonical residue x (low word lo): fingerprint = lo & 0xFFFF (0xFFFF marks an empty
+// slot, so it maps to 0xFFFE), bucket = ((lo >> 16) ^ lo) & (2^nbbits - 1).
 
-template <class F, int W>
-__global__ __launch_bounds__(64 * W) void k_bsgs(PassBuffers pb, uint32_t tbits) {
-    typedef typename F::T T;
-    extern __shared__ uint32_t lds_tab[];
-    const uint32_t lane = lane_id();
-    const uint32_t wv = threadIdx.x >> 6;
-    const uint32_t tsize = 1u << tbits, tmask = tsize - 1u;
-    uint32_t* tab = lds_tab + wv * tsize;
-    const uint32_t n = min(pb.cnt->n_ent, pb.ent_cap);
-    for (uint32_t e0 = blockIdx.x * W + wv; e0 < n; e0 += gridDim.x * W) {
-        const uint32_t e = __builtin_amdgcn_readfirstlane(e0);
-        const uint64_t* __restrict__ E = pb.ent + (uint64_t)e * kEntWords;
-        const uint64_t p = E[E_P];
-        F f;
-        f.set(p, E[E_AUX], E[E_ONE]);
-        const uint64_t mg = E[E_MG];
-        const uint32_t m = (uint32_t)mg, ng = (uint32_t)(mg >> 32);
-        const uint64_t ssrc = E[E_SSRC];
-        const uint32_t s = (uint32_t)ssrc & 0xFFu;
+constexpr uint32_t kEmpty16 = 0xFFFFu;
 
-        uint4* t4 = reinterpret_cast<uint4*>(tab);
-        for (uint32_t i = lane; i < (tsize >> 2); i += 64) t4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+struct BsgsTable {
+    uint16_t* fp;    // [nbk * 8]
+    uint16_t* idx;   // [nbk * 8]
+    uint32_t* cnt;   // [nbk]
+    uint32_t nbmask;
+};
 
-        // baby steps t * B^i, i = lane, lane + 64, ...
-        const T B64 = F::from_bits(E[E_B64]);
-        T v = f.mul(F::from_bits(E[E_T]),
-                    f.mul(F::from_bits(E[E_BLO + (lane & 7)]), F::from_bits(E[E_BHI + (lane >> 3)])));
-        for (uint32_t i = lane; i < m; i += 64) {
-            const uint32_t w = table_word(f.key(v));
-            uint32_t h = w & tmask;
-            for (;;) {
-                const uint32_t prev = atomicCAS(&tab[h], 0xFFFFFFFFu, w);
-                if (prev == 0xFFFFFFFFu || prev == w) break;
-                h = (h + 1) & tmask;
-            }
-            v = f.mul(v, B64);
+__device__ __forceinline__ void hash_key(uint64_t x, uint32_t nbmask, uint32_t& fp, uint32_t& bucket) {
+    const uint32_t lo = (uint32_t)x;
+    fp = min(lo & 0xFFFFu, 0xFFFEu);
+    bucket = ((lo >> 16) ^ lo) & nbmask;
+}
+
+__device__ __forceinline__ void tab_insert(const BsgsTable& T, uint32_t bucket, uint32_t fp, uint32_t i) {
+    for (;;) {
+        const uint32_t old = atomicAdd(&T.cnt[bucket], 1u);
+        if (old < 8) {
+            T.fp[bucket * 8 + old] = (uint16_t)fp;
+            T.idx[bucket * 8 + old] = (uint16_t)i;
+            return;
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-
-        // giant steps G_j * (B^m)^g, g = lane, lane + 64, ... < ng
-        const T M64 = F::from_bits(E[E_BM64]);
-        const T Ml = f.mul(F::from_bits(E[E_MLO + (lane & 7)]), F::from_bits(E[E_MHI + (lane >> 3)]));
-        bool found = false;
-        for (uint32_t j = 0; j < s; ++j) {
-            v = f.mul(F::from_bits(E[E_G + j]), Ml);
-            for (uint32_t g = lane; g < ng; g += 64) {
-                const uint32_t w = table_word(f.key(v));
-                uint32_t h = w & tmask;
-                for (;;) {
-                    const uint32_t x = tab[h];
-                    if (x == w) {
-                        found = true;
-                        break;
-                    }
-                    if (x == 0xFFFFFFFFu) break;
-                    h = (h + 1) & tmask;
-                }
-                v = f.mul(v, M64);
-            }
-        }
-        if (__ballot(found) != 0 && lane == 0) {
-            const uint32_t o = atomicAdd(&pb.cnt->n_out, 1u);
-            if (o < pb.out_cap) pb.out[o] = pb.out_index ? (ssrc >> 32) : p;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        __builtin_amdgcn_wave_barrier();
+        bucket = (bucket + 1) & T.nbmask;
     }
+}
+
+__device__ __forceinline__ uint4 read_bucket(const BsgsTable& T, uint32_t bucket) {
+    return reinterpret_cast<const uint4*>(T.fp)[bucket];
+}
+
+// Some slot of the bucket holds fingerprint fp, or the bucket is full (the chain goes on).
+__device__ __forceinline__ bool bucket_needs_look(uint4 w, uint32_t fp) {
+    const bool m = ((w.x & 0xFFFFu) == fp) | ((w.x >> 16) == fp) | ((w.y & 0xFFFFu) == fp) | ((w.y >> 16) == fp) |
+                   ((w.z & 0xFFFFu) == fp) | ((w.z >> 16) == fp) | ((w.w & 0xFFFFu) == fp) | ((w.w >> 16) == fp);
+    return m | (w.w < 0xFFFF0000u);
+}
+
+// Bit k set when slot k of the bucket holds fingerprint fp.
+__device__ __forceinline__ uint32_t match_mask(uint4 w, uint32_t fp) {
+    return (((w.x & 0xFFFFu) == fp) ? 1u : 0u) | (((w.x >> 16) == fp) ? 2u : 0u) |
+           (((w.y & 0xFFFFu) == fp) ? 4u : 0u) | (((w.y >> 16) == fp) ? 8u : 0u) |
+           (((w.z & 0xFFFFu) == fp) ? 16u : 0u) | (((w.z >> 16) == fp) ? 32u : 0u) |
+           (((w.w & 0xFFFFu) == fp) ? 64u : 0u) | (((w.w >> 16) == fp) ? 128u : 0u);
+}
+
+template <class F>
+struct BsgsEntry {
+    typedef typename F::T T;
+    F f;
+    const uint64_t* E;
+    T t, B64;
+    uint32_t m, ng, s, meta, qp;
+    uint64_t nb, klim;
+    // exact baby value t * B^i
+    __device__ __forceinline__ T baby(uint32_t i) const {
+        const uint32_t r = i & 63;
+        T x = f.mul(F::from_bits(E[E_BLO + (r & 7)]), F::from_bits(E[E_BHI + (r >> 3)]));
+        if (i >> 6) x = f.mul(x, fpow(f, B64, i >> 6));
+        return f.mul(t, x);
+    }
+};
+
+// Every exact match of giant value v (canonical key xv, giant index g) in the table: the
+// solution k = g*m - i, when in range, gives n = nj + (k*stride + parity)*cm; keeps the smallest.
+template <class F>
+__device__ __forceinline__ void giant_hits(const BsgsEntry<F>& be, const BsgsTable& T, uint32_t bucket, uint32_t fp,
+                                           uint64_t xv, uint32_t g, uint64_t nj, uint64_t Kj, uint64_t stride,
+                                           uint64_t parity, uint64_t cm, uint64_t& best) {
+    for (uint32_t b = bucket;; b = (b + 1) & T.nbmask) {
+        const uint4 w = read_bucket(T, b);
+        uint32_t mm = match_mask(w, fp);
+        while (mm) {
+            const uint32_t k = __builtin_ctz(mm);
+            mm &= mm - 1;
+            const uint32_t i = T.idx[b * 8 + k];
+            if (be.f.key(be.baby(i)) != xv) continue;
+            const int64_t kk = (int64_t)g * be.m - (int64_t)i;
+            if (kk < 0 || (uint64_t)kk >= Kj) continue;
+            const uint64_t n = nj + ((uint64_t)kk * stride + parity) * cm;
+            if (n < best) best = n;
+        }
+        if (w.w >= 0xFFFF0000u) break;  // slot 7 empty: the chain ends here
+    }
+}
+
+// True when t * B^r == t for some 0 < r < m, i.e. the order of B is below m: then the giant steps
+// might not reach the smallest solution and the host decides the prime.
+template <class F>
+__device__ __forceinline__ bool has_cycle(const BsgsEntry<F>& be, const BsgsTable& T) {
+    const uint64_t xt = be.f.key(be.t);
+    uint32_t fp, b;
+    hash_key(xt, T.nbmask, fp, b);
+    bool cycle = false;
+    for (;; b = (b + 1) & T.nbmask) {
+        const uint4 w = read_bucket(T, b);
+        uint32_t mm = match_mask(w, fp);
+        while (mm) {
+            const uint32_t k = __builtin_ctz(mm);
+            mm &= mm - 1;
+            const uint32_t i = T.idx[b * 8 + k];
+            if (i != 0 && be.f.key(be.baby(i)) == xt) cycle = true;
+        }
+        if (w.w >= 0xFFFF0000u) break;
+    }
+    return cycle;
+}
+
+// x^tid for tid < 64 * WB from the entry's tables: x^(tid & 63) * (x^64)^(tid >> 6).
+template <class F, int WB>
+__device__ __forceinline__ typename F::T thread_pow(const F& f, const uint64_t* E, uint32_t lo_word, uint32_t hi_word,
+                                                  typename F::T x64, uint32_t tid, typename F::T& stride_pow) {
+    typedef typename F::T T;
+    T r = f.mul(F::from_bits(E[lo_word + (tid & 7)]), F::from_bits(E[hi_word + ((tid >> 3) & 7)]));
+    const T x128 = f.mul(x64, x64);
+    const uint32_t w = tid >> 6;
+    if (WB >= 2 && (w & 1)) r = f.mul(r, x64);
+    if (WB >= 4 && (w & 2)) r = f.mul(r, x128);
+    stride_pow = WB == 1 ? x64 : (WB == 2 ? x128 : f.mul(x128, x128));  // x^(64 * WB)
+    return r;
+}
+
+__device__ __forceinline__ bool mr_prime(uint64_t n);
+
+// One workgroup of WB wavefronts per entry: the table lives in LDS once per entry and every lane
+// of the group takes baby steps i == tid and giant steps g == tid (mod 64 * WB).
+template <class F, int WB>
+__global__ __launch_bounds__(64 * WB) void k_bsgs(ItemParams ip, PassBuffers pb, TableGeom tg) {
+    typedef typename F::T T;
+    constexpr uint32_t L = 64 * WB;
+    extern __shared__ uint32_t lds_tab[];
+    __shared__ uint64_t red[WB];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nbk = 1u << tg.nbbits;
+    BsgsTable Tb;
+    Tb.fp = reinterpret_cast<uint16_t*>(lds_tab);
+    Tb.idx = Tb.fp + nbk * 8u;
+    Tb.cnt = reinterpret_cast<uint32_t*>(Tb.idx + nbk * 8u);
+    Tb.nbmask = nbk - 1u;
+    const uint32_t n = min(pb.cnt->n_ent, pb.ent_cap);
+    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+        BsgsEntry<F> be;
+        be.E = pb.ent + (uint64_t)e * kEntWords;
+        const uint64_t* __restrict__ E = be.E;
+        const uint64_t p = E[E_P];
+        be.f.set(p, E[E_AUX], E[E_ONE]);
+        const uint64_t mg = E[E_MG];
+        be.m = (uint32_t)mg;
+        be.ng = (uint32_t)(mg >> 32);
+        const uint64_t ssrc = E[E_SSRC];
+        be.s = (uint32_t)ssrc & 0xFFu;
+        be.meta = (uint32_t)E[E_META];
+        be.qp = be.meta & 0xFFu;
+        be.nb = E[E_NB];
+        be.klim = E[E_KLIM];
+        be.t = F::from_bits(E[E_T]);
+        be.B64 = F::from_bits(E[E_B64]);
+        const F& f = be.f;
+
+        uint4* fp4 = reinterpret_cast<uint4*>(Tb.fp);
+        for (uint32_t i = tid; i < nbk; i += L) {
+            fp4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
+            Tb.cnt[i] = 0u;
+        }
+        __syncthreads();
+
+        // baby steps t * B^i, i == tid (mod L)
+        {
+            T step;
+            T v = f.mul(be.t, thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, tid, step));
+            for (uint32_t i = tid; i < be.m; i += L) {
+                uint32_t fp, b;
+                hash_key(f.key(v), Tb.nbmask, fp, b);
+                tab_insert(Tb, b, fp, i);
+                v = f.mul(v, step);
+            }
+        }
+        __syncthreads();
+
+        if (has_cycle(be, Tb)) {  // the same verdict in every lane
+            if (tid == 0) put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, kHostDecides);
+        } else {
+            // giant steps G_j * (B^m)^g, g == tid (mod L), g < ng
+            const bool qmode = (be.meta >> 10) & 1u;
+            const uint64_t stride = qmode ? 1 : ((be.meta >> 8) & 1u) + 1;
+            const uint64_t parity = qmode ? 0 : (be.meta >> 9) & 1u;
+            const uint64_t cmul = qmode ? be.qp : ip.cm;
+            T mstep;
+            const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(E[E_BM64]), tid, mstep);
+            uint64_t best = ~0ull;
+            for (uint32_t j = 0; j < be.s; ++j) {
+                uint64_t nj, Kj;
+                if (qmode) {
+                    const uint32_t d = (uint32_t)(E[E_DJ + (j >> 3)] >> (8 * (j & 7))) & 0xFFu;
+                    nj = be.nb + d;
+                    Kj = (ip.N1 - nj + be.qp - 1) / be.qp;
+                } else {
+                    nj = be.nb;
+                    Kj = be.klim;
+                }
+                T v = f.mul(F::from_bits(E[E_G + j]), Mt);
+                for (uint32_t g = tid; g < be.ng; g += L) {
+                    const uint64_t x = f.key(v);
+                    uint32_t fp, b;
+                    hash_key(x, Tb.nbmask, fp, b);
+                    const uint4 w = read_bucket(Tb, b);
+                    v = f.mul(v, mstep);  // the next giant value while the LDS read is in flight
+                    if (bucket_needs_look(w, fp)) giant_hits(be, Tb, b, fp, x, g, nj, Kj, stride, parity, cmul, best);
+                }
+            }
+            // smallest solution over the workgroup
+            for (int d = 32; d >= 1; d >>= 1) {
+                const uint64_t o = __shfl_xor((unsigned long long)best, d, 64);
+                best = o < best ? o : best;
+            }
+            if (lane_id() == 0) red[tid >> 6] = best;
+            __syncthreads();
+            if (tid == 0) {
+                for (int w = 1; w < WB; ++w) best = red[w] < best ? red[w] : best;
+                // deterministic Miller-Rabin: only primes reach the host
+                if (best != ~0ull && (pb.out_index || mr_prime(p)))
+                    put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, best);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Deterministic Miller-Rabin with the 12 bases 2..37 (exact for n < 2^64, is_prime64
+// Math.hpp:209-251), in Montgomery arithmetic; n odd, n < 2^62.
+__device__ __forceinline__ bool mr_prime(uint64_t n) {
+    const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
+    for (int k = 0; k < 12; ++k) {
+        if (n == bases[k]) return true;
+        if (n % bases[k] == 0) return false;
+    }
+    MontField f;
+    f.init(n);
+    const uint64_t rr = f.r2();
+    const uint64_t one = f.one(), mone = n - one;  // Montgomery forms of 1 and -1
+    uint64_t d = n - 1;
+    int s = 0;
+    while ((d & 1) == 0) {
+        d >>= 1;
+        ++s;
+    }
+    for (int k = 0; k < 12; ++k) {
+        uint64_t x = fpow(f, f.from(bases[k], rr), d);
+        if (x == one || x == mone) continue;
+        bool composite = true;
+        for (int r = 1; r < s && composite; ++r) {
+            x = f.mul(x, x);
+            if (x == mone) composite = false;
+        }
+        if (composite) return false;
+    }
+    return true;
 }
 
 // ------------------------------------------------------------------------------------- launchers
@@ -555,26 +755,31 @@ void launch_setup(hipStream_t s, FieldKind fk, uint64_t lo, const ItemParams& ip
     }
 }
 
-// n_ent: entries to cover with one wavefront each (0 = a persistent grid of `grid` blocks).
-// Two wavefronts (two tables) per block up to 2^12-word tables, one above, so that a block never
-// asks for more than 64 KiB of LDS.
-void launch_bsgs(hipStream_t s, FieldKind fk, uint32_t tbits, const PassBuffers& pb, uint32_t n_ent,
-                 uint32_t grid) {
-    const int W = tbits <= 12 ? 2 : 1;
-    const size_t lds = (size_t)W * ((size_t)1 << tbits) * sizeof(uint32_t);
-    if (n_ent) grid = (n_ent + W - 1) / W;
+// Table geometry for baby-step counts up to m_max: about two keys per 8-slot bucket.
+TableGeom table_geom(uint64_t m_max, uint64_t p_min) {
+    (void)p_min;
+    TableGeom tg;
+    tg.ibits = 16;
+    tg.fpbits = 16;
+    tg.nbbits = 4;
+    while ((2ull << tg.nbbits) < m_max) ++tg.nbbits;  // 8 * 2^nbbits slots >= 4 * m_max
+    tg.waves = kBsgsWaves;
+    return tg;
+}
+
+// fingerprints (16 B) + indices (16 B) + a fill counter (4 B) per bucket
+size_t bsgs_lds_bytes(const TableGeom& tg) { return (size_t)36 << tg.nbbits; }
+
+// n_ent: entries, one workgroup each (0 = a persistent grid of `grid` workgroups).
+void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const ItemParams& ip, const PassBuffers& pb,
+                 uint32_t n_ent, uint32_t grid) {
+    const size_t lds = bsgs_lds_bytes(tg);
+    if (n_ent) grid = n_ent;
     if (grid == 0) return;
-    if (W == 2) {
-        if (fk == FIELD_F64)
-            hipLaunchKernelGGL((k_bsgs<F64Field, 2>), dim3(grid), dim3(128), lds, s, pb, tbits);
-        else
-            hipLaunchKernelGGL((k_bsgs<MontField, 2>), dim3(grid), dim3(128), lds, s, pb, tbits);
-    } else {
-        if (fk == FIELD_F64)
-            hipLaunchKernelGGL((k_bsgs<F64Field, 1>), dim3(grid), dim3(64), lds, s, pb, tbits);
-        else
-            hipLaunchKernelGGL((k_bsgs<MontField, 1>), dim3(grid), dim3(64), lds, s, pb, tbits);
-    }
+    if (fk == FIELD_F64)
+        hipLaunchKernelGGL((k_bsgs<F64Field, kBsgsWaves>), dim3(grid), dim3(64 * kBsgsWaves), lds, s, ip, pb, tg);
+    else
+        hipLaunchKernelGGL((k_bsgs<MontField, kBsgsWaves>), dim3(grid), dim3(64 * kBsgsWaves), lds, s, ip, pb, tg);
 }
 
 }  // namespace mps

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <string>
+#include <vector>
 
 #include "mps_gpu.h"
 #include "mps_kernels.h"
@@ -9,7 +10,10 @@
 namespace mps {
 
 // Kernel constants of one item (the conversions of Gpu.hpp:862-875 and gpu_classes,
-// Math.hpp:324-337).  Returns false, with the reason, when the BSGS table would not fit.
-bool item_params(const mps_item* it, bool classes_on, ItemParams& ip, uint32_t& tbits, std::string& why);
+// Math.hpp:324-337), the largest baby-step count m_max (plain-mode m is capped at mcap) and the
+// class-mode size table csz[qp * 32 + s] = (m, giant steps per class).
+// Returns false, with the reason, when the BSGS table would not fit.
+bool item_params(const mps_item* it, bool classes_on, uint32_t mcap, ItemParams& ip, uint64_t& m_max,
+                 std::vector<uint2>& csz, std::string& why);
 
 }  // namespace mps

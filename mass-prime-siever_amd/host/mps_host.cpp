@@ -28,9 +28,24 @@ struct ChunkResult {
     int status = MPS_OK;
 };
 
-// One chunk [lo, hi): device sieve + filter, then the exact check of what it flagged
-// (Worker.hpp:216-228 and :270-280).
-void process_chunk(mps_gpu* g, const mps_work_item* it, uint64_t lo, uint64_t hi, std::vector<uint64_t>& buf,
+uint64_t powmod_u64(uint64_t b, uint64_t e, uint64_t m) {
+    unsigned __int128 r = 1, x = b % m;
+    for (; e; e >>= 1) {
+        if (e & 1) r = r * x % m;
+        x = x * x % m;
+    }
+    return (uint64_t)r;
+}
+
+uint64_t mod_i64(int64_t v, uint64_t m) {
+    const i128 r = (i128)v % (i128)m;
+    return (uint64_t)(r < 0 ? r + (i128)m : r);
+}
+
+// One chunk [lo, hi) (Worker.hpp:216-228 and :270-280): the device returns exact relations
+// (p, M) of the primes it decided and the numbers it leaves to the host; the host adds
+// q = ord_p(b) to the former and runs the exact is_prime64 + run_bsgs on the latter.
+void process_chunk(mps_gpu* g, const mps_work_item* it, uint64_t lo, uint64_t hi, std::vector<mps_dev_relation>& buf,
                    ChunkResult& r) {
     const mps_item di = {it->b, it->c, it->N0, it->N1, it->cls_mod, it->cls_res};
     if (!(hi < (1ull << 62) && lo > 3)) {  // gpu_range_ok (Worker.hpp:209)
@@ -39,10 +54,10 @@ void process_chunk(mps_gpu* g, const mps_work_item* it, uint64_t lo, uint64_t hi
     }
     size_t n = 0;
     const double t0 = now_s();
-    int st = mps_gpu_sieve_filter(g, lo, hi, &di, buf.data(), buf.size(), &n);
+    int st = mps_gpu_sieve_relations(g, lo, hi, &di, buf.data(), buf.size(), &n);
     if (st == MPS_CAPACITY) {
         buf.resize(n + 1024);
-        st = mps_gpu_sieve_filter(g, lo, hi, &di, buf.data(), buf.size(), &n);
+        st = mps_gpu_sieve_relations(g, lo, hi, &di, buf.data(), buf.size(), &n);
     }
     const double t1 = now_s();
     r.dev_s += t1 - t0;
@@ -50,15 +65,30 @@ void process_chunk(mps_gpu* g, const mps_work_item* it, uint64_t lo, uint64_t hi
         r.status = st;
         return;
     }
-    std::sort(buf.begin(), buf.begin() + n);
-    n = std::unique(buf.begin(), buf.begin() + n) - buf.begin();
+    std::sort(buf.begin(), buf.begin() + n,
+              [](const mps_dev_relation& a, const mps_dev_relation& b) { return a.p < b.p || (a.p == b.p && a.M < b.M); });
     r.flagged += n;
     for (size_t i = 0; i < n; ++i) {
-        const uint64_t p = buf[i];
-        if (p < lo || p >= hi || !mps_is_prime64(p)) continue;
-        r.prime_flagged++;
+        if (i > 0 && buf[i].p == buf[i - 1].p) continue;
+        const uint64_t p = buf[i].p;
+        if (p < lo || p >= hi) continue;
         mps_relation rel;
-        if (mps_run_bsgs(p, it, &rel)) r.rels.push_back(rel);
+        if (buf[i].M == MPS_HOST_DECIDES) {
+            if (!mps_is_prime64(p)) continue;
+            r.prime_flagged++;
+            if (mps_run_bsgs(p, it, &rel)) r.rels.push_back(rel);
+            continue;
+        }
+        r.prime_flagged++;
+        // the device verified the equation exactly; check it once more before writing it
+        if (powmod_u64(mod_i64(it->b, p), buf[i].M, p) != (p - mod_i64(it->c, p)) % p) {
+            r.status = MPS_ERROR;
+            continue;
+        }
+        rel.p = p;
+        rel.M = buf[i].M;
+        rel.q = mps_multiplicative_order(mod_i64(it->b, p), p);
+        r.rels.push_back(rel);
     }
     r.chk_s += now_s() - t1;
 }
@@ -108,7 +138,7 @@ int mps_run_item(mps_gpu* g, const mps_work_item* item, const mps_run_opts* opts
     ChunkResult total;
     const double t0 = now_s();
     auto feeder = [&]() {
-        std::vector<uint64_t> buf(1 << 16);
+        std::vector<mps_dev_relation> buf(1 << 14);
         ChunkResult mine;
         for (size_t k; (k = next.fetch_add(1)) < chunks.size();) {
             ChunkResult r;

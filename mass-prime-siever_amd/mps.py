@@ -35,6 +35,13 @@ class Relation(ctypes.Structure):
     _fields_ = [("p", ctypes.c_uint64), ("M", ctypes.c_uint64), ("q", ctypes.c_uint64)]
 
 
+class DevRelation(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_uint64), ("M", ctypes.c_uint64)]
+
+
+HOST_DECIDES = 0xFFFFFFFFFFFFFFFF
+
+
 class RunOpts(ctypes.Structure):
     _fields_ = [("chunk_ints", ctypes.c_uint64), ("feeder_threads", ctypes.c_int)]
 
@@ -61,6 +68,7 @@ def build():
 
 
 EXPORTS = ["mps_gpu_create", "mps_gpu_ok", "mps_gpu_set_classes", "mps_gpu_sieve_filter", "mps_gpu_filter",
+           "mps_gpu_sieve_relations",
            "mps_gpu_last_error", "mps_gpu_destroy", "mps_gpu_get_stats", "mps_gpu_reset_stats",
            "mps_gpu_configure", "mps_gpu_debug_stage", "mps_run_item", "mps_results_json", "mps_is_prime64",
            "mps_multiplicative_order", "mps_run_bsgs"]
@@ -86,6 +94,8 @@ def lib():
     L.mps_gpu_set_classes.argtypes = [vp, i32]
     L.mps_gpu_sieve_filter.restype = i32
     L.mps_gpu_sieve_filter.argtypes = [vp, u64, u64, P(Item), vp, sz, P(sz)]
+    L.mps_gpu_sieve_relations.restype = i32
+    L.mps_gpu_sieve_relations.argtypes = [vp, u64, u64, P(Item), vp, sz, P(sz)]
     L.mps_gpu_filter.restype = i32
     L.mps_gpu_filter.argtypes = [vp, vp, sz, P(Item), vp, sz, P(sz)]
     L.mps_gpu_last_error.restype = ctypes.c_char_p
@@ -97,7 +107,7 @@ def lib():
     L.mps_gpu_reset_stats.restype = None
     L.mps_gpu_reset_stats.argtypes = [vp]
     L.mps_gpu_configure.restype = i32
-    L.mps_gpu_configure.argtypes = [vp, i32, u64]
+    L.mps_gpu_configure.argtypes = [vp, i32, u64, ctypes.c_uint32]
     L.mps_gpu_debug_stage.restype = i32
     L.mps_gpu_debug_stage.argtypes = [vp, i32, u64, u64, P(Item), vp, sz, P(sz)]
     L.mps_run_item.restype = i32
@@ -147,8 +157,8 @@ class GpuSiever:
     def set_classes(self, on):
         lib().mps_gpu_set_classes(self.h, 1 if on else 0)
 
-    def configure(self, field=-1, pass_odds=0):
-        return lib().mps_gpu_configure(self.h, field, pass_odds)
+    def configure(self, field=-1, pass_odds=0, mcap=0):
+        return lib().mps_gpu_configure(self.h, field, pass_odds, mcap)
 
     def sieve_filter(self, lo, hi, item, flagged):
         """Appends what the device could not rule out in [lo|1, hi) to `flagged`; False = declined."""
@@ -165,6 +175,22 @@ class GpuSiever:
                 return False
             flagged.extend(int(x) for x in out[: n.value])
             return True
+
+    def sieve_relations(self, lo, hi, item):
+        """Exact device records (p, M) of [lo|1, hi); M == HOST_DECIDES: the host decides p.
+        Returns None when the chunk is declined."""
+        di = device_item(item) if isinstance(item, WorkItem) else item
+        cap = 1 << 14
+        while True:
+            out = (DevRelation * cap)()
+            n = ctypes.c_size_t()
+            st = lib().mps_gpu_sieve_relations(self.h, lo, hi, ctypes.byref(di), out, cap, ctypes.byref(n))
+            if st == MPS_CAPACITY:
+                cap = n.value + 1024
+                continue
+            if st != MPS_OK:
+                return None
+            return [(out[i].p, out[i].M) for i in range(n.value)]
 
     def filter(self, primes, item, todo):
         """Appends the ascending indices of the primes that may divide an in-class candidate."""

@@ -72,3 +72,21 @@ def test_filter_path_matches_oracle(gpu, mps, oracle):
             rels.append(r)
     want, _ = oracle.process_range(oracle.make_item(2, -25, P0, P1, 3_600_000, 5_000_000, 2, 1), P0, P1, 1 << 20, 8)
     assert rels == want
+
+
+@pytest.mark.parametrize("it", ITEMS[:5])
+def test_device_relations_exact(gpu, mps, oracle, it):
+    """The device's own verdicts are exact: every record it decides is an oracle relation with
+    the same M (smallest in-class n), and every oracle relation appears."""
+    b, c, P0, P1, N0, N1, mod, res = it
+    print(f"device relations {it}", flush=True)
+    item = mps.work_item(b, c, P0, P1, N0, N1, mod, res)
+    recs = gpu.sieve_relations(P0, P1, item)
+    assert recs is not None
+    want, _ = oracle.process_range(oracle.make_item(b, c, P0, P1, N0, N1, mod, res), P0, P1, 1 << 20, 8)
+    want_pm = {(p, M) for p, M, q in want}
+    decided = {(p, M) for p, M in recs if M != mps.HOST_DECIDES}
+    host = {p for p, M in recs if M == mps.HOST_DECIDES}
+    assert decided <= want_pm
+    assert {p for p, _ in want_pm} <= {p for p, _ in decided} | host
+    assert len(host) <= max(2, len(want) // 10)

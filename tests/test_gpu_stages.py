@@ -9,6 +9,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
+MCAP = 2048  # default largest baby-step count without classes
+
 SIEVE_RANGES = [
     (1001, 1001 + 2 * 300_000),            # tiny p: survivors are the primes
     (10**9, 10**9 + 2**22),                # c1 scale: sqrt(hi) < 65521, survivors are the primes
@@ -71,16 +73,18 @@ def test_entries_match_oracle_decisions(gpu, mps, oracle, it, field):
         else:
             r, e = oracle.prep(x, oit, True)
             if r == 1:
-                want[x] = (e.m, (e.jcount + e.m - 1) // e.m + 1, e.stride, e.parity)
+                want[x] = (e.jcount, e.stride, e.parity)
     got = {}
     for p, m, ng, sinfo in ent.tolist():
         s, info = sinfo & 0xFF, sinfo >> 8
         if qmode:
             got[p] = (s, info)
-            # coverage of every in-class n: (Gc - 1) * m >= K - 1
-            qp = None
         else:
-            got[p] = (m, ng, 2 if info & 1 else 1, (info >> 1) & 1)
+            # the device picks its own split (m <= MCAP baby steps, ng giant steps g = 0 .. ng-1):
+            # every j in [0, jcount) must be g*m - i for some g < ng, i < m
+            jc = want.get(p, (0,))[0]
+            assert 1 <= m <= MCAP and m * (ng - 1) >= jc, (p, m, ng, jc)
+            got[p] = (jc, 2 if info & 1 else 1, (info >> 1) & 1)
     assert len(got) == len(ent), "an entry was written twice"
     assert set(got) == set(want)
     assert got == want
