@@ -110,8 +110,24 @@ void mps_gpu_reset_stats(mps_gpu* g);
 /* Tuning and test hooks (no counterpart in the reference).
  * field: -1 = automatic (FP64 when hi <= 2^50, else Montgomery), 0 = Montgomery, 1 = FP64.
  * pass_odds: odd numbers per device pass (buffers are sized from it; 0 keeps the current).
- * mcap: largest baby-step count without classes, 64..4096 (0 keeps the current; default 1024). */
-int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap);
+ * mcap: largest baby-step count without classes, 256..4096 (0 keeps the current; default 2048).
+ * waves: wavefronts that share one BSGS entry, 1, 2 or 4 (0 = chosen per item). */
+int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap, uint32_t waves);
+
+/* Measured issue rates (operations per second over the whole chip) and FP64 FMA dependent-issue
+ * latency of device `device_index`; the roofline peaks of the benchmark come from here. */
+typedef struct mps_valu_rates {
+    double fma64_per_s;
+    double mul_lo32_per_s;
+    double mul_hi32_per_s;
+    double mad_u64_u32_per_s;
+    double add32_per_s;
+    double fma64_latency_cycles;
+    double clock_mhz;
+    int cus;
+} mps_valu_rates;
+
+int mps_gpu_measure_valu(int device_index, mps_valu_rates* r);
 
 /* Stage outputs of one pass over [lo|1, hi) for stage-level parity tests:
  *   stage 0: survivors (values), stage 1: candidates (value, info) pairs,

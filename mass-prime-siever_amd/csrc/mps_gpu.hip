@@ -82,15 +82,15 @@ void class_size(uint64_t K, uint32_t s, uint32_t L, uint32_t maxm, uint32_t& m, 
 
 namespace mps {
 
-bool item_params(const mps_item* it, bool classes_on, uint32_t mcap, ItemParams& ip, uint64_t& m_max,
-                 std::vector<uint2>& csz, std::string& why) {
+bool item_params(const mps_item* it, bool classes_on, uint32_t mcap, uint32_t waves, ItemParams& ip,
+                 uint64_t& m_max, std::vector<uint2>& csz, std::string& why) {
     std::memset(&ip, 0, sizeof(ip));
     ip.b = it->b;
     ip.c = it->c;
     const uint64_t N0 = it->N0 < 0 ? 0 : (uint64_t)it->N0;
     const uint64_t N1 = it->N1 < 0 ? 0 : (uint64_t)it->N1;
     const uint64_t cm = it->cls_mod == 0 ? 1 : it->cls_mod;
-    const uint32_t L = 64 * kBsgsWaves;
+    const uint32_t L = 64 * waves;
     ip.N0 = N0;
     ip.N1 = N1;
     ip.cm = cm;
@@ -160,6 +160,7 @@ struct mps_gpu {
     int field_mode = -1;
     uint64_t pass_odds = 1ull << 25;
     uint32_t mcap = 2048;
+    uint32_t waves = 0;  // 0: by item (4 for plain-mode tables above 1024 baby steps, else 2)
     std::mutex mu;
     uint2* d_csz = nullptr;
     uint2* h_csz = nullptr;
@@ -238,9 +239,12 @@ bool read_counts(mps_gpu* g) {
     return true;
 }
 
-uint32_t blocks_for(uint64_t n, uint32_t per_block) {
+// Workgroups for n items of per_block each, capped at 8 per CU: the stage kernels loop over tiles,
+// so a capped grid keeps their per-workgroup atomics few.
+uint32_t blocks_for(const mps_gpu* g, uint64_t n, uint32_t per_block) {
     const uint64_t b = (n + per_block - 1) / per_block;
-    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, 1u << 30));
+    const uint64_t cap = 8ull * (uint64_t)std::max(1, g->num_cu);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
 }
 
 // One pass through the stage chain.  stop_stage: 0 after the sieve, 1 after the filter,
@@ -260,7 +264,7 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (stop_stage == 0 || n_surv == 0) return true;
 
     HIPCHK(hipEventRecord(g->ev[1], s), "event");
-    launch_filter(s, plo, ip, g->pb, blocks_for(n_surv, 256));
+    launch_filter(s, plo, ip, g->pb, blocks_for(g, n_surv, 256));
     HIPCHK(hipGetLastError(), "filter launch");
     HIPCHK(hipEventRecord(g->ev[2], s), "event");
     if (!read_counts(g)) return false;
@@ -269,7 +273,7 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (stop_stage == 1 || n_cand == 0) return true;
 
     HIPCHK(hipEventRecord(g->ev[2], s), "event");
-    launch_setup(s, fk, plo, ip, g->pb, blocks_for(n_cand, 256));
+    launch_setup(s, fk, plo, ip, g->pb, blocks_for(g, n_cand, 256));
     HIPCHK(hipGetLastError(), "setup launch");
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
     if (!read_counts(g)) return false;
@@ -303,9 +307,17 @@ void accumulate(mps_gpu* g, uint64_t pc) {
 }
 
 // Item constants plus the class-size table, staged in pinned memory for run_pass to upload.
-bool prepare_item(mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max) {
+bool prepare_item(mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max,
+                  uint32_t& waves) {
     std::vector<uint2> csz;
-    if (!item_params(item, classes_on, g->mcap, ip, m_max, csz, g->err)) return false;
+    waves = g->waves;
+    if (waves == 0) {
+        // measured on MI355X: two wavefronts per entry for the class tables (m <= 1024) and short
+        // plain ranges; four when plain-mode entries run up to 2048 baby steps
+        if (!item_params(item, classes_on, g->mcap, 2, ip, m_max, csz, g->err)) return false;
+        waves = m_max > 1024 ? 4 : 2;
+    }
+    if (!item_params(item, classes_on, g->mcap, waves, ip, m_max, csz, g->err)) return false;
     std::memcpy(g->h_csz, csz.data(), csz.size() * sizeof(uint2));
     ip.csz = g->d_csz;
     return true;
@@ -383,10 +395,14 @@ void mps_gpu_set_classes(mps_gpu* g, int on) {
     if (g) g->classes_on = on != 0;
 }
 
-int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap) {
+int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap, uint32_t waves) {
     if (g == nullptr) return MPS_ERROR;
     std::lock_guard<std::mutex> lock(g->mu);
     g->field_mode = field;
+    if (waves) {
+        if (waves != 1 && waves != 2 && waves != 4) return MPS_ERROR;
+        g->waves = waves;
+    }
     if (mcap) {
         if (mcap < 256 || mcap > 4096) return MPS_ERROR;
         g->mcap = mcap;
@@ -414,9 +430,10 @@ int sieve_records(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, st
     if (lo >= hi) return MPS_OK;
     ItemParams ip;
     uint64_t m_max = 0;
-    if (!prepare_item(g, item, g->classes_on, ip, m_max)) return MPS_DECLINED;
+    uint32_t waves = 0;
+    if (!prepare_item(g, item, g->classes_on, ip, m_max, waves)) return MPS_DECLINED;
     const FieldKind fk = pick_field(g, hi);
-    const TableGeom tg = table_geom(m_max, lo);
+    const TableGeom tg = table_geom(m_max, waves);
     const uint64_t count = (hi - lo + 1) / 2;
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
     if (!ensure_buffers(g, std::min(count, g->pass_odds))) return MPS_ERROR;
@@ -492,7 +509,8 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
     if (n == 0) return MPS_OK;
     ItemParams ip;
     uint64_t m_max = 0;
-    if (!prepare_item(g, item, false, ip, m_max)) return MPS_DECLINED;  // filter(): plain prep path
+    uint32_t waves = 0;
+    if (!prepare_item(g, item, false, ip, m_max, waves)) return MPS_DECLINED;  // filter(): plain prep path
     uint64_t pmax = 0, pmin = ~0ull;
     for (size_t i = 0; i < n; ++i) {
         pmax = std::max(pmax, primes[i]);
@@ -503,7 +521,7 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
         return MPS_DECLINED;
     }
     const FieldKind fk = pick_field(g, pmax + 1);
-    const TableGeom tg = table_geom(m_max, std::max<uint64_t>(pmin, 3));
+    const TableGeom tg = table_geom(m_max, waves);
     const size_t batch = (size_t)std::min<uint64_t>(g->pass_odds, 1ull << 22);
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
     if (!ensure_buffers(g, batch)) return MPS_ERROR;
@@ -564,9 +582,10 @@ int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const m
     if (lo <= 3 || hi >= (1ull << 62) || lo >= hi) return MPS_DECLINED;
     ItemParams ip;
     uint64_t m_max = 0;
-    if (!prepare_item(g, item, g->classes_on, ip, m_max)) return MPS_DECLINED;
+    uint32_t waves = 0;
+    if (!prepare_item(g, item, g->classes_on, ip, m_max, waves)) return MPS_DECLINED;
     const FieldKind fk = pick_field(g, hi);
-    const TableGeom tg = table_geom(m_max, lo);
+    const TableGeom tg = table_geom(m_max, waves);
     const uint64_t count = (hi - lo + 1) / 2;
     if (count > g->pass_odds) {
         g->err = "debug stage covers one pass only";

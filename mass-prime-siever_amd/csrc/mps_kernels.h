@@ -36,13 +36,14 @@ enum EntryWord : uint32_t {
     E_G = 48,     // giant start per class G_j = b^(n_j), j < s
 };
 
+// Stage counters.  Each counter that many workgroups update has a 128-byte line of its own, so
+// atomics on different counters never contend for one L2 line.
 struct Counters {
-    uint32_t n_surv, n_cand, n_ent, n_out;
-    uint32_t n_flag2, n_fermat_fail, pad0, pad1;
-    unsigned long long w_fermat;  // Montgomery/FP64 products spent, by stage
-    unsigned long long w_setup;
-    unsigned long long w_bsgs;
-    unsigned long long pad2;
+    uint32_t n_surv, p0[31];
+    uint32_t n_cand, p1[31];
+    uint32_t n_ent, p2[31];
+    uint32_t n_out, n_flag2, n_fermat_fail, p3[29];
+    unsigned long long w_fermat, w_setup, w_bsgs, p4[13];
 };
 
 // Per-item constants for the filter and setup stages.
@@ -91,8 +92,8 @@ void launch_setup(hipStream_t s, FieldKind fk, uint64_t lo, const ItemParams& ip
 struct TableGeom {
     uint32_t nbbits, fpbits, ibits, waves;
 };
-TableGeom table_geom(uint64_t m_max, uint64_t p_min);
-constexpr uint32_t kBsgsWaves = 4;                // wavefronts of one BSGS workgroup (one entry)
+TableGeom table_geom(uint64_t m_max, uint64_t waves);
+constexpr uint32_t kBsgsWaves = 4;                // default wavefronts of one BSGS workgroup (one entry)
 size_t bsgs_lds_bytes(const TableGeom& tg);
 
 void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const ItemParams& ip, const PassBuffers& pb,

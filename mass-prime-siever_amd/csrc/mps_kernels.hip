@@ -32,6 +32,30 @@ __device__ __forceinline__ uint32_t wave_append(bool keep, uint32_t* counter) {
     return base + rank;
 }
 
+// Block-aggregated append: every thread of the workgroup must call it (it synchronises the
+// block).  One atomic per call and workgroup; sh needs NW + 1 words of LDS.
+template <int NW>
+__device__ __forceinline__ uint32_t block_append(bool keep, uint32_t* counter, uint32_t* sh) {
+    const uint64_t mask = __ballot(keep);
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    if (lane == 0) sh[w] = (uint32_t)__popcll(mask);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < NW; ++i) {
+            const uint32_t c = sh[i];
+            sh[i] = tot;
+            tot += c;
+        }
+        sh[NW] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+    const uint32_t slot = sh[NW] + sh[w] + rank;
+    __syncthreads();
+    return slot;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
@@ -229,12 +253,12 @@ __device__ __forceinline__ void filter_class(uint64_t p, const ItemParams& ip, b
 }
 
 __global__ __launch_bounds__(256) void k_filter(uint64_t lo, ItemParams ip, PassBuffers pb) {
+    __shared__ uint32_t sh[5];
     const uint32_t n = min(pb.cnt->n_surv, pb.surv_cap);
     const uint32_t lane = lane_id();
-    const uint32_t wglobal = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    for (uint32_t i0 = wglobal * 64; i0 < n; i0 += nwaves * 64) {
-        const uint32_t i = i0 + lane;
+    uint32_t nflag2 = 0;
+    for (uint32_t i0 = blockIdx.x * 256; i0 < n; i0 += gridDim.x * 256) {
+        const uint32_t i = i0 + threadIdx.x;
         bool keep = false, flag2 = false;
         uint32_t info = 0, idx = 0;
         uint64_t p = 0;
@@ -246,13 +270,14 @@ __global__ __launch_bounds__(256) void k_filter(uint64_t lo, ItemParams ip, Pass
             else
                 filter_plain(p, ip, keep, flag2, info);
         }
-        const uint32_t slot = wave_append(keep, &pb.cnt->n_cand);
+        const uint32_t slot = block_append<4>(keep, &pb.cnt->n_cand, sh);
         if (keep && slot < pb.cand_cap) pb.cand[slot] = (uint64_t)idx | ((uint64_t)info << 32);
         const uint32_t oslot = wave_append(flag2, &pb.cnt->n_out);
         if (flag2) put_record(pb, oslot, pb.out_index ? idx : p, kHostDecides);
-        const uint64_t fb = __ballot(flag2);
-        if (lane == 0 && fb != 0) atomicAdd(&pb.cnt->n_flag2, (uint32_t)__popcll(fb));
+        nflag2 += flag2 ? 1u : 0u;
     }
+    nflag2 = (uint32_t)wave_sum64(nflag2);
+    if (lane == 0 && nflag2 != 0) atomicAdd(&pb.cnt->n_flag2, nflag2);
 }
 
 // ------------------------------------------------------------------------------------- setup
@@ -290,14 +315,13 @@ __device__ __forceinline__ uint32_t bitlen64(uint64_t x) { return x ? 64 - __bui
 template <class F, bool QMODE>
 __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassBuffers pb) {
     typedef typename F::T T;
+    __shared__ uint32_t sh[5];
     const uint32_t n = min(pb.cnt->n_cand, pb.cand_cap);
     const uint32_t lane = lane_id();
-    const uint32_t wglobal = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     unsigned long long w_fermat = 0, w_setup = 0, w_bsgs = 0;
     uint32_t fermat_fail = 0;
-    for (uint32_t i0 = wglobal * 64; i0 < n; i0 += nwaves * 64) {
-        const uint32_t i = i0 + lane;
+    for (uint32_t i0 = blockIdx.x * 256; i0 < n; i0 += gridDim.x * 256) {
+        const uint32_t i = i0 + threadIdx.x;
         bool keep = false;
         uint64_t p = 0, m = 0, ng = 0, jc = 0;
         uint32_t idx = 0, info = 0, s = 1, mask = 0, dmin = 0;
@@ -376,7 +400,7 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
                 }
             }
         }
-        const uint32_t slot = wave_append(keep, &pb.cnt->n_ent);
+        const uint32_t slot = block_append<4>(keep, &pb.cnt->n_ent, sh);
         if (keep && slot >= pb.ent_cap) {
             const uint32_t o = atomicAdd(&pb.cnt->n_out, 1u);  // no room: the host decides this one
             put_record(pb, o, pb.out_index ? idx : p, kHostDecides);
@@ -440,15 +464,28 @@ __global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassB
             w_bsgs += m + (uint64_t)s * ng;
         }
     }
+    // one atomic per counter and workgroup
+    __shared__ unsigned long long red[4][4];
+    const uint32_t w = threadIdx.x >> 6;
     w_fermat = wave_sum64(w_fermat);
     w_setup = wave_sum64(w_setup);
     w_bsgs = wave_sum64(w_bsgs);
     const unsigned long long ff = wave_sum64(fermat_fail);
     if (lane == 0) {
-        if (w_fermat) atomicAdd(&pb.cnt->w_fermat, w_fermat);
-        if (w_setup) atomicAdd(&pb.cnt->w_setup, w_setup);
-        if (w_bsgs) atomicAdd(&pb.cnt->w_bsgs, w_bsgs);
-        if (ff) atomicAdd(&pb.cnt->n_fermat_fail, (uint32_t)ff);
+        red[0][w] = w_fermat;
+        red[1][w] = w_setup;
+        red[2][w] = w_bsgs;
+        red[3][w] = ff;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const unsigned long long v = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] + red[threadIdx.x][3];
+        if (v != 0) {
+            if (threadIdx.x == 0) atomicAdd(&pb.cnt->w_fermat, v);
+            if (threadIdx.x == 1) atomicAdd(&pb.cnt->w_setup, v);
+            if (threadIdx.x == 2) atomicAdd(&pb.cnt->w_bsgs, v);
+            if (threadIdx.x == 3) atomicAdd(&pb.cnt->n_fermat_fail, (uint32_t)v);
+        }
     }
 }
 
@@ -756,14 +793,13 @@ void launch_setup(hipStream_t s, FieldKind fk, uint64_t lo, const ItemParams& ip
 }
 
 // Table geometry for baby-step counts up to m_max: about two keys per 8-slot bucket.
-TableGeom table_geom(uint64_t m_max, uint64_t p_min) {
-    (void)p_min;
+TableGeom table_geom(uint64_t m_max, uint64_t waves) {
     TableGeom tg;
     tg.ibits = 16;
     tg.fpbits = 16;
     tg.nbbits = 4;
     while ((2ull << tg.nbbits) < m_max) ++tg.nbbits;  // 8 * 2^nbbits slots >= 4 * m_max
-    tg.waves = kBsgsWaves;
+    tg.waves = (uint32_t)waves;
     return tg;
 }
 
@@ -771,15 +807,26 @@ TableGeom table_geom(uint64_t m_max, uint64_t p_min) {
 size_t bsgs_lds_bytes(const TableGeom& tg) { return (size_t)36 << tg.nbbits; }
 
 // n_ent: entries, one workgroup each (0 = a persistent grid of `grid` workgroups).
+template <class F>
+void launch_bsgs_f(hipStream_t s, const TableGeom& tg, const ItemParams& ip, const PassBuffers& pb, uint32_t grid,
+                   size_t lds) {
+    if (tg.waves == 1)
+        hipLaunchKernelGGL((k_bsgs<F, 1>), dim3(grid), dim3(64), lds, s, ip, pb, tg);
+    else if (tg.waves == 2)
+        hipLaunchKernelGGL((k_bsgs<F, 2>), dim3(grid), dim3(128), lds, s, ip, pb, tg);
+    else
+        hipLaunchKernelGGL((k_bsgs<F, 4>), dim3(grid), dim3(256), lds, s, ip, pb, tg);
+}
+
 void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const ItemParams& ip, const PassBuffers& pb,
                  uint32_t n_ent, uint32_t grid) {
     const size_t lds = bsgs_lds_bytes(tg);
     if (n_ent) grid = n_ent;
     if (grid == 0) return;
     if (fk == FIELD_F64)
-        hipLaunchKernelGGL((k_bsgs<F64Field, kBsgsWaves>), dim3(grid), dim3(64 * kBsgsWaves), lds, s, ip, pb, tg);
+        launch_bsgs_f<F64Field>(s, tg, ip, pb, grid, lds);
     else
-        hipLaunchKernelGGL((k_bsgs<MontField, kBsgsWaves>), dim3(grid), dim3(64 * kBsgsWaves), lds, s, ip, pb, tg);
+        launch_bsgs_f<MontField>(s, tg, ip, pb, grid, lds);
 }
 
 }  // namespace mps

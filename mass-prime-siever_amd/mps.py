@@ -62,6 +62,20 @@ class GpuStats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class ValuRates(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("fma64_per_s", "mul_lo32_per_s", "mul_hi32_per_s",
+                                               "mad_u64_u32_per_s", "add32_per_s", "fma64_latency_cycles",
+                                               "clock_mhz")] + [("cus", ctypes.c_int)]
+
+
+def measure_valu(device_index=0):
+    """Measured VALU issue rates of the device (include/mps_gpu.h mps_gpu_measure_valu)."""
+    r = ValuRates()
+    if lib().mps_gpu_measure_valu(device_index, ctypes.byref(r)) != MPS_OK:
+        raise RuntimeError("VALU rate probe failed")
+    return {n: getattr(r, n) for n, _ in r._fields_}
+
+
 def build():
     """Compiles libmps.so for gfx950 (hipcc cross-compiles without a GPU)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -71,7 +85,7 @@ EXPORTS = ["mps_gpu_create", "mps_gpu_ok", "mps_gpu_set_classes", "mps_gpu_sieve
            "mps_gpu_sieve_relations",
            "mps_gpu_last_error", "mps_gpu_destroy", "mps_gpu_get_stats", "mps_gpu_reset_stats",
            "mps_gpu_configure", "mps_gpu_debug_stage", "mps_run_item", "mps_results_json", "mps_is_prime64",
-           "mps_multiplicative_order", "mps_run_bsgs"]
+           "mps_multiplicative_order", "mps_run_bsgs", "mps_gpu_measure_valu"]
 
 _lib = None
 
@@ -107,7 +121,7 @@ def lib():
     L.mps_gpu_reset_stats.restype = None
     L.mps_gpu_reset_stats.argtypes = [vp]
     L.mps_gpu_configure.restype = i32
-    L.mps_gpu_configure.argtypes = [vp, i32, u64, ctypes.c_uint32]
+    L.mps_gpu_configure.argtypes = [vp, i32, u64, ctypes.c_uint32, ctypes.c_uint32]
     L.mps_gpu_debug_stage.restype = i32
     L.mps_gpu_debug_stage.argtypes = [vp, i32, u64, u64, P(Item), vp, sz, P(sz)]
     L.mps_run_item.restype = i32
@@ -120,6 +134,8 @@ def lib():
     L.mps_multiplicative_order.argtypes = [u64, u64]
     L.mps_run_bsgs.restype = i32
     L.mps_run_bsgs.argtypes = [u64, P(WorkItem), P(Relation)]
+    L.mps_gpu_measure_valu.restype = i32
+    L.mps_gpu_measure_valu.argtypes = [i32, P(ValuRates)]
     _lib = L
     return L
 
@@ -157,8 +173,8 @@ class GpuSiever:
     def set_classes(self, on):
         lib().mps_gpu_set_classes(self.h, 1 if on else 0)
 
-    def configure(self, field=-1, pass_odds=0, mcap=0):
-        return lib().mps_gpu_configure(self.h, field, pass_odds, mcap)
+    def configure(self, field=-1, pass_odds=0, mcap=0, waves=0):
+        return lib().mps_gpu_configure(self.h, field, pass_odds, mcap, waves)
 
     def sieve_filter(self, lo, hi, item, flagged):
         """Appends what the device could not rule out in [lo|1, hi) to `flagged`; False = declined."""
