@@ -1,0 +1,87 @@
+"""CPU checks of the product's host side (no GPU needed): libmps.so loads and exports every
+symbol the C headers declare, and the exact host checks (is_prime64, run_bsgs,
+multiplicative_order) and the results.json writer agree with the oracle and the golden fixtures."""
+import json
+import os
+import random
+import re
+
+import pytest
+import sympy
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+
+with open(os.path.join(HERE, "golden", "relations_small.json")) as f:
+    GOLDEN = json.load(f)
+
+
+def declared_functions():
+    names = set()
+    for h in ("mps_gpu.h", "mps_host.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(mps_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol(mps):
+    L = mps.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing
+    assert set(mps.EXPORTS) <= names
+
+
+def test_create_without_gpu_fails_cleanly(mps):
+    """GpuSiever::init without a usable device: no handle and a 'GPU disabled' reason
+    (Gpu.hpp:1054), never a crash."""
+    import ctypes
+    g = mps.GpuSiever()
+    if g.init(0):  # a GPU is present: nothing to check here
+        g.close()
+        pytest.skip("a HIP device is present")
+    assert g.banner.startswith("GPU disabled")
+
+
+def test_host_is_prime64(mps, oracle):
+    rng = random.Random(5)
+    for _ in range(3000):
+        n = rng.randrange(2 ** rng.randrange(2, 64))
+        assert mps.is_prime64(n) == oracle.is_prime64(n) == sympy.isprime(n)
+    assert not mps.is_prime64(3825123056546413051)
+
+
+def test_host_multiplicative_order(mps, oracle):
+    rng = random.Random(6)
+    for bits in (20, 40, 62):
+        for _ in range(30):
+            p = sympy.randprime(2 ** (bits - 1), 2 ** bits)
+            b = rng.randrange(2, 1000)
+            assert mps.multiplicative_order(b, p) == oracle.multiplicative_order(b, p)
+
+
+@pytest.mark.parametrize("rec", GOLDEN[:6], ids=lambda r: ",".join(map(str, r["item"])))
+def test_host_run_bsgs_matches_golden(mps, rec):
+    b, c, P0, P1, N0, N1, mod, res = rec["item"]
+    item = mps.work_item(b, c, P0, P1, N0, N1, mod, res)
+    want = {r[0]: tuple(r) for r in rec["relations"]}
+    for p in sympy.primerange(P0, P1):
+        assert mps.run_bsgs(p, item) == want.get(p), p
+
+
+@pytest.mark.parametrize("rec", [r for r in GOLDEN if "results_json" in r], ids=lambda r: ",".join(map(str, r["item"])))
+def test_host_results_json_matches_golden(mps, rec):
+    b, c, P0, P1, N0, N1, mod, res = rec["item"]
+    item = mps.work_item(b, c, P0, P1, N0, N1, mod, res)
+    rels = [tuple(r) for r in rec["relations"]]
+    assert mps.results_json(item, rels, "2026-01-01 00:00:00 UTC+0") == rec["results_json"]
+
+
+def test_host_results_json_matches_oracle_large(mps, oracle):
+    """A long exponent range and relations with small q (many factored n)."""
+    item = mps.work_item(10, 111, 10 ** 6, 10 ** 6 + 50000, 0, 200000, 6, 4)
+    oit = oracle.make_item(10, 111, 10 ** 6, 10 ** 6 + 50000, 0, 200000, 6, 4)
+    rels, _ = oracle.process_range(oit, 10 ** 6, 10 ** 6 + 50000, 1 << 20, 4)
+    assert mps.results_json(item, rels, "t") == oracle.results_json(oit, rels, "t")
