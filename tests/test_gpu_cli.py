@@ -1,0 +1,56 @@
+"""The command-line siever (mass-prime-siever_amd/mps_siever) on the GPU, compared with the
+oracle's CPU path the way the reference's own self-test compares its GPU and CPU runs
+(kaggle/kaggle_run.py:64-95): sorted relations.p lines must be equal, with at least one relation;
+the log must say "GPU enabled"; and results.json must hold the oracle's object for the item."""
+import json
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "mass-prime-siever_amd", "mps_siever")
+
+
+def run_siever(tmp_path, lines, ini):
+    (tmp_path / "worktodo.txt").write_text("".join(l + "\n" for l in lines))
+    (tmp_path / "prime.ini").write_text(ini)
+    r = subprocess.run([BIN], cwd=tmp_path, stdin=subprocess.DEVNULL, capture_output=True, text=True, timeout=300)
+    return r
+
+
+def test_cli_self_test_item(tmp_path, oracle):
+    item = (2, -25, 10_000_000_000, 10_020_000_000, 3_600_000, 5_000_000, 2, 1)
+    r = run_siever(tmp_path, ["Factor=" + ",".join(map(str, item))],
+                   "CPU_count=0\nMemoryAllowed=16\nUseGPU=1\nGPU_device=0\nGPU_threads=2\nGPU_sieve=1\nGPU_strict=1\n")
+    print(r.stderr[-2000:], flush=True)
+    assert r.returncode == 0
+    log = (tmp_path / "siever.log").read_text()
+    assert "GPU enabled" in log
+    got = sorted((tmp_path / "relations.p").read_text().split("\n"))
+    oit = oracle.make_item(*item)
+    rels, _ = oracle.process_range(oit, item[2], item[3], 1 << 22, 8)
+    want = sorted([f"{p}, {M}, {q}" for p, M, q in rels] + [""])
+    assert got == want
+    assert len(want) > 1
+    res = json.loads((tmp_path / "results.json").read_text())
+    assert len(res) == 1
+    ours = res[0]
+    ref = json.loads(oracle.results_json(oit, rels, ours["Timestamp"]))
+    assert ours == ref
+
+
+def test_cli_two_items_append_results(tmp_path, oracle):
+    items = [(10, 111, 10 ** 9, 10 ** 9 + 2 ** 20, 0, 200_000, 6, 4), (3, -2, 10 ** 9, 10 ** 9 + 2 ** 19, 0, 100_000)]
+    r = run_siever(tmp_path, ["Factor=" + ",".join(map(str, it)) for it in items],
+                   "CPU_count=0\nMemoryAllowed=64\nUseGPU=1\nGPU_sieve=1\nGPU_classes=1\n")
+    assert r.returncode == 0, r.stderr[-2000:]
+    res = json.loads((tmp_path / "results.json").read_text())
+    assert len(res) == 2
+    for it, obj in zip(items, res):
+        full = it + (1, 0) if len(it) == 6 else it
+        oit = oracle.make_item(*full)
+        rels, _ = oracle.process_range(oit, full[2], full[3], 1 << 20, 8)
+        assert obj["prp"] == json.loads(oracle.results_json(oit, rels, "t"))["prp"]

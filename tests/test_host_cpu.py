@@ -85,3 +85,18 @@ def test_host_results_json_matches_oracle_large(mps, oracle):
     oit = oracle.make_item(10, 111, 10 ** 6, 10 ** 6 + 50000, 0, 200000, 6, 4)
     rels, _ = oracle.process_range(oit, 10 ** 6, 10 ** 6 + 50000, 1 << 20, 4)
     assert mps.results_json(item, rels, "t") == oracle.results_json(oit, rels, "t")
+
+
+def test_cli_without_gpu_fails_loudly(tmp_path, mps):
+    """No CPU sieve path: without a usable GPU the command-line siever stops with exit 1."""
+    import subprocess
+    g = mps.GpuSiever()
+    if g.init(0):
+        g.close()
+        pytest.skip("a HIP device is present")
+    binary = os.path.join(ROOT, "mass-prime-siever_amd", "mps_siever")
+    (tmp_path / "worktodo.txt").write_text("Factor=2,-25,1000000000,1001000000,3600000,5000000,2,1\n")
+    (tmp_path / "prime.ini").write_text("CPU_count=0\nMemoryAllowed=64\nUseGPU=1\n")
+    r = subprocess.run([binary], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "GPU disabled" in (tmp_path / "siever.log").read_text()
