@@ -59,6 +59,88 @@ def cpu_baseline(threads, slice_ints):
                       f"{len(rels)} relations) in {dt:.2f} s on {threads} threads"}
 
 
+def load_mix(n):
+    with open(os.path.join(ROOT, "tests", "golden", "mix_seed0.json")) as f:
+        items = json.load(f)["items"]
+    return items[:n]
+
+
+def pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, int(q * len(xs)))]
+
+
+def run_mix(mps, g, n_items, load, log):
+    """The serving mix (seed 0): the plain per-chunk worker one item after another, the batched
+    engine with every item submitted at once (throughput), and the batched engine under Poisson
+    arrivals at `load` times the plain worker's capacity (latency).  Plain-worker latencies under
+    the same arrivals follow from its measured service times (one FIFO server)."""
+    items = load_mix(n_items)
+    primes = sum(d["primes"] for d in items)
+    wis = [mps.work_item(d["b"], d["c"], d["P0"], d["P1"], d["N0"], d["N1"], d["mod"], d["res"]) for d in items]
+    # plain worker, sequential
+    svc, rel_plain = [], []
+    t0 = time.perf_counter()
+    for k, w in enumerate(wis):
+        t = time.perf_counter()
+        rels, _ = g.run_item(w, chunk_ints=CHUNK)
+        svc.append(time.perf_counter() - t)
+        rel_plain.append(rels)
+        if k % 64 == 63:
+            log(f"mix plain: {k + 1} items, {time.perf_counter() - t0:.2f} s")
+    t_plain = sum(svc)
+    # batched, all at once
+    eng = mps.Batch(g)
+    t = time.perf_counter()
+    ids = [eng.submit(w, tag=k) for k, w in enumerate(wis)]
+    eng.wait_all()
+    t_batch = time.perf_counter() - t
+    done = eng.poll(len(wis) + 16)
+    same = all(eng.relations(i) == rel_plain[k] for k, i in enumerate(ids))
+    bst = eng.stats()
+    eng.close()
+    log(f"mix batched: {t_batch:.2f} s (plain {t_plain:.2f} s), identical={same}")
+    # served: Poisson arrivals at load x plain capacity
+    rate = load * len(wis) / t_plain
+    arrivals, acc = [], 0.0
+    for d in items:
+        acc += d["gap"] / rate
+        arrivals.append(acc)
+    eng = mps.Batch(g)
+    start = time.perf_counter()
+    ids = []
+    for k, w in enumerate(wis):
+        delay = arrivals[k] - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        ids.append(eng.submit(w, tag=k))
+    eng.wait_all()
+    served = eng.poll(len(wis) + 16)
+    served_same = all(eng.relations(i) == rel_plain[k] for k, i in enumerate(ids))
+    eng.close()
+    lat = [d["t_done"] - d["t_submit"] for d in served]
+    # plain worker under the same arrivals: Lindley recursion over its service times
+    plain_lat, free_at = [], 0.0
+    for a, s_ in zip(arrivals, svc):
+        begin = max(a, free_at)
+        free_at = begin + s_
+        plain_lat.append(free_at - a)
+    log(f"mix served: p50 {pct(lat, 0.5):.3f} s p99 {pct(lat, 0.99):.3f} s, identical={served_same}")
+    return {
+        "workload": f"serving mix seed 0, {len(wis)} items, {primes} primes",
+        "primes": primes,
+        "plain_s": t_plain, "plain_primes_per_s": primes / t_plain,
+        "batched_s": t_batch, "batched_primes_per_s": primes / t_batch,
+        "speedup_batched_vs_plain": t_plain / t_batch,
+        "relations_identical": bool(same and served_same),
+        "statuses_ok": all(d["status"] == 0 for d in done + served),
+        "served_load": load, "arrival_rate_items_per_s": rate,
+        "served_p50_s": pct(lat, 0.5), "served_p99_s": pct(lat, 0.99),
+        "plain_p50_s": pct(plain_lat, 0.5), "plain_p99_s": pct(plain_lat, 0.99),
+        "batch_stats": bst,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +148,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-slice", type=int, default=1 << 26)
+    ap.add_argument("--mix", type=int, default=256, help="items of the serving mix (0 = skip the mix)")
+    ap.add_argument("--mix-load", type=float, default=0.7, help="arrival rate / plain worker capacity")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this worker runs on one GPU; --gpus must be 1 "
@@ -132,6 +216,29 @@ def main():
                    "flagged_per_step": s["flagged"] // args.steps},
         "roofline": roof,
     }
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    # the same item through the batched engine (passes of four items' worth of slots in flight)
+    eng = mps.Batch(g)
+    for _ in range(args.warmup):
+        eng.submit(item)
+    eng.wait_all()
+    eng.poll()
+    t = time.perf_counter()
+    bids = [eng.submit(item) for _ in range(args.steps)]
+    eng.wait_all()
+    ms_batched = 1000.0 * (time.perf_counter() - t) / args.steps
+    same = all(eng.relations(i) == rels for i in bids)
+    eng.close()
+    line["config"]["batched_ms_per_step"] = ms_batched
+    line["config"]["batched_primes_per_s"] = nprimes / (ms_batched / 1000.0)
+    line["config"]["batched_relations_identical"] = same
+    log(f"c2 plain {ms_per_step:.2f} ms/step, batched {ms_batched:.2f} ms/step")
+    if args.mix:
+        line["mix"] = run_mix(mps, g, args.mix, args.mix_load, log)
+        line["p50_item_latency_s"] = line["mix"]["served_p50_s"]
+        line["p99_item_latency_s"] = line["mix"]["served_p99_s"]
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(threads, args.cpu_slice)

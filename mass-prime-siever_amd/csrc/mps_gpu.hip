@@ -150,36 +150,24 @@ bool item_params(const mps_item* it, bool classes_on, uint32_t mcap, uint32_t wa
     return true;
 }
 
-}  // namespace mps
+bool plan_item(const mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max,
+               uint32_t& waves, std::vector<uint2>& csz, std::string& why) {
+    waves = g->waves;
+    if (waves == 0) {
+        // measured on MI355X: two wavefronts per entry for the class tables (m <= 1024) and short
+        // plain ranges; four when plain-mode entries run up to 2048 baby steps
+        if (!item_params(item, classes_on, g->mcap, 2, ip, m_max, csz, why)) return false;
+        waves = m_max > 1024 ? 4 : 2;
+    }
+    return item_params(item, classes_on, g->mcap, waves, ip, m_max, csz, why);
+}
 
-struct mps_gpu {
-    int device = 0;
-    bool ok = false;
-    bool classes_on = true;
-    bool broken = false;
-    int field_mode = -1;
-    uint64_t pass_odds = 1ull << 25;
-    uint32_t mcap = 2048;
-    uint32_t waves = 0;  // 0: by item (4 for plain-mode tables above 1024 baby steps, else 2)
-    std::mutex mu;
-    uint2* d_csz = nullptr;
-    uint2* h_csz = nullptr;
-    hipStream_t stream = nullptr;
-    uint32_t* d_sp = nullptr;
-    uint32_t* d_sp_r32 = nullptr;
-    uint32_t nsp = 0, nsmall = 0;
-    PassBuffers pb{};
-    uint64_t buf_odds = 0;
-    uint64_t* d_vals = nullptr;
-    size_t vals_cap = 0;
-    Counters* h_cnt = nullptr;
-    uint64_t* h_out = nullptr;
-    hipEvent_t ev[6] = {};
-    mps_gpu_stats st{};
-    std::string err;
-    std::string name;
-    int num_cu = 0;
-};
+FieldKind pass_field(const mps_gpu* g, uint64_t hi) {
+    if (g->field_mode == 0) return FIELD_MONT;
+    return hi <= (1ull << 50) ? FIELD_F64 : FIELD_MONT;
+}
+
+}  // namespace mps
 
 namespace {
 
@@ -252,10 +240,16 @@ uint32_t blocks_for(const mps_gpu* g, uint64_t n, uint32_t per_block) {
 bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, FieldKind fk, const TableGeom& tg,
               int stop_stage) {
     hipStream_t s = g->stream;
+    g->h_desc->lo = plo;
+    g->h_desc->nodd = pc;
+    g->h_desc->ip = ip;
+    g->h_desc->tg = tg;
     HIPCHK(hipMemcpyAsync(g->d_csz, g->h_csz, 49 * 32 * sizeof(uint2), hipMemcpyHostToDevice, s), "size table");
+    HIPCHK(hipMemcpyAsync(g->d_desc, g->h_desc, sizeof(PassDesc), hipMemcpyHostToDevice, s), "pass descriptor");
+    const PassDesc* d = g->d_desc;
     launch_reset(s, g->pb.cnt, g->pb.vals == nullptr ? 0u : (uint32_t)pc);
     HIPCHK(hipEventRecord(g->ev[0], s), "event");
-    if (g->pb.vals == nullptr) launch_sieve(s, plo, pc, g->d_sp, g->d_sp_r32, g->nsp, g->nsmall, g->pb, 0);
+    if (g->pb.vals == nullptr) launch_sieve(s, d, pc, g->d_sp, g->d_sp_r32, g->nsp, g->nsmall, g->pb, 0);
     HIPCHK(hipGetLastError(), "sieve launch");
     HIPCHK(hipEventRecord(g->ev[1], s), "event");
     if (!read_counts(g)) return false;
@@ -264,7 +258,7 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (stop_stage == 0 || n_surv == 0) return true;
 
     HIPCHK(hipEventRecord(g->ev[1], s), "event");
-    launch_filter(s, plo, ip, g->pb, blocks_for(g, n_surv, 256));
+    launch_filter(s, d, g->pb, blocks_for(g, n_surv, 256));
     HIPCHK(hipGetLastError(), "filter launch");
     HIPCHK(hipEventRecord(g->ev[2], s), "event");
     if (!read_counts(g)) return false;
@@ -273,7 +267,7 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (stop_stage == 1 || n_cand == 0) return true;
 
     HIPCHK(hipEventRecord(g->ev[2], s), "event");
-    launch_setup(s, fk, plo, ip, g->pb, blocks_for(g, n_cand, 256));
+    launch_setup(s, fk, ip.qmode != 0, d, g->pb, blocks_for(g, n_cand, 256));
     HIPCHK(hipGetLastError(), "setup launch");
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
     if (!read_counts(g)) return false;
@@ -282,7 +276,7 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (stop_stage == 2 || n_ent == 0) return true;
 
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
-    launch_bsgs(s, fk, tg, ip, g->pb, n_ent, 0);
+    launch_bsgs(s, fk, tg, d, g->pb, n_ent, 0);
     HIPCHK(hipGetLastError(), "bsgs launch");
     HIPCHK(hipEventRecord(g->ev[4], s), "event");
     if (!read_counts(g)) return false;
@@ -310,24 +304,12 @@ void accumulate(mps_gpu* g, uint64_t pc) {
 bool prepare_item(mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max,
                   uint32_t& waves) {
     std::vector<uint2> csz;
-    waves = g->waves;
-    if (waves == 0) {
-        // measured on MI355X: two wavefronts per entry for the class tables (m <= 1024) and short
-        // plain ranges; four when plain-mode entries run up to 2048 baby steps
-        if (!item_params(item, classes_on, g->mcap, 2, ip, m_max, csz, g->err)) return false;
-        waves = m_max > 1024 ? 4 : 2;
-    }
-    if (!item_params(item, classes_on, g->mcap, waves, ip, m_max, csz, g->err)) return false;
+    if (!plan_item(g, item, classes_on, ip, m_max, waves, csz, g->err)) return false;
     std::memcpy(g->h_csz, csz.data(), csz.size() * sizeof(uint2));
     ip.csz = g->d_csz;
     return true;
 }
 
-FieldKind pick_field(const mps_gpu* g, uint64_t hi) {
-    if (g->field_mode == 0) return FIELD_MONT;
-    if (g->field_mode == 1 && hi <= (1ull << 50)) return FIELD_F64;
-    return hi <= (1ull << 50) ? FIELD_F64 : FIELD_MONT;
-}
 
 }  // namespace
 
@@ -374,6 +356,8 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
          hipMemcpy(g->d_sp_r32, r32.data(), r32.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMalloc(&g->pb.cnt, sizeof(Counters)) == hipSuccess &&
          hipMalloc(&g->d_csz, 49 * 32 * sizeof(uint2)) == hipSuccess &&
+         hipMalloc(&g->d_desc, sizeof(PassDesc)) == hipSuccess &&
+         hipHostMalloc((void**)&g->h_desc, sizeof(PassDesc), hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc((void**)&g->h_csz, 49 * 32 * sizeof(uint2), hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc((void**)&g->h_cnt, sizeof(Counters), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
@@ -432,7 +416,7 @@ int sieve_records(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, st
     uint64_t m_max = 0;
     uint32_t waves = 0;
     if (!prepare_item(g, item, g->classes_on, ip, m_max, waves)) return MPS_DECLINED;
-    const FieldKind fk = pick_field(g, hi);
+    const FieldKind fk = pass_field(g, hi);
     const TableGeom tg = table_geom(m_max, waves);
     const uint64_t count = (hi - lo + 1) / 2;
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
@@ -520,7 +504,7 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
         g->err = "prime above 2^62";
         return MPS_DECLINED;
     }
-    const FieldKind fk = pick_field(g, pmax + 1);
+    const FieldKind fk = pass_field(g, pmax + 1);
     const TableGeom tg = table_geom(m_max, waves);
     const size_t batch = (size_t)std::min<uint64_t>(g->pass_odds, 1ull << 22);
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
@@ -584,7 +568,7 @@ int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const m
     uint64_t m_max = 0;
     uint32_t waves = 0;
     if (!prepare_item(g, item, g->classes_on, ip, m_max, waves)) return MPS_DECLINED;
-    const FieldKind fk = pick_field(g, hi);
+    const FieldKind fk = pass_field(g, hi);
     const TableGeom tg = table_geom(m_max, waves);
     const uint64_t count = (hi - lo + 1) / 2;
     if (count > g->pass_odds) {
@@ -662,6 +646,8 @@ void mps_gpu_destroy(mps_gpu* g) {
     hipHostFree(g->h_cnt);
     hipFree(g->d_csz);
     hipHostFree(g->h_csz);
+    hipFree(g->d_desc);
+    hipHostFree(g->h_desc);
     for (int i = 0; i < 6; ++i)
         if (g->ev[i]) hipEventDestroy(g->ev[i]);
     if (g->stream) hipStreamDestroy(g->stream);

@@ -67,6 +67,20 @@ struct ItemParams {
     const uint2* csz;
 };
 
+// BSGS table geometry of a pass: 2^nbbits buckets of 8 slots; a slot holds a fingerprint of
+// fpbits bits over a baby-step index of ibits bits.
+struct TableGeom {
+    uint32_t nbbits, fpbits, ibits, waves;
+};
+// Everything a pass's kernels need besides the buffers, resident in device memory so that a
+// captured graph of the pass chain can be replayed for any pass by rewriting this descriptor.
+struct PassDesc {
+    uint64_t lo;      // first odd number of the pass
+    uint64_t nodd;    // odd numbers in the pass
+    ItemParams ip;
+    TableGeom tg;
+};
+
 struct PassBuffers {
     const uint64_t* vals;   // filter path: the numbers themselves (then surv is unused), else null
     uint32_t* surv;         // survivor odd indices relative to the pass start
@@ -82,21 +96,40 @@ struct PassBuffers {
 enum FieldKind : int { FIELD_MONT = 0, FIELD_F64 = 1 };
 
 void launch_reset(hipStream_t s, Counters* cnt, uint32_t n_surv0);
-void launch_sieve(hipStream_t s, uint64_t lo, uint64_t nodd, const uint32_t* sp, const uint32_t* sp_r32,
+void launch_sieve(hipStream_t s, const PassDesc* d, uint64_t nodd, const uint32_t* sp, const uint32_t* sp_r32,
                   uint32_t nsp, uint32_t nsmall, const PassBuffers& pb, uint32_t grid_override);
-void launch_filter(hipStream_t s, uint64_t lo, const ItemParams& ip, const PassBuffers& pb, uint32_t grid);
-void launch_setup(hipStream_t s, FieldKind fk, uint64_t lo, const ItemParams& ip, const PassBuffers& pb,
-                  uint32_t grid);
-// BSGS table geometry of a pass: 2^nbbits buckets of 8 slots; a slot holds a fingerprint of
-// fpbits bits over a baby-step index of ibits bits.
-struct TableGeom {
-    uint32_t nbbits, fpbits, ibits, waves;
-};
+void launch_filter(hipStream_t s, const PassDesc* d, const PassBuffers& pb, uint32_t grid);
+void launch_setup(hipStream_t s, FieldKind fk, bool qmode, const PassDesc* d, const PassBuffers& pb, uint32_t grid);
 TableGeom table_geom(uint64_t m_max, uint64_t waves);
 constexpr uint32_t kBsgsWaves = 4;                // default wavefronts of one BSGS workgroup (one entry)
 size_t bsgs_lds_bytes(const TableGeom& tg);
 
-void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const ItemParams& ip, const PassBuffers& pb,
+void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const PassDesc* d, const PassBuffers& pb,
                  uint32_t n_ent, uint32_t grid);
+// Workgroups of this BSGS variant resident on the device at once (cus compute units): the grid
+// of a launch that drains however many entries the pass produced.
+uint32_t bsgs_persistent_grid(FieldKind fk, const TableGeom& tg, int cus);
+
+// One pass as a fixed chain: descriptor upload, reset, sieve, filter, setup, BSGS, counter and
+// record download.  Every grid is fixed (the kernels read their element counts on the device).
+struct PassChain {
+    PassDesc* d_desc;
+    const PassDesc* h_desc;   // pinned; read when the chain runs
+    Counters* h_cnt;          // pinned
+    uint64_t* h_out;          // pinned, out_words words
+    size_t out_words;
+    const uint32_t* sp;
+    const uint32_t* sp_r32;
+    uint32_t nsp, nsmall;
+    PassBuffers pb;
+    uint32_t grid_sieve, grid_stage, grid_bsgs;
+    FieldKind fk;
+    bool qmode;
+    TableGeom tg;
+};
+// Enqueues the chain on stream s.
+hipError_t enqueue_pass_chain(hipStream_t s, const PassChain& c);
+// Builds the chain as a graph of explicit nodes (no stream capture).
+hipError_t build_pass_graph(const PassChain& c, hipGraph_t* out);
 
 }  // namespace mps

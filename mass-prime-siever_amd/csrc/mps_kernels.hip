@@ -11,6 +11,8 @@
 //   k_bsgs    one wavefront per entry: baby steps t*B^i into an LDS fingerprint table, giant steps
 //             G_j*(B^m)^g looked up; a hit appends p to the flagged list.
 // Numbers the device cannot decide (p < 1000, p | b) are appended to the flagged list directly.
+#include <algorithm>
+
 #include "mps_device.hpp"
 #include "mps_kernels.h"
 
@@ -84,9 +86,10 @@ __device__ __forceinline__ uint32_t seg_start(uint64_t v0, uint32_t q, uint32_t 
     return j;
 }
 
-__global__ __launch_bounds__(kSieveThreads) void k_sieve(uint64_t lo, uint64_t nodd, const uint32_t* __restrict__ sp,
+__global__ __launch_bounds__(kSieveThreads) void k_sieve(const PassDesc* __restrict__ d, const uint32_t* __restrict__ sp,
                                                          const uint32_t* __restrict__ sp_r32, uint32_t nsp,
                                                          uint32_t nsmall, PassBuffers pb) {
+    const uint64_t lo = d->lo, nodd = d->nodd;
     __shared__ uint32_t bits[kSegOdds / 32];
     __shared__ uint32_t st[kSmallPrimeBound / 2];
     __shared__ uint32_t wtot[kSieveThreads / 64];
@@ -252,7 +255,9 @@ __device__ __forceinline__ void filter_class(uint64_t p, const ItemParams& ip, b
     info = mask;
 }
 
-__global__ __launch_bounds__(256) void k_filter(uint64_t lo, ItemParams ip, PassBuffers pb) {
+__global__ __launch_bounds__(256) void k_filter(const PassDesc* __restrict__ d, PassBuffers pb) {
+    const uint64_t lo = d->lo;
+    const ItemParams& ip = d->ip;  // read in place: uniform fields load as scalars, tables by index
     __shared__ uint32_t sh[5];
     const uint32_t n = min(pb.cnt->n_surv, pb.surv_cap);
     const uint32_t lane = lane_id();
@@ -313,7 +318,9 @@ __device__ __forceinline__ typename F::T write_pow_tables(const F& f, typename F
 __device__ __forceinline__ uint32_t bitlen64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
 template <class F, bool QMODE>
-__global__ __launch_bounds__(256) void k_setup(uint64_t lo, ItemParams ip, PassBuffers pb) {
+__global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, PassBuffers pb) {
+    const uint64_t lo = d->lo;
+    const ItemParams& ip = d->ip;  // read in place: uniform fields load as scalars, tables by index
     typedef typename F::T T;
     __shared__ uint32_t sh[5];
     const uint32_t n = min(pb.cnt->n_cand, pb.cand_cap);
@@ -629,7 +636,9 @@ __device__ __forceinline__ bool mr_prime(uint64_t n);
 // One workgroup of WB wavefronts per entry: the table lives in LDS once per entry and every lane
 // of the group takes baby steps i == tid and giant steps g == tid (mod 64 * WB).
 template <class F, int WB>
-__global__ __launch_bounds__(64 * WB) void k_bsgs(ItemParams ip, PassBuffers pb, TableGeom tg) {
+__global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d, PassBuffers pb) {
+    const uint64_t item_N1 = d->ip.N1, item_cm = d->ip.cm;
+    const TableGeom tg = d->tg;
     typedef typename F::T T;
     constexpr uint32_t L = 64 * WB;
     extern __shared__ uint32_t lds_tab[];
@@ -642,6 +651,8 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(ItemParams ip, PassBuffers pb,
     Tb.cnt = reinterpret_cast<uint32_t*>(Tb.idx + nbk * 8u);
     Tb.nbmask = nbk - 1u;
     const uint32_t n = min(pb.cnt->n_ent, pb.ent_cap);
+    // entries blockIdx.x, blockIdx.x + gridDim.x, ...: no shared claim counter (one same-address
+    // atomic per entry costs more than a short entry's whole BSGS)
     for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
         BsgsEntry<F> be;
         be.E = pb.ent + (uint64_t)e * kEntWords;
@@ -688,7 +699,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(ItemParams ip, PassBuffers pb,
             const bool qmode = (be.meta >> 10) & 1u;
             const uint64_t stride = qmode ? 1 : ((be.meta >> 8) & 1u) + 1;
             const uint64_t parity = qmode ? 0 : (be.meta >> 9) & 1u;
-            const uint64_t cmul = qmode ? be.qp : ip.cm;
+            const uint64_t cmul = qmode ? be.qp : item_cm;
             T mstep;
             const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(E[E_BM64]), tid, mstep);
             uint64_t best = ~0ull;
@@ -697,7 +708,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(ItemParams ip, PassBuffers pb,
                 if (qmode) {
                     const uint32_t d = (uint32_t)(E[E_DJ + (j >> 3)] >> (8 * (j & 7))) & 0xFFu;
                     nj = be.nb + d;
-                    Kj = (ip.N1 - nj + be.qp - 1) / be.qp;
+                    Kj = (item_N1 - nj + be.qp - 1) / be.qp;
                 } else {
                     nj = be.nb;
                     Kj = be.klim;
@@ -767,28 +778,27 @@ void launch_reset(hipStream_t s, Counters* cnt, uint32_t n_surv0) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, cnt, n_surv0);
 }
 
-void launch_sieve(hipStream_t s, uint64_t lo, uint64_t nodd, const uint32_t* sp, const uint32_t* sp_r32,
+void launch_sieve(hipStream_t s, const PassDesc* d, uint64_t nodd, const uint32_t* sp, const uint32_t* sp_r32,
                   uint32_t nsp, uint32_t nsmall, const PassBuffers& pb, uint32_t grid_override) {
     const uint32_t grid = grid_override ? grid_override : (uint32_t)((nodd + kSegOdds - 1) / kSegOdds);
-    hipLaunchKernelGGL(k_sieve, dim3(grid), dim3(kSieveThreads), 0, s, lo, nodd, sp, sp_r32, nsp, nsmall, pb);
+    hipLaunchKernelGGL(k_sieve, dim3(grid), dim3(kSieveThreads), 0, s, d, sp, sp_r32, nsp, nsmall, pb);
 }
 
-void launch_filter(hipStream_t s, uint64_t lo, const ItemParams& ip, const PassBuffers& pb, uint32_t grid) {
-    hipLaunchKernelGGL(k_filter, dim3(grid), dim3(256), 0, s, lo, ip, pb);
+void launch_filter(hipStream_t s, const PassDesc* d, const PassBuffers& pb, uint32_t grid) {
+    hipLaunchKernelGGL(k_filter, dim3(grid), dim3(256), 0, s, d, pb);
 }
 
-void launch_setup(hipStream_t s, FieldKind fk, uint64_t lo, const ItemParams& ip, const PassBuffers& pb,
-                  uint32_t grid) {
+void launch_setup(hipStream_t s, FieldKind fk, bool qmode, const PassDesc* d, const PassBuffers& pb, uint32_t grid) {
     if (fk == FIELD_F64) {
-        if (ip.qmode)
-            hipLaunchKernelGGL((k_setup<F64Field, true>), dim3(grid), dim3(256), 0, s, lo, ip, pb);
+        if (qmode)
+            hipLaunchKernelGGL((k_setup<F64Field, true>), dim3(grid), dim3(256), 0, s, d, pb);
         else
-            hipLaunchKernelGGL((k_setup<F64Field, false>), dim3(grid), dim3(256), 0, s, lo, ip, pb);
+            hipLaunchKernelGGL((k_setup<F64Field, false>), dim3(grid), dim3(256), 0, s, d, pb);
     } else {
-        if (ip.qmode)
-            hipLaunchKernelGGL((k_setup<MontField, true>), dim3(grid), dim3(256), 0, s, lo, ip, pb);
+        if (qmode)
+            hipLaunchKernelGGL((k_setup<MontField, true>), dim3(grid), dim3(256), 0, s, d, pb);
         else
-            hipLaunchKernelGGL((k_setup<MontField, false>), dim3(grid), dim3(256), 0, s, lo, ip, pb);
+            hipLaunchKernelGGL((k_setup<MontField, false>), dim3(grid), dim3(256), 0, s, d, pb);
     }
 }
 
@@ -806,27 +816,146 @@ TableGeom table_geom(uint64_t m_max, uint64_t waves) {
 // fingerprints (16 B) + indices (16 B) + a fill counter (4 B) per bucket
 size_t bsgs_lds_bytes(const TableGeom& tg) { return (size_t)36 << tg.nbbits; }
 
-// n_ent: entries, one workgroup each (0 = a persistent grid of `grid` workgroups).
+// n_ent: entries, one workgroup each (0 = a persistent grid of `grid` workgroups that drains the
+// pass's entries).
 template <class F>
-void launch_bsgs_f(hipStream_t s, const TableGeom& tg, const ItemParams& ip, const PassBuffers& pb, uint32_t grid,
+void launch_bsgs_f(hipStream_t s, const TableGeom& tg, const PassDesc* d, const PassBuffers& pb, uint32_t grid,
                    size_t lds) {
     if (tg.waves == 1)
-        hipLaunchKernelGGL((k_bsgs<F, 1>), dim3(grid), dim3(64), lds, s, ip, pb, tg);
+        hipLaunchKernelGGL((k_bsgs<F, 1>), dim3(grid), dim3(64), lds, s, d, pb);
     else if (tg.waves == 2)
-        hipLaunchKernelGGL((k_bsgs<F, 2>), dim3(grid), dim3(128), lds, s, ip, pb, tg);
+        hipLaunchKernelGGL((k_bsgs<F, 2>), dim3(grid), dim3(128), lds, s, d, pb);
     else
-        hipLaunchKernelGGL((k_bsgs<F, 4>), dim3(grid), dim3(256), lds, s, ip, pb, tg);
+        hipLaunchKernelGGL((k_bsgs<F, 4>), dim3(grid), dim3(256), lds, s, d, pb);
 }
 
-void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const ItemParams& ip, const PassBuffers& pb,
+template <class F>
+int bsgs_blocks_per_cu(const TableGeom& tg, size_t lds) {
+    int nb = 0;
+    hipError_t e;
+    if (tg.waves == 1)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bsgs<F, 1>, 64, lds);
+    else if (tg.waves == 2)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bsgs<F, 2>, 128, lds);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_bsgs<F, 4>, 256, lds);
+    return e == hipSuccess ? nb : 0;
+}
+
+uint32_t bsgs_persistent_grid(FieldKind fk, const TableGeom& tg, int cus) {
+    const size_t lds = bsgs_lds_bytes(tg);
+    int nb = fk == FIELD_F64 ? bsgs_blocks_per_cu<F64Field>(tg, lds) : bsgs_blocks_per_cu<MontField>(tg, lds);
+    if (nb <= 0) nb = 1;
+    return (uint32_t)nb * (uint32_t)std::max(1, cus);
+}
+
+// The BSGS kernel of a variant and its workgroup size.
+const void* bsgs_kernel(FieldKind fk, uint32_t waves, uint32_t& threads) {
+    threads = 64 * (waves == 1 ? 1 : waves == 2 ? 2 : 4);
+    if (fk == FIELD_F64)
+        return waves == 1   ? reinterpret_cast<const void*>(&k_bsgs<F64Field, 1>)
+               : waves == 2 ? reinterpret_cast<const void*>(&k_bsgs<F64Field, 2>)
+                            : reinterpret_cast<const void*>(&k_bsgs<F64Field, 4>);
+    return waves == 1   ? reinterpret_cast<const void*>(&k_bsgs<MontField, 1>)
+           : waves == 2 ? reinterpret_cast<const void*>(&k_bsgs<MontField, 2>)
+                        : reinterpret_cast<const void*>(&k_bsgs<MontField, 4>);
+}
+
+const void* setup_kernel(FieldKind fk, bool qmode) {
+    if (fk == FIELD_F64)
+        return qmode ? reinterpret_cast<const void*>(&k_setup<F64Field, true>)
+                     : reinterpret_cast<const void*>(&k_setup<F64Field, false>);
+    return qmode ? reinterpret_cast<const void*>(&k_setup<MontField, true>)
+                 : reinterpret_cast<const void*>(&k_setup<MontField, false>);
+}
+
+hipError_t enqueue_pass_chain(hipStream_t s, const PassChain& c) {
+    hipError_t e = hipMemcpyAsync(c.d_desc, c.h_desc, sizeof(PassDesc), hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return e;
+    launch_reset(s, c.pb.cnt, 0);
+    launch_sieve(s, c.d_desc, 0, c.sp, c.sp_r32, c.nsp, c.nsmall, c.pb, c.grid_sieve);
+    launch_filter(s, c.d_desc, c.pb, c.grid_stage);
+    launch_setup(s, c.fk, c.qmode, c.d_desc, c.pb, c.grid_stage);
+    launch_bsgs(s, c.fk, c.tg, c.d_desc, c.pb, 0, c.grid_bsgs);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(c.h_cnt, c.pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(c.h_out, c.pb.out, c.out_words * sizeof(uint64_t), hipMemcpyDeviceToHost, s);
+}
+
+namespace {
+
+hipError_t add_kernel(hipGraph_t g, hipGraphNode_t& prev, const void* fn, uint32_t grid, uint32_t threads,
+                      uint32_t lds, void** args) {
+    hipKernelNodeParams kp{};
+    kp.func = const_cast<void*>(fn);
+    kp.gridDim = dim3(grid);
+    kp.blockDim = dim3(threads);
+    kp.sharedMemBytes = lds;
+    kp.kernelParams = args;
+    kp.extra = nullptr;
+    hipGraphNode_t node;
+    const hipError_t e = hipGraphAddKernelNode(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, &kp);
+    if (e == hipSuccess) prev = node;
+    return e;
+}
+
+hipError_t add_copy(hipGraph_t g, hipGraphNode_t& prev, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    hipGraphNode_t node;
+    const hipError_t e = hipGraphAddMemcpyNode1D(&node, g, prev ? &prev : nullptr, prev ? 1 : 0, dst, src, bytes, kind);
+    if (e == hipSuccess) prev = node;
+    return e;
+}
+
+}  // namespace
+
+hipError_t build_pass_graph(const PassChain& c, hipGraph_t* out) {
+    hipGraph_t g = nullptr;
+    hipError_t e = hipGraphCreate(&g, 0);
+    if (e != hipSuccess) return e;
+    hipGraphNode_t prev = nullptr;
+    // kernel arguments by address, in declaration order
+    const PassDesc* d = c.d_desc;
+    Counters* cnt = c.pb.cnt;
+    uint32_t zero = 0;
+    PassBuffers pb = c.pb;
+    const uint32_t* sp = c.sp;
+    const uint32_t* sp_r32 = c.sp_r32;
+    uint32_t nsp = c.nsp, nsmall = c.nsmall;
+    void* a_reset[] = {&cnt, &zero};
+    void* a_sieve[] = {&d, &sp, &sp_r32, &nsp, &nsmall, &pb};
+    void* a_stage[] = {&d, &pb};
+    uint32_t bsgs_threads = 0;
+    const void* bsgs = bsgs_kernel(c.fk, c.tg.waves, bsgs_threads);
+    if (e == hipSuccess) e = add_copy(g, prev, c.d_desc, c.h_desc, sizeof(PassDesc), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = add_kernel(g, prev, reinterpret_cast<const void*>(&k_reset), 1, 64, 0, a_reset);
+    if (e == hipSuccess)
+        e = add_kernel(g, prev, reinterpret_cast<const void*>(&k_sieve), c.grid_sieve, kSieveThreads, 0, a_sieve);
+    if (e == hipSuccess) e = add_kernel(g, prev, reinterpret_cast<const void*>(&k_filter), c.grid_stage, 256, 0, a_stage);
+    if (e == hipSuccess) e = add_kernel(g, prev, setup_kernel(c.fk, c.qmode), c.grid_stage, 256, 0, a_stage);
+    if (e == hipSuccess)
+        e = add_kernel(g, prev, bsgs, c.grid_bsgs, bsgs_threads, (uint32_t)bsgs_lds_bytes(c.tg), a_stage);
+    if (e == hipSuccess) e = add_copy(g, prev, c.h_cnt, c.pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost);
+    if (e == hipSuccess)
+        e = add_copy(g, prev, c.h_out, c.pb.out, c.out_words * sizeof(uint64_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return e;
+    }
+    *out = g;
+    return hipSuccess;
+}
+
+void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const PassDesc* d, const PassBuffers& pb,
                  uint32_t n_ent, uint32_t grid) {
     const size_t lds = bsgs_lds_bytes(tg);
     if (n_ent) grid = n_ent;
     if (grid == 0) return;
     if (fk == FIELD_F64)
-        launch_bsgs_f<F64Field>(s, tg, ip, pb, grid, lds);
+        launch_bsgs_f<F64Field>(s, tg, d, pb, grid, lds);
     else
-        launch_bsgs_f<MontField>(s, tg, ip, pb, grid, lds);
+        launch_bsgs_f<MontField>(s, tg, d, pb, grid, lds);
 }
 
 }  // namespace mps

@@ -1,11 +1,47 @@
 // Helpers shared by the plain (mps_gpu.hip) and batched (mps_batch.hip) device workers.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "mps_gpu.h"
 #include "mps_kernels.h"
+
+// One device handle: the plain worker's stream and pass buffers, and the device-resident sieving
+// primes that the batched worker shares.
+struct mps_gpu {
+    int device = 0;
+    bool ok = false;
+    bool classes_on = true;
+    bool broken = false;
+    int field_mode = -1;
+    uint64_t pass_odds = 1ull << 25;
+    uint32_t mcap = 2048;
+    uint32_t waves = 0;  // 0: by item (4 for plain-mode tables above 1024 baby steps, else 2)
+    std::mutex mu;
+    uint2* d_csz = nullptr;
+    uint2* h_csz = nullptr;
+    mps::PassDesc* d_desc = nullptr;
+    mps::PassDesc* h_desc = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t* d_sp = nullptr;
+    uint32_t* d_sp_r32 = nullptr;
+    uint32_t nsp = 0, nsmall = 0;
+    mps::PassBuffers pb{};
+    uint64_t buf_odds = 0;
+    uint64_t* d_vals = nullptr;
+    size_t vals_cap = 0;
+    mps::Counters* h_cnt = nullptr;
+    uint64_t* h_out = nullptr;
+    hipEvent_t ev[6] = {};
+    mps_gpu_stats st{};
+    std::string err;
+    std::string name;
+    int num_cu = 0;
+};
 
 namespace mps {
 
@@ -15,5 +51,12 @@ namespace mps {
 // Returns false, with the reason, when the BSGS table would not fit.
 bool item_params(const mps_item* it, bool classes_on, uint32_t mcap, uint32_t waves, ItemParams& ip,
                  uint64_t& m_max, std::vector<uint2>& csz, std::string& why);
+
+// Item constants and table choices under the handle's tuning (mcap, waves: 0 = by item).
+bool plan_item(const mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max,
+               uint32_t& waves, std::vector<uint2>& csz, std::string& why);
+
+// Field of a pass ending below hi under the handle's setting.
+FieldKind pass_field(const mps_gpu* g, uint64_t hi);
 
 }  // namespace mps

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "mps_host.h"
+#include "mps_host_internal.h"
 
 namespace {
 
@@ -42,9 +43,49 @@ uint64_t mod_i64(int64_t v, uint64_t m) {
     return (uint64_t)(r < 0 ? r + (i128)m : r);
 }
 
-// One chunk [lo, hi) (Worker.hpp:216-228 and :270-280): the device returns exact relations
-// (p, M) of the primes it decided and the numbers it leaves to the host; the host adds
-// q = ord_p(b) to the former and runs the exact is_prime64 + run_bsgs on the latter.
+}  // namespace
+
+namespace mps {
+
+// The host part of one chunk [lo, hi) (Worker.hpp:223-228 and :270-280): the device returned exact
+// relations (p, M) of the primes it decided and the numbers it leaves to the host; the host adds
+// q = ord_p(b) to the former and runs the exact is_prime64 + run_bsgs on the latter.  Sorts recs.
+// Returns MPS_ERROR if a device relation fails the direct check b^M == -c (mod p).
+int finish_records(const mps_work_item* it, uint64_t lo, uint64_t hi, mps_dev_relation* recs, size_t n,
+                   std::vector<mps_relation>& rels, uint64_t* prime_flagged) {
+    int status = MPS_OK;
+    std::sort(recs, recs + n,
+              [](const mps_dev_relation& a, const mps_dev_relation& b) { return a.p < b.p || (a.p == b.p && a.M < b.M); });
+    for (size_t i = 0; i < n; ++i) {
+        if (i > 0 && recs[i].p == recs[i - 1].p) continue;
+        const uint64_t p = recs[i].p;
+        if (p < lo || p >= hi) continue;
+        mps_relation rel;
+        if (recs[i].M == MPS_HOST_DECIDES) {
+            if (!mps_is_prime64(p)) continue;
+            if (prime_flagged) ++*prime_flagged;
+            if (mps_run_bsgs(p, it, &rel)) rels.push_back(rel);
+            continue;
+        }
+        if (prime_flagged) ++*prime_flagged;
+        // the device verified the equation exactly; check it once more before writing it
+        if (powmod_u64(mod_i64(it->b, p), recs[i].M, p) != (p - mod_i64(it->c, p)) % p) {
+            status = MPS_ERROR;
+            continue;
+        }
+        rel.p = p;
+        rel.M = recs[i].M;
+        rel.q = mps_multiplicative_order(mod_i64(it->b, p), p);
+        rels.push_back(rel);
+    }
+    return status;
+}
+
+}  // namespace mps
+
+namespace {
+
+// One chunk on the plain worker: device pass(es), then finish_records.
 void process_chunk(mps_gpu* g, const mps_work_item* it, uint64_t lo, uint64_t hi, std::vector<mps_dev_relation>& buf,
                    ChunkResult& r) {
     const mps_item di = {it->b, it->c, it->N0, it->N1, it->cls_mod, it->cls_res};
@@ -65,31 +106,8 @@ void process_chunk(mps_gpu* g, const mps_work_item* it, uint64_t lo, uint64_t hi
         r.status = st;
         return;
     }
-    std::sort(buf.begin(), buf.begin() + n,
-              [](const mps_dev_relation& a, const mps_dev_relation& b) { return a.p < b.p || (a.p == b.p && a.M < b.M); });
     r.flagged += n;
-    for (size_t i = 0; i < n; ++i) {
-        if (i > 0 && buf[i].p == buf[i - 1].p) continue;
-        const uint64_t p = buf[i].p;
-        if (p < lo || p >= hi) continue;
-        mps_relation rel;
-        if (buf[i].M == MPS_HOST_DECIDES) {
-            if (!mps_is_prime64(p)) continue;
-            r.prime_flagged++;
-            if (mps_run_bsgs(p, it, &rel)) r.rels.push_back(rel);
-            continue;
-        }
-        r.prime_flagged++;
-        // the device verified the equation exactly; check it once more before writing it
-        if (powmod_u64(mod_i64(it->b, p), buf[i].M, p) != (p - mod_i64(it->c, p)) % p) {
-            r.status = MPS_ERROR;
-            continue;
-        }
-        rel.p = p;
-        rel.M = buf[i].M;
-        rel.q = mps_multiplicative_order(mod_i64(it->b, p), p);
-        r.rels.push_back(rel);
-    }
+    if (mps::finish_records(it, lo, hi, buf.data(), n, r.rels, &r.prime_flagged) != MPS_OK) r.status = MPS_ERROR;
     r.chk_s += now_s() - t1;
 }
 

@@ -62,6 +62,30 @@ class GpuStats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class BatchOpts(ctypes.Structure):
+    """mps_batch_opts (include/mps_batch.h)."""
+    _fields_ = [("slots", ctypes.c_int), ("pass_odds", ctypes.c_uint64), ("use_graphs", ctypes.c_int),
+                ("policy", ctypes.c_int), ("max_active", ctypes.c_int), ("mem_budget", ctypes.c_uint64)]
+
+
+class ItemDone(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("tag", ctypes.c_uint64), ("status", ctypes.c_int),
+                ("relations", ctypes.c_uint64), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("primes_flagged", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("passes", "items_submitted", "items_done", "graph_launches",
+                                               "graphs_built")] + \
+               [(n, ctypes.c_double) for n in ("busy_s", "device_s", "host_s")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class ValuRates(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("fma64_per_s", "mul_lo32_per_s", "mul_hi32_per_s",
                                                "mad_u64_u32_per_s", "add32_per_s", "fma64_latency_cycles",
@@ -85,7 +109,9 @@ EXPORTS = ["mps_gpu_create", "mps_gpu_ok", "mps_gpu_set_classes", "mps_gpu_sieve
            "mps_gpu_sieve_relations",
            "mps_gpu_last_error", "mps_gpu_destroy", "mps_gpu_get_stats", "mps_gpu_reset_stats",
            "mps_gpu_configure", "mps_gpu_debug_stage", "mps_run_item", "mps_results_json", "mps_is_prime64",
-           "mps_multiplicative_order", "mps_run_bsgs", "mps_gpu_measure_valu"]
+           "mps_multiplicative_order", "mps_run_bsgs", "mps_gpu_measure_valu", "mps_batch_create",
+           "mps_batch_submit", "mps_batch_poll", "mps_batch_wait_all", "mps_batch_relations", "mps_batch_clock",
+           "mps_batch_get_stats", "mps_batch_last_error", "mps_batch_destroy"]
 
 _lib = None
 
@@ -136,6 +162,24 @@ def lib():
     L.mps_run_bsgs.argtypes = [u64, P(WorkItem), P(Relation)]
     L.mps_gpu_measure_valu.restype = i32
     L.mps_gpu_measure_valu.argtypes = [i32, P(ValuRates)]
+    L.mps_batch_create.restype = vp
+    L.mps_batch_create.argtypes = [vp, P(BatchOpts)]
+    L.mps_batch_submit.restype = ctypes.c_int64
+    L.mps_batch_submit.argtypes = [vp, P(WorkItem), u64]
+    L.mps_batch_poll.restype = i32
+    L.mps_batch_poll.argtypes = [vp, P(ItemDone), sz, P(sz)]
+    L.mps_batch_wait_all.restype = i32
+    L.mps_batch_wait_all.argtypes = [vp]
+    L.mps_batch_relations.restype = i32
+    L.mps_batch_relations.argtypes = [vp, u64, vp, sz, P(sz)]
+    L.mps_batch_clock.restype = ctypes.c_double
+    L.mps_batch_clock.argtypes = [vp]
+    L.mps_batch_get_stats.restype = i32
+    L.mps_batch_get_stats.argtypes = [vp, P(BatchStats)]
+    L.mps_batch_last_error.restype = ctypes.c_char_p
+    L.mps_batch_last_error.argtypes = [vp]
+    L.mps_batch_destroy.restype = None
+    L.mps_batch_destroy.argtypes = [vp]
     _lib = L
     return L
 
@@ -269,6 +313,68 @@ class GpuSiever:
     def close(self):
         if self.h:
             lib().mps_gpu_destroy(self.h)
+            self.h = None
+
+
+class Batch:
+    """The batched / served engine (include/mps_batch.h) on a GpuSiever's device.
+
+    submit() queues an item and returns at once; poll() returns the items finished since the last
+    call; relations(id) the sorted (p, M, q) of a finished item.  Items admitted together share
+    the device's slots (continuous batching)."""
+
+    def __init__(self, siever, slots=0, pass_odds=0, use_graphs=True, policy=0, max_active=0, mem_budget=0):
+        self.opts = BatchOpts(slots, pass_odds, 1 if use_graphs else 0, policy, max_active, mem_budget)
+        self.h = lib().mps_batch_create(siever.h, ctypes.byref(self.opts))
+        if not self.h:
+            raise RuntimeError(f"batch engine: {siever.last_error()}")
+        self.items = {}
+
+    def submit(self, item, tag=0):
+        i = lib().mps_batch_submit(self.h, ctypes.byref(item), tag)
+        if i < 0:
+            raise RuntimeError("batch submit failed")
+        self.items[i] = item
+        return i
+
+    def poll(self, cap=1024):
+        arr = (ItemDone * cap)()
+        n = ctypes.c_size_t()
+        lib().mps_batch_poll(self.h, arr, cap, ctypes.byref(n))
+        return [arr[i].as_dict() for i in range(n.value)]
+
+    def wait_all(self):
+        rc = lib().mps_batch_wait_all(self.h)
+        if rc != MPS_OK:
+            raise RuntimeError(f"batch engine: {self.last_error()}")
+
+    def relations(self, item_id):
+        cap = 1 << 12
+        while True:
+            rels = (Relation * cap)()
+            n = ctypes.c_size_t()
+            rc = lib().mps_batch_relations(self.h, item_id, rels, cap, ctypes.byref(n))
+            if rc == MPS_CAPACITY:
+                cap = n.value + 16
+                continue
+            if rc != MPS_OK:
+                raise RuntimeError(f"relations of item {item_id}: status {rc}")
+            return [(rels[i].p, rels[i].M, rels[i].q) for i in range(n.value)]
+
+    def clock(self):
+        return lib().mps_batch_clock(self.h)
+
+    def stats(self):
+        s = BatchStats()
+        lib().mps_batch_get_stats(self.h, ctypes.byref(s))
+        return s.as_dict()
+
+    def last_error(self):
+        return lib().mps_batch_last_error(self.h).decode() if self.h else "no engine"
+
+    def close(self):
+        if self.h:
+            lib().mps_batch_destroy(self.h)
             self.h = None
 
 

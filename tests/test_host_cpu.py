@@ -18,7 +18,7 @@ with open(os.path.join(HERE, "golden", "relations_small.json")) as f:
 
 def declared_functions():
     names = set()
-    for h in ("mps_gpu.h", "mps_host.h"):
+    for h in ("mps_gpu.h", "mps_host.h", "mps_batch.h"):
         text = open(os.path.join(ROOT, "include", h)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         names |= set(re.findall(r"\b(mps_[a-z0-9_]+)\s*\(", text))
