@@ -70,6 +70,19 @@ def pct(xs, q):
     return xs[min(len(xs) - 1, int(q * len(xs)))]
 
 
+def drain(eng, n, log, label):
+    """Finished-item records of an engine until n have finished, with a progress line every few
+    seconds (the engine runs on its own threads; this thread only polls)."""
+    done, t0, last = [], time.perf_counter(), time.perf_counter()
+    while len(done) < n:
+        time.sleep(0.002)
+        done += eng.poll()
+        if time.perf_counter() - last > 5:
+            last = time.perf_counter()
+            log(f"{label}: {len(done)}/{n} items, {last - t0:.1f} s")
+    return done
+
+
 def run_mix(mps, g, n_items, load, log):
     """The serving mix (seed 0): the plain per-chunk worker one item after another, the batched
     engine with every item submitted at once (throughput), and the batched engine under Poisson
@@ -93,9 +106,8 @@ def run_mix(mps, g, n_items, load, log):
     eng = mps.Batch(g)
     t = time.perf_counter()
     ids = [eng.submit(w, tag=k) for k, w in enumerate(wis)]
-    eng.wait_all()
+    done = drain(eng, len(wis), log, "mix batched")
     t_batch = time.perf_counter() - t
-    done = eng.poll(len(wis) + 16)
     same = all(eng.relations(i) == rel_plain[k] for k, i in enumerate(ids))
     bst = eng.stats()
     eng.close()
@@ -114,8 +126,9 @@ def run_mix(mps, g, n_items, load, log):
         if delay > 0:
             time.sleep(delay)
         ids.append(eng.submit(w, tag=k))
-    eng.wait_all()
-    served = eng.poll(len(wis) + 16)
+        if k % 64 == 63:
+            log(f"mix served: {k + 1} items submitted")
+    served = drain(eng, len(wis), log, "mix served")
     served_same = all(eng.relations(i) == rel_plain[k] for k, i in enumerate(ids))
     eng.close()
     lat = [d["t_done"] - d["t_submit"] for d in served]
@@ -147,9 +160,11 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-slice", type=int, default=1 << 26)
+    ap.add_argument("--cpu-slice", type=int, default=1 << 27)
     ap.add_argument("--mix", type=int, default=256, help="items of the serving mix (0 = skip the mix)")
     ap.add_argument("--mix-load", type=float, default=0.7, help="arrival rate / plain worker capacity")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB streaming copy (k_copy), the byte-count reference of HBM counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this worker runs on one GPU; --gpus must be 1 "
@@ -163,6 +178,7 @@ def main():
         raise SystemExit(g.banner)
     item = mps.work_item(w["b"], w["c"], w["P0"], w["P1"], w["N0"], w["N1"], w["mod"], w["res"])
     nprimes = prime_count(w["P0"], w["P1"])
+    copy_gbps = mps.copy_probe(1 << 30, 3) if args.copy_probe else None
 
     for _ in range(args.warmup):
         g.run_item(item, chunk_ints=CHUNK)
@@ -183,11 +199,11 @@ def main():
     products_launch = s["w_bsgs"] / launches
     if s["field"] == 1:
         achieved = products_launch * FLOP_PER_PRODUCT_F64 / (ms_launch * 1e-3) / 1e12
-        roof = {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
     else:
         achieved = products_launch * MUL_PER_PRODUCT_MONT / (ms_launch * 1e-3) / 1e12
-        roof = {"bound": "valu-int-mul", "achieved": achieved, "peak": PEAK_INT_MUL_TOPS, "unit": "TOP/s",
+        roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_MUL_TOPS, "unit": "TOP/s",
                 "frac": achieved / PEAK_INT_MUL_TOPS, "traffic": None}
     roof.update({"kernel": "k_bsgs", "ms_per_launch": ms_launch, "products_per_launch": products_launch})
 
@@ -223,11 +239,10 @@ def main():
     eng = mps.Batch(g)
     for _ in range(args.warmup):
         eng.submit(item)
-    eng.wait_all()
-    eng.poll()
+    drain(eng, args.warmup, log, "c2 batched warmup")
     t = time.perf_counter()
     bids = [eng.submit(item) for _ in range(args.steps)]
-    eng.wait_all()
+    drain(eng, args.steps, log, "c2 batched")
     ms_batched = 1000.0 * (time.perf_counter() - t) / args.steps
     same = all(eng.relations(i) == rels for i in bids)
     eng.close()
@@ -235,6 +250,8 @@ def main():
     line["config"]["batched_primes_per_s"] = nprimes / (ms_batched / 1000.0)
     line["config"]["batched_relations_identical"] = same
     log(f"c2 plain {ms_per_step:.2f} ms/step, batched {ms_batched:.2f} ms/step")
+    if copy_gbps is not None:
+        line["config"]["hbm_copy_gbps"] = copy_gbps
     if args.mix:
         line["mix"] = run_mix(mps, g, args.mix, args.mix_load, log)
         line["p50_item_latency_s"] = line["mix"]["served_p50_s"]

@@ -47,6 +47,13 @@ int mps_gpu_ok(const mps_gpu* g);
 /* GpuSiever::set_classes (Gpu.hpp:1063): per-prime residue classes on the sieve path (default on). */
 void mps_gpu_set_classes(mps_gpu* g, int on);
 
+/* Which items use the per-prime residue classes when classes are on (no reference counterpart;
+ * the reference's rule is gpu_classes, Math.hpp:324-337):
+ *   0 = as the reference: an even class modulus cm with lcm(cm, 24) <= 48 and N1 - N0 >= 4096 lcm;
+ *   1 = any cm with lcm(cm, 24) <= 48 and the same span (default).
+ * Relations are identical under both; only the device work differs. */
+int mps_gpu_set_class_policy(mps_gpu* g, int policy);
+
 /* GpuSiever::sieve_filter (Gpu.hpp:1064-1067, OpenClSiever::sieve_filter Gpu.hpp:857-979).
  * Generates the primes of [lo|1, hi) on the device and writes to out[] every number it cannot
  * rule out: each prime p of the range for which some in-class n in [N0, N1) has p | b^n + c,
@@ -128,6 +135,11 @@ typedef struct mps_valu_rates {
 } mps_valu_rates;
 
 int mps_gpu_measure_valu(int device_index, mps_valu_rates* r);
+
+/* Copies `bytes` between two device buffers `reps` times with a plain streaming kernel (k_copy)
+ * and returns the achieved read + write rate in GB/s.  Under a hardware-counter profiler its
+ * FETCH_SIZE and WRITE_SIZE per dispatch calibrate the counters against a known byte count. */
+int mps_gpu_copy_probe(int device_index, uint64_t bytes, int reps, double* gbps);
 
 /* Stage outputs of one pass over [lo|1, hi) for stage-level parity tests:
  *   stage 0: survivors (values), stage 1: candidates (value, info) pairs,

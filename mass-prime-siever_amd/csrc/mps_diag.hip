@@ -66,7 +66,48 @@ __global__ __launch_bounds__(64) void k_lat_fma64(double* out, int iters, double
     out[blockIdx.x * blockDim.x + threadIdx.x] = a;
 }
 
+// Streams n16 16-byte words from src to dst: reads and writes n16 * 16 bytes of HBM once each.
+__global__ __launch_bounds__(256) void k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+extern "C" int mps_gpu_copy_probe(int device_index, uint64_t bytes, int reps, double* gbps) {
+    if (gbps == nullptr || reps <= 0 || bytes < 16) return MPS_ERROR;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device_index < 0 || device_index >= ndev) return MPS_ERROR;
+    if (hipSetDevice(device_index) != hipSuccess) return MPS_ERROR;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device_index) != hipSuccess) return MPS_ERROR;
+    const size_t n16 = bytes / 16;
+    uint4 *src = nullptr, *dst = nullptr;
+    if (hipMalloc(&src, n16 * 16) != hipSuccess) return MPS_ERROR;
+    if (hipMalloc(&dst, n16 * 16) != hipSuccess) {
+        hipFree(src);
+        return MPS_ERROR;
+    }
+    hipMemset(src, 1, n16 * 16);
+    const uint32_t grid = (uint32_t)prop.multiProcessorCount * 16;
+    hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, src, dst, n16);  // warm
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy, dim3(grid), dim3(256), 0, 0, src, dst, n16);
+    hipEventRecord(e1, 0);
+    hipEventSynchronize(e1);
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    *gbps = 2.0 * (double)n16 * 16 * reps / (ms * 1e-3) / 1e9;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    const bool ok = hipGetLastError() == hipSuccess;
+    hipFree(src);
+    hipFree(dst);
+    return ok ? MPS_OK : MPS_ERROR;
+}
 
 extern "C" int mps_gpu_measure_valu(int device_index, mps_valu_rates* r) {
     if (r == nullptr) return MPS_ERROR;

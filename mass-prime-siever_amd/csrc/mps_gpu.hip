@@ -82,8 +82,8 @@ void class_size(uint64_t K, uint32_t s, uint32_t L, uint32_t maxm, uint32_t& m, 
 
 namespace mps {
 
-bool item_params(const mps_item* it, bool classes_on, uint32_t mcap, uint32_t waves, ItemParams& ip,
-                 uint64_t& m_max, std::vector<uint2>& csz, std::string& why) {
+bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t waves,
+                 ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why) {
     std::memset(&ip, 0, sizeof(ip));
     ip.b = it->b;
     ip.c = it->c;
@@ -100,12 +100,14 @@ bool item_params(const mps_item* it, bool classes_on, uint32_t mcap, uint32_t wa
     ip.cm_r32 = cm < 65536 ? (uint32_t)((1ull << 32) % cm) : 0;
     ip.mcap = mcap;
     ip.lanes = L;
-    // gpu_classes (Math.hpp:324-337): only for an even cm with lcm(cm, 24) <= 48 and a long range
+    // gpu_classes (Math.hpp:324-337): lcm(cm, 24) <= 48 and a long range; the reference also
+    // requires an even cm (class_policy 0).  The split is exact for any cm: Q is even and so is
+    // p - 1, so g = gcd(Q, p - 1) and qp = lcm(cm, g) are even and every class keeps one parity of n.
     bool qmode = false;
     if (classes_on && it->N0 >= 0 && it->N1 > it->N0) {
         const uint64_t span = (uint64_t)(it->N1 - it->N0);
         const uint64_t Q = cm / gcd_u64(cm, 24) * 24;
-        if ((cm & 1) == 0 && Q <= 48 && span / Q >= 4096) {
+        if ((class_policy == 1 || (cm & 1) == 0) && Q <= 48 && span / Q >= 4096) {
             qmode = true;
             ip.Qmax = Q;
             ip.n0modQ = (uint32_t)(N0 % Q);
@@ -156,10 +158,10 @@ bool plan_item(const mps_gpu* g, const mps_item* item, bool classes_on, ItemPara
     if (waves == 0) {
         // measured on MI355X: two wavefronts per entry for the class tables (m <= 1024) and short
         // plain ranges; four when plain-mode entries run up to 2048 baby steps
-        if (!item_params(item, classes_on, g->mcap, 2, ip, m_max, csz, why)) return false;
+        if (!item_params(item, classes_on, g->class_policy, g->mcap, 2, ip, m_max, csz, why)) return false;
         waves = m_max > 1024 ? 4 : 2;
     }
-    return item_params(item, classes_on, g->mcap, waves, ip, m_max, csz, why);
+    return item_params(item, classes_on, g->class_policy, g->mcap, waves, ip, m_max, csz, why);
 }
 
 FieldKind pass_field(const mps_gpu* g, uint64_t hi) {
@@ -377,6 +379,13 @@ int mps_gpu_ok(const mps_gpu* g) { return g != nullptr && g->ok ? 1 : 0; }
 
 void mps_gpu_set_classes(mps_gpu* g, int on) {
     if (g) g->classes_on = on != 0;
+}
+
+int mps_gpu_set_class_policy(mps_gpu* g, int policy) {
+    if (g == nullptr || (policy != 0 && policy != 1)) return MPS_ERROR;
+    std::lock_guard<std::mutex> lock(g->mu);
+    g->class_policy = policy;
+    return MPS_OK;
 }
 
 int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap, uint32_t waves) {

@@ -21,6 +21,7 @@ struct mps_gpu {
     uint64_t pass_odds = 1ull << 25;
     uint32_t mcap = 2048;
     uint32_t waves = 0;  // 0: by item (4 for plain-mode tables above 1024 baby steps, else 2)
+    int class_policy = 1;  // 0: classes for an even cm only (gpu_classes, Math.hpp:324-337); 1: any cm
     std::mutex mu;
     uint2* d_csz = nullptr;
     uint2* h_csz = nullptr;
@@ -49,8 +50,8 @@ namespace mps {
 // Math.hpp:324-337), the largest baby-step count m_max (plain-mode m is capped at mcap) and the
 // class-mode size table csz[qp * 32 + s] = (m, giant steps per class).
 // Returns false, with the reason, when the BSGS table would not fit.
-bool item_params(const mps_item* it, bool classes_on, uint32_t mcap, uint32_t waves, ItemParams& ip,
-                 uint64_t& m_max, std::vector<uint2>& csz, std::string& why);
+bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t waves,
+                 ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why);
 
 // Item constants and table choices under the handle's tuning (mcap, waves: 0 = by item).
 bool plan_item(const mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max,

@@ -100,16 +100,24 @@ def measure_valu(device_index=0):
     return {n: getattr(r, n) for n, _ in r._fields_}
 
 
+def copy_probe(bytes_=1 << 30, reps=5, device_index=0):
+    """Achieved HBM copy rate (read + write, GB/s) of a plain streaming kernel."""
+    gbps = ctypes.c_double()
+    if lib().mps_gpu_copy_probe(device_index, bytes_, reps, ctypes.byref(gbps)) != MPS_OK:
+        raise RuntimeError("copy probe failed")
+    return gbps.value
+
+
 def build():
     """Compiles libmps.so for gfx950 (hipcc cross-compiles without a GPU)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-EXPORTS = ["mps_gpu_create", "mps_gpu_ok", "mps_gpu_set_classes", "mps_gpu_sieve_filter", "mps_gpu_filter",
+EXPORTS = ["mps_gpu_create", "mps_gpu_ok", "mps_gpu_set_classes", "mps_gpu_set_class_policy", "mps_gpu_sieve_filter", "mps_gpu_filter",
            "mps_gpu_sieve_relations",
            "mps_gpu_last_error", "mps_gpu_destroy", "mps_gpu_get_stats", "mps_gpu_reset_stats",
            "mps_gpu_configure", "mps_gpu_debug_stage", "mps_run_item", "mps_results_json", "mps_is_prime64",
-           "mps_multiplicative_order", "mps_run_bsgs", "mps_gpu_measure_valu", "mps_batch_create",
+           "mps_multiplicative_order", "mps_run_bsgs", "mps_gpu_measure_valu", "mps_gpu_copy_probe", "mps_batch_create",
            "mps_batch_submit", "mps_batch_poll", "mps_batch_wait_all", "mps_batch_relations", "mps_batch_clock",
            "mps_batch_get_stats", "mps_batch_last_error", "mps_batch_destroy"]
 
@@ -132,6 +140,8 @@ def lib():
     L.mps_gpu_ok.argtypes = [vp]
     L.mps_gpu_set_classes.restype = None
     L.mps_gpu_set_classes.argtypes = [vp, i32]
+    L.mps_gpu_set_class_policy.restype = i32
+    L.mps_gpu_set_class_policy.argtypes = [vp, i32]
     L.mps_gpu_sieve_filter.restype = i32
     L.mps_gpu_sieve_filter.argtypes = [vp, u64, u64, P(Item), vp, sz, P(sz)]
     L.mps_gpu_sieve_relations.restype = i32
@@ -162,6 +172,8 @@ def lib():
     L.mps_run_bsgs.argtypes = [u64, P(WorkItem), P(Relation)]
     L.mps_gpu_measure_valu.restype = i32
     L.mps_gpu_measure_valu.argtypes = [i32, P(ValuRates)]
+    L.mps_gpu_copy_probe.restype = i32
+    L.mps_gpu_copy_probe.argtypes = [i32, u64, i32, P(ctypes.c_double)]
     L.mps_batch_create.restype = vp
     L.mps_batch_create.argtypes = [vp, P(BatchOpts)]
     L.mps_batch_submit.restype = ctypes.c_int64
@@ -216,6 +228,11 @@ class GpuSiever:
 
     def set_classes(self, on):
         lib().mps_gpu_set_classes(self.h, 1 if on else 0)
+
+    def set_class_policy(self, policy):
+        """0: classes for an even class modulus only (the reference's rule); 1: for any (default)."""
+        if lib().mps_gpu_set_class_policy(self.h, policy) != MPS_OK:
+            raise ValueError(f"class policy {policy}")
 
     def configure(self, field=-1, pass_odds=0, mcap=0, waves=0):
         return lib().mps_gpu_configure(self.h, field, pass_odds, mcap, waves)

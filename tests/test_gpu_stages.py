@@ -43,26 +43,38 @@ ITEMS = [
 ]
 
 
-@pytest.mark.parametrize("field", [-1, 0])
+def class_modulus(oracle, oit, mod, N0, N1, classes, policy):
+    """(class mode?, Q) of the device: the reference's gpu_classes (Math.hpp:324-337), which
+    needs an even class modulus, or under class policy 1 the same rule for any modulus."""
+    qmode, Q, _ = oracle.gpu_classes(oit)
+    if not qmode and policy == 1 and N1 > N0 >= 0:
+        cm = max(1, mod)
+        Q = cm // np.gcd(cm, 24) * 24
+        qmode = Q <= 48 and (N1 - N0) // Q >= 4096
+    return qmode and classes, int(Q)
+
+
+@pytest.mark.parametrize("field,policy", [(-1, 1), (0, 1), (-1, 0)])
 @pytest.mark.parametrize("it", ITEMS)
-def test_entries_match_oracle_decisions(gpu, mps, oracle, it, field):
+def test_entries_match_oracle_decisions(gpu, mps, oracle, it, field, policy):
     b, c, lo, hi, N0, N1, mod, res, classes = it
     if field == 0 or hi > (1 << 50):
         field_name = "montgomery"
     else:
         field_name = "fp64"
-    print(f"entries {it} field={field_name}", flush=True)
+    print(f"entries {it} field={field_name} class policy {policy}", flush=True)
     gpu.configure(field, 0)
     gpu.set_classes(classes)
+    gpu.set_class_policy(policy)
     try:
         item = mps.work_item(b, c, lo, hi, N0, N1, mod, res)
         ent = gpu.debug_stage(2, lo, hi, item)
     finally:
         gpu.configure(-1, 0)
         gpu.set_classes(True)
+        gpu.set_class_policy(1)
     oit = oracle.make_item(b, c, lo, hi, N0, N1, mod, res)
-    qmode, Q, _ = oracle.gpu_classes(oit)
-    qmode = qmode and classes
+    qmode, Q = class_modulus(oracle, oit, mod, N0, N1, classes, policy)
     surv = oracle.sieve_survivors(lo, hi)
     want = {}
     for x in surv.tolist():
