@@ -25,7 +25,18 @@ FLOP_PER_PRODUCT_F64 = 9
 # 32-bit multiplies per 64-bit Montgomery product (a*b: 4, lo*ninv: 3, high half of m*p: 4)
 MUL_PER_PRODUCT_MONT = 11
 PEAK_FP64_TFLOPS = 78.6          # MI355X published FP64 vector/matrix peak
-PEAK_INT_MUL_TOPS = 9.8          # 256 CU x 16 lanes/clk x 2.4 GHz (quarter-rate 32-bit multiply)
+PEAK_INT_MUL_TOPS = 31.3         # measured full-rate v_mul_lo_u32 issue over the chip (mps_gpu_measure_valu)
+# HBM bytes per k_bsgs launch on this workload: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scale factors
+# from a 1 GiB streaming copy), from profiles/k_bsgs_c2_hbm_traffic.txt
+K_BSGS_TRAFFIC_BYTES = 2.376e8
+TUNING_FILE = os.path.join(ROOT, "mass-prime-siever_amd", "tuning_mi355x.json")
+
+
+def load_tune():
+    spec = importlib.util.spec_from_file_location("mps_tune", os.path.join(ROOT, "mass-prime-siever_amd", "tune.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
 
 
 def load_mps():
@@ -83,24 +94,34 @@ def drain(eng, n, log, label):
     return done
 
 
-def run_mix(mps, g, n_items, load, log):
-    """The serving mix (seed 0): the plain per-chunk worker one item after another, the batched
-    engine with every item submitted at once (throughput), and the batched engine under Poisson
-    arrivals at `load` times the plain worker's capacity (latency).  Plain-worker latencies under
-    the same arrivals follow from its measured service times (one FIFO server)."""
+def mix_items(mps, n_items):
     items = load_mix(n_items)
     primes = sum(d["primes"] for d in items)
     wis = [mps.work_item(d["b"], d["c"], d["P0"], d["P1"], d["N0"], d["N1"], d["mod"], d["res"]) for d in items]
-    # plain worker, sequential
-    svc, rel_plain = [], []
+    return items, wis, primes
+
+
+def plain_mix(g, wis, log, label):
+    """The plain per-chunk worker over the mix, one item after another: service times, relations."""
+    svc, rels_all = [], []
     t0 = time.perf_counter()
     for k, w in enumerate(wis):
         t = time.perf_counter()
         rels, _ = g.run_item(w, chunk_ints=CHUNK)
         svc.append(time.perf_counter() - t)
-        rel_plain.append(rels)
+        rels_all.append(rels)
         if k % 64 == 63:
-            log(f"mix plain: {k + 1} items, {time.perf_counter() - t0:.2f} s")
+            log(f"{label}: {k + 1} items, {time.perf_counter() - t0:.2f} s")
+    return svc, rels_all
+
+
+def run_mix(mps, g, n_items, load, log, plain_untuned):
+    """The serving mix (seed 0): the plain per-chunk worker one item after another, the batched
+    engine with every item submitted at once (throughput), and the batched engine under Poisson
+    arrivals at `load` times the plain worker's capacity (latency).  Plain-worker latencies under
+    the same arrivals follow from its measured service times (one FIFO server)."""
+    items, wis, primes = mix_items(mps, n_items)
+    svc, rel_plain = plain_mix(g, wis, log, "mix plain")
     t_plain = sum(svc)
     # batched, all at once
     eng = mps.Batch(g)
@@ -139,7 +160,7 @@ def run_mix(mps, g, n_items, load, log):
         free_at = begin + s_
         plain_lat.append(free_at - a)
     log(f"mix served: p50 {pct(lat, 0.5):.3f} s p99 {pct(lat, 0.99):.3f} s, identical={served_same}")
-    return {
+    out = {
         "workload": f"serving mix seed 0, {len(wis)} items, {primes} primes",
         "primes": primes,
         "plain_s": t_plain, "plain_primes_per_s": primes / t_plain,
@@ -152,6 +173,27 @@ def run_mix(mps, g, n_items, load, log):
         "plain_p50_s": pct(plain_lat, 0.5), "plain_p99_s": pct(plain_lat, 0.99),
         "batch_stats": bst,
     }
+    if plain_untuned is not None:
+        t_u, rels_u = plain_untuned
+        out["plain_untuned_s"] = t_u
+        out["speedup_batched_vs_plain_untuned"] = t_u / t_batch
+        out["relations_identical"] = out["relations_identical"] and rels_u == rel_plain
+    return out
+
+
+def time_plain(g, item, steps, warmup):
+    """Wall time of `steps` runs of the item on the plain worker, after `warmup` runs."""
+    for _ in range(warmup):
+        g.run_item(item, chunk_ints=CHUNK)
+    g.reset_stats()
+    times, rels = [], None
+    t_all = time.perf_counter()
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        rels, _ = g.run_item(item, chunk_ints=CHUNK)
+        times.append(time.perf_counter() - t0)
+    ms = 1000.0 * (time.perf_counter() - t_all) / steps
+    return ms, times, rels, g.stats()
 
 
 def main():
@@ -163,6 +205,9 @@ def main():
     ap.add_argument("--cpu-slice", type=int, default=1 << 27)
     ap.add_argument("--mix", type=int, default=256, help="items of the serving mix (0 = skip the mix)")
     ap.add_argument("--mix-load", type=float, default=0.7, help="arrival rate / plain worker capacity")
+    ap.add_argument("--tuning", default=TUNING_FILE,
+                    help="tuning table to load after the untuned legs ('none': keep the defaults)")
+    ap.add_argument("--tune", action="store_true", help="autotune the workloads' keys live instead of loading")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB streaming copy (k_copy), the byte-count reference of HBM counters")
     args = ap.parse_args()
@@ -171,27 +216,45 @@ def main():
                                    "(P ranges shard across GPUs as independent processes)"}))
         sys.exit(2)
 
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
     mps = load_mps()
+    tune = load_tune()
     w = WORKLOAD
     g = mps.GpuSiever()
     if not g.init(0):
         raise SystemExit(g.banner)
+    g.clear_tuning()
     item = mps.work_item(w["b"], w["c"], w["P0"], w["P1"], w["N0"], w["N1"], w["mod"], w["res"])
     nprimes = prime_count(w["P0"], w["P1"])
     copy_gbps = mps.copy_probe(1 << 30, 3) if args.copy_probe else None
+    tuned = args.tune or args.tuning != "none"
 
-    for _ in range(args.warmup):
-        g.run_item(item, chunk_ints=CHUNK)
-    g.reset_stats()
-    times, rels = [], None
-    t_all = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        rels, st = g.run_item(item, chunk_ints=CHUNK)
-        times.append(time.perf_counter() - t0)
-    total = time.perf_counter() - t_all
-    ms_per_step = 1000.0 * total / args.steps
-    s = g.stats()
+    # untuned legs first (the handle's default shapes)
+    ms_untuned, rels_untuned, mix_untuned = None, None, None
+    if tuned:
+        ms_untuned, _, rels_untuned, _ = time_plain(g, item, args.steps, args.warmup)
+        log(f"c2 plain untuned {ms_untuned:.2f} ms/step")
+        if args.mix:
+            _, wis_u, _ = mix_items(mps, args.mix)
+            svc_u, rels_mix_u = plain_mix(g, wis_u, log, "mix plain untuned")
+            mix_untuned = (sum(svc_u), rels_mix_u)
+    tuning_info = {"source": "defaults"}
+    if args.tune:
+        items = [tuple(w[k] for k in ("b", "c", "P0", "P1", "N0", "N1", "mod", "res"))]
+        if args.mix:
+            items += [tuple(d[k] for k in ("b", "c", "P0", "P1", "N0", "N1", "mod", "res"))
+                      for d in load_mix(args.mix)]
+        t = time.perf_counter()
+        table = tune.autotune_items(g, items, log=log)
+        tuning_info = {"source": "autotuned live", "keys": len(table), "tune_s": time.perf_counter() - t}
+    elif args.tuning != "none" and os.path.exists(args.tuning):
+        tuning_info = {"source": os.path.relpath(args.tuning, ROOT), "keys": tune.load(g, args.tuning)}
+    log(f"tuning: {tuning_info}")
+
+    ms_per_step, times, rels, s = time_plain(g, item, args.steps, args.warmup)
+    log(f"c2 plain {ms_per_step:.2f} ms/step")
 
     # roofline of the dominant kernel (k_bsgs), from HIP events on the worker's stream
     launches = max(1, s["n_bsgs_launches"])
@@ -200,12 +263,13 @@ def main():
     if s["field"] == 1:
         achieved = products_launch * FLOP_PER_PRODUCT_F64 / (ms_launch * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None}
+                "frac": achieved / PEAK_FP64_TFLOPS}
     else:
         achieved = products_launch * MUL_PER_PRODUCT_MONT / (ms_launch * 1e-3) / 1e12
         roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_MUL_TOPS, "unit": "TOP/s",
-                "frac": achieved / PEAK_INT_MUL_TOPS, "traffic": None}
-    roof.update({"kernel": "k_bsgs", "ms_per_launch": ms_launch, "products_per_launch": products_launch})
+                "frac": achieved / PEAK_INT_MUL_TOPS}
+    roof.update({"traffic": K_BSGS_TRAFFIC_BYTES, "kernel": "k_bsgs", "ms_per_launch": ms_launch,
+                 "products_per_launch": products_launch, "tuning": g.get_tuning(item)[0]})
 
     line = {
         "metric": "primes/sec sieved+BSGS'd at p~1e12 (1 GPU)",
@@ -222,6 +286,7 @@ def main():
         "data": "synthetic (deterministic work item)",
         "config": {"workload": "Factor=10,111,1000000000000,1000268435456,0,10000000,6,4 GPU_classes=1",
                    "chunk_ints": CHUNK, "primes": nprimes, "relations": len(rels),
+                   "tuning": tuning_info,
                    "p50_step_ms": sorted(times)[len(times) // 2] * 1000,
                    "stage_ms_per_step": {k: s[k] / args.steps for k in ("ms_sieve", "ms_filter", "ms_setup",
                                                                           "ms_bsgs")},
@@ -232,10 +297,11 @@ def main():
                    "flagged_per_step": s["flagged"] // args.steps},
         "roofline": roof,
     }
-    def log(msg):
-        print(msg, file=sys.stderr, flush=True)
+    if ms_untuned is not None:
+        line["config"]["untuned_ms_per_step"] = ms_untuned
+        line["config"]["untuned_relations_identical"] = rels_untuned == rels
 
-    # the same item through the batched engine (passes of four items' worth of slots in flight)
+    # the same item through the batched engine (its passes spread over the engine's slots)
     eng = mps.Batch(g)
     for _ in range(args.warmup):
         eng.submit(item)
@@ -253,7 +319,7 @@ def main():
     if copy_gbps is not None:
         line["config"]["hbm_copy_gbps"] = copy_gbps
     if args.mix:
-        line["mix"] = run_mix(mps, g, args.mix, args.mix_load, log)
+        line["mix"] = run_mix(mps, g, args.mix, args.mix_load, log, mix_untuned)
         line["p50_item_latency_s"] = line["mix"]["served_p50_s"]
         line["p99_item_latency_s"] = line["mix"]["served_p99_s"]
     if not args.no_cpu:

@@ -121,6 +121,36 @@ void mps_gpu_reset_stats(mps_gpu* g);
  * waves: wavefronts that share one BSGS entry, 1, 2 or 4 (0 = chosen per item). */
 int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap, uint32_t waves);
 
+/* Shape of the BSGS stage for one kind of item (no reference counterpart).  In a tuning passed to
+ * mps_gpu_set_tuning a zero field keeps the handle's default. */
+typedef struct mps_tuning {
+    uint32_t waves;   /* wavefronts that share one BSGS entry: 1, 2 or 4 (0 = by item) */
+    uint32_t mcap;    /* plain mode: largest baby-step count, 256..4096 (default 2048) */
+    uint32_t cmax;    /* class mode: largest baby-step count, 256..2048 (default 1024) */
+    uint32_t load;    /* keys per 8-slot LDS table bucket the table is sized for: 2 or 4 (default 2) */
+} mps_tuning;
+
+/* Autotuner.  Items are keyed by class mode and kind of class modulus, p magnitude (log2 P0 in
+ * steps of 6) and exponent span per class modulus (log2 in steps of 2).  For the key of `item`
+ * whose P range starts at P0, times every candidate shape (waves 1/2/4 x load 2/4 x the baby-step
+ * cap of its mode: 1024/2048/4096 plain, 512/1024/2048 class mode; shapes whose table exceeds
+ * 64 KB of LDS are skipped) on one pass over the first 2 * sample_odds integers from P0 (device
+ * time of the pass chain, best of two runs), requires every candidate's records to agree with the
+ * default's (the same smallest solution wherever both decided a number; a number may be left to
+ * the host under one shape only) and installs the fastest for the key.  best, best_ms, default_ms
+ * may be NULL. */
+int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sample_odds, mps_tuning* best,
+                     double* best_ms, double* default_ms);
+
+/* The tuning the handle applies to `item` starting at P0 (*key receives its key; may be NULL). */
+int mps_gpu_get_tuning(mps_gpu* g, const mps_item* item, uint64_t P0, mps_tuning* t, uint32_t* key);
+
+/* Installs a tuning for the key of `item` starting at P0 (for instance one saved earlier). */
+int mps_gpu_set_tuning(mps_gpu* g, const mps_item* item, uint64_t P0, const mps_tuning* t);
+
+/* Forgets every installed tuning. */
+void mps_gpu_clear_tuning(mps_gpu* g);
+
 /* Measured issue rates (operations per second over the whole chip) and FP64 FMA dependent-issue
  * latency of device `device_index`; the roofline peaks of the benchmark come from here. */
 typedef struct mps_valu_rates {

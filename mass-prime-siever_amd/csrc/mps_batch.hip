@@ -54,7 +54,6 @@ struct BItem {
     mps_item di{};
     ItemParams ip{};
     uint64_t m_max = 0;
-    uint32_t waves = 2;
     TableGeom tg{};
     std::vector<uint2> csz;
     int table = -1;  // arena slot of the class-size table while admitted
@@ -239,9 +238,17 @@ void admit(mps_batch* b) {
         if (b->broken) {
             it->status = MPS_ERROR;
             it->chunks.clear();
-        } else if (!plan_item(b->g, &it->di, b->g->classes_on, it->ip, it->m_max, it->waves, it->csz, why)) {
-            it->status = MPS_DECLINED;
-            it->chunks.clear();
+        } else {
+            Plan pl;
+            if (!plan_item(b->g, &it->di, it->w.P0, b->g->classes_on, pl, why)) {
+                it->status = MPS_DECLINED;
+                it->chunks.clear();
+            } else {
+                it->ip = pl.ip;
+                it->m_max = pl.m_max;
+                it->tg = pl.tg;
+                it->csz = std::move(pl.csz);
+            }
         }
         if (!it->chunks.empty()) {
             const int t = b->free_tables.back();
@@ -258,7 +265,6 @@ void admit(mps_batch* b) {
                 b->free_tables.pop_back();
                 it->table = t;
                 it->ip.csz = dst;
-                it->tg = table_geom(it->m_max, it->waves);
                 b->mem_used += kCszBytes;
             }
         }
@@ -405,7 +411,7 @@ mps_batch* mps_batch_create(mps_gpu* g, const mps_batch_opts* opts) {
     mps_batch* b = new mps_batch();
     b->g = g;
     if (opts) b->o = *opts;
-    if (b->o.slots <= 0) b->o.slots = 4;
+    if (b->o.slots <= 0) b->o.slots = 8;
     if (b->o.slots > 16) b->o.slots = 16;
     if (b->o.pass_odds == 0) b->o.pass_odds = 1ull << 24;
     if (b->o.pass_odds < 4096 || b->o.pass_odds > (1ull << 30)) {

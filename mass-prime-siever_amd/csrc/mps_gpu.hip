@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -82,8 +83,19 @@ void class_size(uint64_t K, uint32_t s, uint32_t L, uint32_t maxm, uint32_t& m, 
 
 namespace mps {
 
-bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t waves,
-                 ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why) {
+// gpu_classes (Math.hpp:324-337): lcm(cm, 24) <= 48 and a long range; the reference also
+// requires an even cm (class_policy 0).  The split is exact for any cm: Q is even and so is
+// p - 1, so g = gcd(Q, p - 1) and qp = lcm(cm, g) are even and every class keeps one parity of n.
+bool class_mode(const mps_item* it, bool classes_on, int class_policy, uint64_t& Q) {
+    const uint64_t cm = it->cls_mod == 0 ? 1 : it->cls_mod;
+    Q = cm / gcd_u64(cm, 24) * 24;
+    if (!classes_on || it->N0 < 0 || it->N1 <= it->N0) return false;
+    const uint64_t span = (uint64_t)(it->N1 - it->N0);
+    return (class_policy == 1 || (cm & 1) == 0) && Q <= 48 && span / Q >= 4096;
+}
+
+bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t cmax,
+                 uint32_t waves, ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why) {
     std::memset(&ip, 0, sizeof(ip));
     ip.b = it->b;
     ip.c = it->c;
@@ -100,19 +112,12 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
     ip.cm_r32 = cm < 65536 ? (uint32_t)((1ull << 32) % cm) : 0;
     ip.mcap = mcap;
     ip.lanes = L;
-    // gpu_classes (Math.hpp:324-337): lcm(cm, 24) <= 48 and a long range; the reference also
-    // requires an even cm (class_policy 0).  The split is exact for any cm: Q is even and so is
-    // p - 1, so g = gcd(Q, p - 1) and qp = lcm(cm, g) are even and every class keeps one parity of n.
-    bool qmode = false;
-    if (classes_on && it->N0 >= 0 && it->N1 > it->N0) {
-        const uint64_t span = (uint64_t)(it->N1 - it->N0);
-        const uint64_t Q = cm / gcd_u64(cm, 24) * 24;
-        if ((class_policy == 1 || (cm & 1) == 0) && Q <= 48 && span / Q >= 4096) {
-            qmode = true;
-            ip.Qmax = Q;
-            ip.n0modQ = (uint32_t)(N0 % Q);
-            ip.Q_r32 = (uint32_t)((1ull << 32) % Q);
-        }
+    uint64_t Q = 0;
+    const bool qmode = class_mode(it, classes_on, class_policy, Q);
+    if (qmode) {
+        ip.Qmax = Q;
+        ip.n0modQ = (uint32_t)(N0 % Q);
+        ip.Q_r32 = (uint32_t)((1ull << 32) % Q);
     }
     ip.qmode = qmode ? 1 : 0;
     m_max = 1;
@@ -124,7 +129,7 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
             const uint64_t K = ceil_div_u(span, qp);
             for (uint32_t sc = 1; sc <= qp / cm; ++sc) {
                 uint32_t m = 1, gc = L;
-                class_size(K, sc, L, kClassMaxM, m, gc);
+                class_size(K, sc, L, cmax, m, gc);
                 csz[qp * 32 + sc] = make_uint2(m, gc);
                 m_max = std::max<uint64_t>(m_max, m);
             }
@@ -152,16 +157,56 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
     return true;
 }
 
-bool plan_item(const mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max,
-               uint32_t& waves, std::vector<uint2>& csz, std::string& why) {
-    waves = g->waves;
+uint32_t tune_key(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on) {
+    const uint64_t cm = item->cls_mod == 0 ? 1 : item->cls_mod;
+    uint64_t Q = 0;
+    const bool qmode = class_mode(item, classes_on, g->class_policy, Q);
+    const uint64_t span = item->N1 > item->N0 && item->N0 >= 0 ? (uint64_t)(item->N1 - item->N0) : 0;
+    const uint32_t kind = cm == 1 ? 0u : (cm & 1) ? 2u : 1u;
+    const uint32_t pbits = 64 - __builtin_clzll(P0 | 1), sbits = 64 - __builtin_clzll(span / cm | 1);
+    return (qmode ? 1u : 0u) | kind << 1 | (pbits / 6) << 4 | (sbits / 2) << 8 | (g->field_mode == 0 ? 1u << 16 : 0u);
+}
+
+// The candidate shape the autotuner is timing, seen only by the autotuning thread.
+thread_local const mps_tuning* t_candidate = nullptr;
+
+mps_tuning item_tuning(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on) {
+    mps_tuning t{g->waves, g->mcap, kClassMaxM, 2};
+    std::lock_guard<std::mutex> l(g->tune_mu);
+    const mps_tuning* o = nullptr;
+    if (t_candidate) {
+        o = t_candidate;
+    } else {
+        const auto f = g->tuned.find(tune_key(g, item, P0, classes_on));
+        if (f != g->tuned.end()) o = &f->second;
+    }
+    if (o) {
+        if (o->waves) t.waves = o->waves;
+        if (o->mcap) t.mcap = o->mcap;
+        if (o->cmax) t.cmax = o->cmax;
+        if (o->load) t.load = o->load;
+    }
+    return t;
+}
+
+bool plan_item(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on, Plan& pl, std::string& why) {
+    const mps_tuning t = item_tuning(g, item, P0, classes_on);
+    uint32_t waves = t.waves;
     if (waves == 0) {
         // measured on MI355X: two wavefronts per entry for the class tables (m <= 1024) and short
         // plain ranges; four when plain-mode entries run up to 2048 baby steps
-        if (!item_params(item, classes_on, g->class_policy, g->mcap, 2, ip, m_max, csz, why)) return false;
-        waves = m_max > 1024 ? 4 : 2;
+        if (!item_params(item, classes_on, g->class_policy, t.mcap, t.cmax, 2, pl.ip, pl.m_max, pl.csz, why))
+            return false;
+        waves = pl.m_max > 1024 ? 4 : 2;
     }
-    return item_params(item, classes_on, g->class_policy, g->mcap, waves, ip, m_max, csz, why);
+    if (!item_params(item, classes_on, g->class_policy, t.mcap, t.cmax, waves, pl.ip, pl.m_max, pl.csz, why))
+        return false;
+    pl.tg = table_geom(pl.m_max, waves, t.load);
+    if (bsgs_lds_bytes(pl.tg) > kMaxBsgsLds) {
+        why = "BSGS table would exceed the LDS budget";
+        return false;
+    }
+    return true;
 }
 
 FieldKind pass_field(const mps_gpu* g, uint64_t hi) {
@@ -303,12 +348,10 @@ void accumulate(mps_gpu* g, uint64_t pc) {
 }
 
 // Item constants plus the class-size table, staged in pinned memory for run_pass to upload.
-bool prepare_item(mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max,
-                  uint32_t& waves) {
-    std::vector<uint2> csz;
-    if (!plan_item(g, item, classes_on, ip, m_max, waves, csz, g->err)) return false;
-    std::memcpy(g->h_csz, csz.data(), csz.size() * sizeof(uint2));
-    ip.csz = g->d_csz;
+bool prepare_item(mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on, Plan& pl) {
+    if (!plan_item(g, item, P0, classes_on, pl, g->err)) return false;
+    std::memcpy(g->h_csz, pl.csz.data(), pl.csz.size() * sizeof(uint2));
+    pl.ip.csz = g->d_csz;
     return true;
 }
 
@@ -421,12 +464,11 @@ int sieve_records(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, st
     }
     lo |= 1;
     if (lo >= hi) return MPS_OK;
-    ItemParams ip;
-    uint64_t m_max = 0;
-    uint32_t waves = 0;
-    if (!prepare_item(g, item, g->classes_on, ip, m_max, waves)) return MPS_DECLINED;
+    Plan pl;
+    if (!prepare_item(g, item, lo, g->classes_on, pl)) return MPS_DECLINED;
+    const ItemParams& ip = pl.ip;
     const FieldKind fk = pass_field(g, hi);
-    const TableGeom tg = table_geom(m_max, waves);
+    const TableGeom tg = pl.tg;
     const uint64_t count = (hi - lo + 1) / 2;
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
     if (!ensure_buffers(g, std::min(count, g->pass_odds))) return MPS_ERROR;
@@ -500,21 +542,20 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
     std::lock_guard<std::mutex> lock(g->mu);
     if (!g->ok || g->broken) return MPS_ERROR;
     if (n == 0) return MPS_OK;
-    ItemParams ip;
-    uint64_t m_max = 0;
-    uint32_t waves = 0;
-    if (!prepare_item(g, item, false, ip, m_max, waves)) return MPS_DECLINED;  // filter(): plain prep path
     uint64_t pmax = 0, pmin = ~0ull;
     for (size_t i = 0; i < n; ++i) {
         pmax = std::max(pmax, primes[i]);
         pmin = std::min(pmin, primes[i]);
     }
+    Plan pl;
+    if (!prepare_item(g, item, pmin, false, pl)) return MPS_DECLINED;  // filter(): plain prep path
+    const ItemParams& ip = pl.ip;
     if (pmax >= (1ull << 62)) {
         g->err = "prime above 2^62";
         return MPS_DECLINED;
     }
     const FieldKind fk = pass_field(g, pmax + 1);
-    const TableGeom tg = table_geom(m_max, waves);
+    const TableGeom tg = pl.tg;
     const size_t batch = (size_t)std::min<uint64_t>(g->pass_odds, 1ull << 22);
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
     if (!ensure_buffers(g, batch)) return MPS_ERROR;
@@ -573,12 +614,11 @@ int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const m
     if (!g->ok || g->broken) return MPS_ERROR;
     lo |= 1;
     if (lo <= 3 || hi >= (1ull << 62) || lo >= hi) return MPS_DECLINED;
-    ItemParams ip;
-    uint64_t m_max = 0;
-    uint32_t waves = 0;
-    if (!prepare_item(g, item, g->classes_on, ip, m_max, waves)) return MPS_DECLINED;
+    Plan pl;
+    if (!prepare_item(g, item, lo, g->classes_on, pl)) return MPS_DECLINED;
+    const ItemParams& ip = pl.ip;
     const FieldKind fk = pass_field(g, hi);
-    const TableGeom tg = table_geom(m_max, waves);
+    const TableGeom tg = pl.tg;
     const uint64_t count = (hi - lo + 1) / 2;
     if (count > g->pass_odds) {
         g->err = "debug stage covers one pass only";
@@ -661,6 +701,148 @@ void mps_gpu_destroy(mps_gpu* g) {
         if (g->ev[i]) hipEventDestroy(g->ev[i]);
     if (g->stream) hipStreamDestroy(g->stream);
     delete g;
+}
+
+}  // extern "C"
+
+namespace {
+
+bool valid_tuning(const mps_tuning& t) {
+    return (t.waves == 0 || t.waves == 1 || t.waves == 2 || t.waves == 4) && (t.mcap == 0 || (t.mcap >= 256 && t.mcap <= 4096)) &&
+           (t.cmax == 0 || (t.cmax >= 256 && t.cmax <= 2048)) && (t.load == 0 || t.load == 2 || t.load == 4);
+}
+
+double device_ms(const mps_gpu* g) { return g->st.ms_sieve + g->st.ms_filter + g->st.ms_setup + g->st.ms_bsgs; }
+
+// One timed run of the sample under the current forced tuning: device ms, records sorted.
+int timed_sample(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, std::vector<uint64_t>& rec, double& ms) {
+    const double before = device_ms(g);
+    const int st = sieve_records(g, lo, hi, item, rec);
+    ms = device_ms(g) - before;
+    std::vector<std::pair<uint64_t, uint64_t>> pr(rec.size() / 2);
+    for (size_t i = 0; i < pr.size(); ++i) pr[i] = {rec[2 * i], rec[2 * i + 1]};
+    std::sort(pr.begin(), pr.end());
+    for (size_t i = 0; i < pr.size(); ++i) {
+        rec[2 * i] = pr[i].first;
+        rec[2 * i + 1] = pr[i].second;
+    }
+    return st;
+}
+
+// Records of two runs agree: a number both decided has the same smallest solution, and a number
+// one run decided to have a solution appears in the other (decided alike, or left to the host).
+// A number may be left to the host by one shape and decided by another (the cycle test depends on
+// the baby-step count), so neither side's host-decided numbers need a counterpart.
+bool records_agree(const std::vector<uint64_t>& a, const std::vector<uint64_t>& b) {
+    std::map<uint64_t, uint64_t> ma, mb;
+    for (size_t i = 0; i + 1 < a.size(); i += 2) ma[a[i]] = a[i + 1];
+    for (size_t i = 0; i + 1 < b.size(); i += 2) mb[b[i]] = b[i + 1];
+    for (int pass = 0; pass < 2; ++pass) {
+        const auto& x = pass ? mb : ma;
+        const auto& y = pass ? ma : mb;
+        for (const auto& kv : x) {
+            if (kv.second == kHostDecides) continue;
+            const auto f = y.find(kv.first);
+            if (f == y.end() || (f->second != kHostDecides && f->second != kv.second)) return false;
+        }
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sample_odds, mps_tuning* best,
+                     double* best_ms, double* default_ms) {
+    if (g == nullptr || item == nullptr || sample_odds == 0) return MPS_ERROR;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if (!g->ok || g->broken) return MPS_ERROR;
+    const uint64_t lo = P0 | 1;
+    sample_odds = std::min(sample_odds, g->pass_odds);
+    const uint64_t hi = lo + 2 * sample_odds;
+    if (lo <= 3 || hi >= (1ull << 62)) return MPS_DECLINED;
+    const mps_gpu_stats keep = g->st;
+    mps_tuning cand{};
+    auto force = [&](bool on, const mps_tuning& t) {
+        cand = t;
+        t_candidate = on ? &cand : nullptr;
+    };
+    // the defaults (no table entry for the key) give the reference records and time
+    const uint32_t key = tune_key(g, item, P0, g->classes_on);
+    force(true, mps_tuning{g->waves, g->mcap, kClassMaxM, 2});
+    std::vector<uint64_t> ref, rec;
+    double ms = 0, t_default = 1e300;
+    int st = MPS_OK;
+    for (int rep = 0; rep < 3 && st == MPS_OK; ++rep) {  // the first run warms the kernels up
+        st = timed_sample(g, lo, hi, item, ref, ms);
+        if (rep > 0) t_default = std::min(t_default, ms);
+    }
+    if (st != MPS_OK) {
+        force(false, mps_tuning{});
+        g->st = keep;
+        return st;
+    }
+    uint64_t Q = 0;
+    const bool qmode = class_mode(item, g->classes_on, g->class_policy, Q);
+    mps_tuning win{0, 0, 0, 0};
+    double t_win = t_default;
+    for (uint32_t waves : {1u, 2u, 4u}) {
+        for (uint32_t load : {2u, 4u}) {
+            for (uint32_t cap : {0u, 1u, 2u}) {
+                mps_tuning t{waves, qmode ? g->mcap : 1024u << cap, qmode ? 512u << cap : kClassMaxM, load};
+                Plan pl;
+                std::string why;
+                force(true, t);
+                if (!plan_item(g, item, P0, g->classes_on, pl, why)) continue;  // the table would not fit
+                double tbest = 1e300;
+                for (int rep = 0; rep < 2 && st == MPS_OK; ++rep) {
+                    st = timed_sample(g, lo, hi, item, rec, ms);
+                    tbest = std::min(tbest, ms);
+                    if (st == MPS_OK && !records_agree(rec, ref)) {
+                        g->err = "autotune: a candidate shape contradicts the default's records";
+                        st = MPS_ERROR;
+                    }
+                }
+                if (st != MPS_OK) break;
+                if (tbest < t_win) {
+                    t_win = tbest;
+                    win = t;
+                }
+            }
+        }
+    }
+    force(false, mps_tuning{});
+    g->st = keep;
+    if (st != MPS_OK) return st;
+    {
+        std::lock_guard<std::mutex> l(g->tune_mu);
+        g->tuned[key] = win;  // all zero: the defaults were fastest
+    }
+    if (best) *best = item_tuning(g, item, P0, g->classes_on);
+    if (best_ms) *best_ms = t_win;
+    if (default_ms) *default_ms = t_default;
+    return MPS_OK;
+}
+
+int mps_gpu_get_tuning(mps_gpu* g, const mps_item* item, uint64_t P0, mps_tuning* t, uint32_t* key) {
+    if (g == nullptr || item == nullptr || t == nullptr) return MPS_ERROR;
+    *t = item_tuning(g, item, P0, g->classes_on);
+    if (key) *key = tune_key(g, item, P0, g->classes_on);
+    return MPS_OK;
+}
+
+int mps_gpu_set_tuning(mps_gpu* g, const mps_item* item, uint64_t P0, const mps_tuning* t) {
+    if (g == nullptr || item == nullptr || t == nullptr || !valid_tuning(*t)) return MPS_ERROR;
+    std::lock_guard<std::mutex> l(g->tune_mu);
+    g->tuned[tune_key(g, item, P0, g->classes_on)] = *t;
+    return MPS_OK;
+}
+
+void mps_gpu_clear_tuning(mps_gpu* g) {
+    if (g == nullptr) return;
+    std::lock_guard<std::mutex> l(g->tune_mu);
+    g->tuned.clear();
 }
 
 }  // extern "C"

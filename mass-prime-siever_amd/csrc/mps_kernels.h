@@ -12,7 +12,8 @@ constexpr uint32_t kSieveThreads = 256;
 constexpr uint32_t kSmallPrimeBound = 1024;       // primes below this are shared by all lanes
 constexpr uint32_t kEntWords = 72;                // u64 words per BSGS entry (576 B)
 constexpr uint32_t kMaxClasses = 24;              // classes per prime in class mode
-constexpr uint32_t kClassMaxM = 1024;             // largest baby-step count in class mode
+constexpr uint32_t kClassMaxM = 1024;             // default largest baby-step count in class mode
+constexpr size_t kMaxBsgsLds = 64 * 1024;          // LDS of one BSGS workgroup at most
 constexpr uint64_t kHostDecides = ~0ull;          // M of a record the host must decide exactly
 
 // Entry layout (u64 words).  B is the baby-step base, Bm = B^m the giant-step base.
@@ -33,6 +34,8 @@ enum EntryWord : uint32_t {
     E_META = 41,  // qp (bits 0..7) | stride 2 (bit 8) | odd k (bit 9) | class mode (bit 10)
     E_KLIM = 42,  // plain: jcount (solutions k = g*m - i must lie in [0, jcount))
     E_DJ = 43,    // class mode: dj of the kept classes, 8 bits each, 3 words (n_j = N0 + dj)
+    E_BL = 46,    // B^L, L = lanes of the BSGS workgroup (the baby-step stride of one lane)
+    E_BML = 47,   // Bm^L (the giant-step stride of one lane)
     E_G = 48,     // giant start per class G_j = b^(n_j), j < s
 };
 
@@ -100,7 +103,7 @@ void launch_sieve(hipStream_t s, const PassDesc* d, uint64_t nodd, const uint32_
                   uint32_t nsp, uint32_t nsmall, const PassBuffers& pb, uint32_t grid_override);
 void launch_filter(hipStream_t s, const PassDesc* d, const PassBuffers& pb, uint32_t grid);
 void launch_setup(hipStream_t s, FieldKind fk, bool qmode, const PassDesc* d, const PassBuffers& pb, uint32_t grid);
-TableGeom table_geom(uint64_t m_max, uint64_t waves);
+TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load);
 constexpr uint32_t kBsgsWaves = 4;                // default wavefronts of one BSGS workgroup (one entry)
 size_t bsgs_lds_bytes(const TableGeom& tg);
 

@@ -317,6 +317,14 @@ __device__ __forceinline__ typename F::T write_pow_tables(const F& f, typename F
 
 __device__ __forceinline__ uint32_t bitlen64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
+// x^L from x^64 for L = 64, 128 or 256 lanes.
+template <class F>
+__device__ __forceinline__ typename F::T lane_stride_pow(const F& f, typename F::T x64, uint32_t lanes) {
+    typename F::T r = x64;
+    for (uint32_t l = 64; l < lanes; l *= 2) r = f.mul(r, r);
+    return r;
+}
+
 template <class F, bool QMODE>
 __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, PassBuffers pb) {
     const uint64_t lo = d->lo;
@@ -466,8 +474,12 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
             }
             const T Bm = fpow(f, B, m);
             w_setup += 2 * bitlen64(m);
-            E[E_B64] = F::to_bits(write_pow_tables(f, B, E + E_BLO, w_setup));
-            E[E_BM64] = F::to_bits(write_pow_tables(f, Bm, E + E_MLO, w_setup));
+            const T B64 = write_pow_tables(f, B, E + E_BLO, w_setup);
+            const T Bm64 = write_pow_tables(f, Bm, E + E_MLO, w_setup);
+            E[E_B64] = F::to_bits(B64);
+            E[E_BM64] = F::to_bits(Bm64);
+            E[E_BL] = F::to_bits(lane_stride_pow(f, B64, ip.lanes));
+            E[E_BML] = F::to_bits(lane_stride_pow(f, Bm64, ip.lanes));
             w_bsgs += m + (uint64_t)s * ng;
         }
     }
@@ -617,17 +629,18 @@ __device__ __forceinline__ bool has_cycle(const BsgsEntry<F>& be, const BsgsTabl
     return cycle;
 }
 
-// x^tid for tid < 64 * WB from the entry's tables: x^(tid & 63) * (x^64)^(tid >> 6).
+// x^tid for tid < 64 * WB from the entry's tables: x^(tid & 63) * (x^64)^(tid >> 6); the lane
+// stride x^(64 * WB) comes precomputed from the entry (word stride_word).
 template <class F, int WB>
 __device__ __forceinline__ typename F::T thread_pow(const F& f, const uint64_t* E, uint32_t lo_word, uint32_t hi_word,
-                                                  typename F::T x64, uint32_t tid, typename F::T& stride_pow) {
+                                                  typename F::T x64, uint32_t stride_word, uint32_t tid,
+                                                  typename F::T& stride_pow) {
     typedef typename F::T T;
     T r = f.mul(F::from_bits(E[lo_word + (tid & 7)]), F::from_bits(E[hi_word + ((tid >> 3) & 7)]));
-    const T x128 = f.mul(x64, x64);
     const uint32_t w = tid >> 6;
     if (WB >= 2 && (w & 1)) r = f.mul(r, x64);
-    if (WB >= 4 && (w & 2)) r = f.mul(r, x128);
-    stride_pow = WB == 1 ? x64 : (WB == 2 ? x128 : f.mul(x128, x128));  // x^(64 * WB)
+    if (WB >= 4 && (w & 2)) r = f.mul(r, f.mul(x64, x64));
+    stride_pow = F::from_bits(E[stride_word]);
     return r;
 }
 
@@ -682,7 +695,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         // baby steps t * B^i, i == tid (mod L)
         {
             T step;
-            T v = f.mul(be.t, thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, tid, step));
+            T v = f.mul(be.t, thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, step));
             for (uint32_t i = tid; i < be.m; i += L) {
                 uint32_t fp, b;
                 hash_key(f.key(v), Tb.nbmask, fp, b);
@@ -701,7 +714,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             const uint64_t parity = qmode ? 0 : (be.meta >> 9) & 1u;
             const uint64_t cmul = qmode ? be.qp : item_cm;
             T mstep;
-            const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(E[E_BM64]), tid, mstep);
+            const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(E[E_BM64]), E_BML, tid, mstep);
             uint64_t best = ~0ull;
             for (uint32_t j = 0; j < be.s; ++j) {
                 uint64_t nj, Kj;
@@ -802,13 +815,14 @@ void launch_setup(hipStream_t s, FieldKind fk, bool qmode, const PassDesc* d, co
     }
 }
 
-// Table geometry for baby-step counts up to m_max: about two keys per 8-slot bucket.
-TableGeom table_geom(uint64_t m_max, uint64_t waves) {
+// Table geometry for baby-step counts up to m_max: at most `load` keys per 8-slot bucket on
+// average (2: a quarter of the slots filled, 4: half).
+TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load) {
     TableGeom tg;
     tg.ibits = 16;
     tg.fpbits = 16;
     tg.nbbits = 4;
-    while ((2ull << tg.nbbits) < m_max) ++tg.nbbits;  // 8 * 2^nbbits slots >= 4 * m_max
+    while (((uint64_t)load << tg.nbbits) < m_max) ++tg.nbbits;
     tg.waves = (uint32_t)waves;
     return tg;
 }

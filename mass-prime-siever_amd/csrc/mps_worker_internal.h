@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -22,6 +23,9 @@ struct mps_gpu {
     uint32_t mcap = 2048;
     uint32_t waves = 0;  // 0: by item (4 for plain-mode tables above 1024 baby steps, else 2)
     int class_policy = 1;  // 0: classes for an even cm only (gpu_classes, Math.hpp:324-337); 1: any cm
+    // autotuned table shapes by item key
+    mutable std::mutex tune_mu;
+    std::map<uint32_t, mps_tuning> tuned;
     std::mutex mu;
     uint2* d_csz = nullptr;
     uint2* h_csz = nullptr;
@@ -50,12 +54,28 @@ namespace mps {
 // Math.hpp:324-337), the largest baby-step count m_max (plain-mode m is capped at mcap) and the
 // class-mode size table csz[qp * 32 + s] = (m, giant steps per class).
 // Returns false, with the reason, when the BSGS table would not fit.
-bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t waves,
-                 ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why);
+bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t cmax,
+                 uint32_t waves, ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why);
 
-// Item constants and table choices under the handle's tuning (mcap, waves: 0 = by item).
-bool plan_item(const mps_gpu* g, const mps_item* item, bool classes_on, ItemParams& ip, uint64_t& m_max,
-               uint32_t& waves, std::vector<uint2>& csz, std::string& why);
+// Everything a pass of one item needs besides the range: constants, class-size table, geometry.
+struct Plan {
+    ItemParams ip{};
+    uint64_t m_max = 0;
+    TableGeom tg{};
+    std::vector<uint2> csz;
+};
+
+// Key of the tuning table: class mode and modulus kind, p magnitude (6-bit steps of log2 P0) and
+// exponent span per class modulus (2-bit steps of its log2).
+uint32_t tune_key(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on);
+
+// The tuning an item gets: the candidate the autotuner is timing on this thread, else the table
+// entry of its key, else the handle's settings (waves 0 = by item, mcap, class-mode cap
+// kClassMaxM, load 2).
+mps_tuning item_tuning(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on);
+
+// Item constants and table choices for an item whose P range starts at P0.
+bool plan_item(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on, Plan& pl, std::string& why);
 
 // Field of a pass ending below hi under the handle's setting.
 FieldKind pass_field(const mps_gpu* g, uint64_t hi);

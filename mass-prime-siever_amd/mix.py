@@ -84,3 +84,13 @@ def worktodo_line(it):
     if it["mod"] > 1:
         s += f",{it['mod']},{it['res']}"
     return s
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="Prints the seeded serving mix as worktodo.txt lines.")
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("-n", type=int, default=256)
+    args = ap.parse_args()
+    for it in serving_mix(args.seed, args.n):
+        print(worktodo_line(it))

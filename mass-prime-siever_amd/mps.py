@@ -62,6 +62,15 @@ class GpuStats(ctypes.Structure):
         return {n: getattr(self, n) for n, _ in self._fields_}
 
 
+class Tuning(ctypes.Structure):
+    """mps_tuning (include/mps_gpu.h): the BSGS shape of one kind of item."""
+    _fields_ = [("waves", ctypes.c_uint32), ("mcap", ctypes.c_uint32), ("cmax", ctypes.c_uint32),
+                ("load", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
 class BatchOpts(ctypes.Structure):
     """mps_batch_opts (include/mps_batch.h)."""
     _fields_ = [("slots", ctypes.c_int), ("pass_odds", ctypes.c_uint64), ("use_graphs", ctypes.c_int),
@@ -117,7 +126,8 @@ EXPORTS = ["mps_gpu_create", "mps_gpu_ok", "mps_gpu_set_classes", "mps_gpu_set_c
            "mps_gpu_sieve_relations",
            "mps_gpu_last_error", "mps_gpu_destroy", "mps_gpu_get_stats", "mps_gpu_reset_stats",
            "mps_gpu_configure", "mps_gpu_debug_stage", "mps_run_item", "mps_results_json", "mps_is_prime64",
-           "mps_multiplicative_order", "mps_run_bsgs", "mps_gpu_measure_valu", "mps_gpu_copy_probe", "mps_batch_create",
+           "mps_multiplicative_order", "mps_run_bsgs", "mps_gpu_measure_valu", "mps_gpu_copy_probe", "mps_gpu_autotune", "mps_gpu_get_tuning",
+           "mps_gpu_set_tuning", "mps_gpu_clear_tuning", "mps_batch_create",
            "mps_batch_submit", "mps_batch_poll", "mps_batch_wait_all", "mps_batch_relations", "mps_batch_clock",
            "mps_batch_get_stats", "mps_batch_last_error", "mps_batch_destroy"]
 
@@ -172,6 +182,14 @@ def lib():
     L.mps_run_bsgs.argtypes = [u64, P(WorkItem), P(Relation)]
     L.mps_gpu_measure_valu.restype = i32
     L.mps_gpu_measure_valu.argtypes = [i32, P(ValuRates)]
+    L.mps_gpu_autotune.restype = i32
+    L.mps_gpu_autotune.argtypes = [vp, P(Item), u64, u64, P(Tuning), P(ctypes.c_double), P(ctypes.c_double)]
+    L.mps_gpu_get_tuning.restype = i32
+    L.mps_gpu_get_tuning.argtypes = [vp, P(Item), u64, P(Tuning), P(ctypes.c_uint32)]
+    L.mps_gpu_set_tuning.restype = i32
+    L.mps_gpu_set_tuning.argtypes = [vp, P(Item), u64, P(Tuning)]
+    L.mps_gpu_clear_tuning.restype = None
+    L.mps_gpu_clear_tuning.argtypes = [vp]
     L.mps_gpu_copy_probe.restype = i32
     L.mps_gpu_copy_probe.argtypes = [i32, u64, i32, P(ctypes.c_double)]
     L.mps_batch_create.restype = vp
@@ -315,6 +333,29 @@ class GpuSiever:
             if rc != MPS_OK:
                 raise RuntimeError(f"run_item status {rc}: {self.last_error()}")
             return [(rels[i].p, rels[i].M, rels[i].q) for i in range(n.value)], st
+
+    def autotune(self, item, sample_odds=1 << 22):
+        """Times the BSGS shapes on a sample pass of a work item and installs the fastest for its
+        key; returns (tuning dict, best ms, default ms)."""
+        t, best, dflt = Tuning(), ctypes.c_double(), ctypes.c_double()
+        rc = lib().mps_gpu_autotune(self.h, ctypes.byref(device_item(item)), item.P0, sample_odds, ctypes.byref(t),
+                                    ctypes.byref(best), ctypes.byref(dflt))
+        if rc != MPS_OK:
+            raise RuntimeError(f"autotune status {rc}: {self.last_error()}")
+        return t.as_dict(), best.value, dflt.value
+
+    def get_tuning(self, item):
+        t, key = Tuning(), ctypes.c_uint32()
+        lib().mps_gpu_get_tuning(self.h, ctypes.byref(device_item(item)), item.P0, ctypes.byref(t), ctypes.byref(key))
+        return t.as_dict(), key.value
+
+    def set_tuning(self, item, **kw):
+        t = Tuning(kw.get("waves", 0), kw.get("mcap", 0), kw.get("cmax", 0), kw.get("load", 0))
+        if lib().mps_gpu_set_tuning(self.h, ctypes.byref(device_item(item)), item.P0, ctypes.byref(t)) != MPS_OK:
+            raise ValueError(f"tuning {kw}")
+
+    def clear_tuning(self):
+        lib().mps_gpu_clear_tuning(self.h)
 
     def stats(self):
         s = GpuStats()

@@ -1,7 +1,7 @@
 // CPU stand-in for the device side of the batch engine (test infrastructure only).
 //
 // It provides the HIP runtime calls of tests/mock_hip/hip/hip_runtime.h and the device-facing
-// helpers the engine links against (plan_item, pass_field, table_geom, build_pass_graph, ...), so
+// helpers the engine links against (plan_item, pass_field, build_pass_graph, ...), so
 // that the real scheduler in mass-prime-siever_amd/csrc/mps_batch.hip runs on the CPU.
 //
 // A simulated pass reads its descriptor when it runs (as the device does), checks that the
@@ -170,26 +170,23 @@ hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t s) {
 
 namespace mps {
 
-bool plan_item(const mps_gpu*, const mps_item* item, bool, ItemParams& ip, uint64_t& m_max, uint32_t& waves,
-               std::vector<uint2>& csz, std::string& why) {
+bool plan_item(const mps_gpu*, const mps_item* item, uint64_t, bool, Plan& pl, std::string& why) {
     if (item->N1 - item->N0 > 1000000000000ll) {
         why = "BSGS table would exceed the LDS budget";
         return false;
     }
-    ip = ItemParams{};
-    ip.b = item->b;
-    ip.c = item->c;
-    ip.qmode = item->cls_mod > 1 ? 1 : 0;
-    m_max = 64;
-    waves = item->cls_mod > 1 ? 2 : 4;
-    csz.assign(49 * 32, uint2{0, 0});
-    csz[0] = uint2{(uint32_t)item->b, (uint32_t)item->c};
+    pl.ip = ItemParams{};
+    pl.ip.b = item->b;
+    pl.ip.c = item->c;
+    pl.ip.qmode = item->cls_mod > 1 ? 1 : 0;
+    pl.m_max = 64;
+    pl.tg = TableGeom{4, 16, 16, item->cls_mod > 1 ? 2u : 4u};
+    pl.csz.assign(49 * 32, uint2{0, 0});
+    pl.csz[0] = uint2{(uint32_t)item->b, (uint32_t)item->c};
     return true;
 }
 
 FieldKind pass_field(const mps_gpu*, uint64_t hi) { return hi <= (1ull << 50) ? FIELD_F64 : FIELD_MONT; }
-
-TableGeom table_geom(uint64_t, uint64_t waves) { return TableGeom{4, 16, 16, (uint32_t)waves}; }
 
 uint32_t bsgs_persistent_grid(FieldKind, const TableGeom&, int cus) { return (uint32_t)cus; }
 
