@@ -181,6 +181,48 @@ def run_mix(mps, g, n_items, load, log, plain_untuned):
     return out
 
 
+CONFIGS = {  # the other configurations of the metric, as (b, c, P0, P1, N0, N1, mod, res, classes)
+    "c1": (2, -25, 10**9, 10**9 + 2**26, 3_600_000, 5_000_000, 2, 1, False),
+    "c3": (2, -25, 95 * 10**12, 95 * 10**12 + 2**28, 3_600_000, 5_000_000, 2, 1, True),
+    "c4": (3, -2, 4 * 10**18, 4 * 10**18 + 2**30, 0, 10_000_000, 1, 0, True),
+}
+
+
+def run_configs(mps, g, log, reps=2):
+    """Item latency and primes/s of c1, c3 and c4 on the plain worker and the batched engine."""
+    out = {}
+    for name, (b, c, P0, P1, N0, N1, mod, res, classes) in CONFIGS.items():
+        w = mps.work_item(b, c, P0, P1, N0, N1, mod, res)
+        g.set_classes(classes)
+        try:
+            g.run_item(w, chunk_ints=CHUNK)  # warm-up
+            t_plain = []
+            for _ in range(reps):
+                t = time.perf_counter()
+                rels, _ = g.run_item(w, chunk_ints=CHUNK)
+                t_plain.append(time.perf_counter() - t)
+            eng = mps.Batch(g)
+            t_batch, same = [], True
+            for _ in range(reps):
+                t = time.perf_counter()
+                i = eng.submit(w)
+                done = drain(eng, 1, log, f"{name} batched")
+                t_batch.append(time.perf_counter() - t)
+                same = same and done[0]["status"] == 0 and eng.relations(i) == rels
+            eng.close()
+        finally:
+            g.set_classes(True)
+        n = prime_count(P0, P1)
+        out[name] = {"item": f"Factor={b},{c},{P0},{P1},{N0},{N1}" + (f",{mod},{res}" if mod > 1 else "") +
+                             f" GPU_classes={int(classes)}",
+                     "primes": n, "relations": len(rels),
+                     "plain_item_s": min(t_plain), "plain_primes_per_s": n / min(t_plain),
+                     "batched_item_s": min(t_batch), "batched_primes_per_s": n / min(t_batch),
+                     "relations_identical": same}
+        log(f"{name}: plain {min(t_plain)*1e3:.1f} ms, batched {min(t_batch)*1e3:.1f} ms per item")
+    return out
+
+
 def time_plain(g, item, steps, warmup):
     """Wall time of `steps` runs of the item on the plain worker, after `warmup` runs."""
     for _ in range(warmup):
@@ -208,6 +250,7 @@ def main():
     ap.add_argument("--tuning", default=TUNING_FILE,
                     help="tuning table to load after the untuned legs ('none': keep the defaults)")
     ap.add_argument("--tune", action="store_true", help="autotune the workloads' keys live instead of loading")
+    ap.add_argument("--no-configs", action="store_true", help="skip the c1/c3/c4 item legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB streaming copy (k_copy), the byte-count reference of HBM counters")
     args = ap.parse_args()
@@ -318,6 +361,8 @@ def main():
     log(f"c2 plain {ms_per_step:.2f} ms/step, batched {ms_batched:.2f} ms/step")
     if copy_gbps is not None:
         line["config"]["hbm_copy_gbps"] = copy_gbps
+    if not args.no_configs:
+        line["configs"] = run_configs(mps, g, log)
     if args.mix:
         line["mix"] = run_mix(mps, g, args.mix, args.mix_load, log, mix_untuned)
         line["p50_item_latency_s"] = line["mix"]["served_p50_s"]
