@@ -57,9 +57,12 @@ int mps_gpu_set_class_policy(mps_gpu* g, int policy);
 /* GpuSiever::sieve_filter (Gpu.hpp:1064-1067, OpenClSiever::sieve_filter Gpu.hpp:857-979).
  * Generates the primes of [lo|1, hi) on the device and writes to out[] every number it cannot
  * rule out: each prime p of the range for which some in-class n in [N0, N1) has p | b^n + c,
- * plus the numbers left to the host (p < 1000, p | b) and rare fingerprint false positives or
- * base-2 pseudoprimes.  Order is unspecified; the caller sorts, dedups, range-checks and runs
- * the exact check.  Precondition of the GPU path: hi < 2^62 and lo > 3 (Worker.hpp:209). */
+ * plus the rare numbers the device leaves to the host (p < 1000, p | b, an order of the baby
+ * step below the baby-step count, an entry that did not fit).  The device verifies every table
+ * match exactly and tests primality with the 12-base Miller-Rabin test, so the list holds no
+ * false positives or pseudoprimes otherwise.  Order is unspecified; the caller sorts, dedups,
+ * range-checks and runs the exact check, as the reference's caller does (Worker.hpp:223-227).
+ * Precondition of the GPU path: hi < 2^62 and lo > 3 (Worker.hpp:209). */
 int mps_gpu_sieve_filter(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, uint64_t* out,
                          size_t out_cap, size_t* out_n);
 
