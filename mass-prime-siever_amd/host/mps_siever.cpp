@@ -12,11 +12,13 @@
 #include <cstring>
 #include <ctime>
 #include <fstream>
+#include <map>
 #include <sstream>
 #include <string>
 #include <thread>
 #include <vector>
 
+#include "mps_batch.h"
 #include "mps_gpu.h"
 #include "mps_host.h"
 
@@ -65,6 +67,8 @@ struct Config {  // PrimeConfig (Config.hpp:16-26)
     int gpu_sieve = 0;
     int gpu_classes = 1;
     int gpu_strict = 0;
+    int gpu_batch = 0;     // this build: 1 = every item through the batch engine (mps_batch.h)
+    int gpu_autotune = 0;  // this build: 1 = autotune each kind of item before running it
 };
 
 int hw_threads() {
@@ -104,6 +108,10 @@ Config load_config(const char* path) {  // load_prime_config (Config.hpp:50-121)
                 cfg.gpu_classes = std::stoi(v);
             } else if (k == "GPU_strict") {
                 cfg.gpu_strict = std::stoi(v);
+            } else if (k == "GPU_batch") {
+                cfg.gpu_batch = std::stoi(v);
+            } else if (k == "GPU_autotune") {
+                cfg.gpu_autotune = std::stoi(v);
             } else if (k == "MemoryAllowed") {
                 cfg.memory_allowed_mb = std::stoi(v);
                 has_mem = true;
@@ -209,6 +217,59 @@ void append_results(const std::string& obj) {  // Results.hpp:109-139, as a comp
     f << out << '\n';
 }
 
+// relations.p, the results.json object and the log line of one finished item.
+void write_item(const mps_work_item& it, size_t w, size_t nitems, const std::vector<mps_relation>& rels,
+                const std::string& stats) {
+    FILE* f = std::fopen("relations.p", "wb");
+    if (!f) fatal("Failed to create relations.p");
+    for (const mps_relation& r : rels)
+        std::fprintf(f, "%llu, %llu, %llu\n", (unsigned long long)r.p, (unsigned long long)r.M, (unsigned long long)r.q);
+    std::fclose(f);
+    log_line("INFO", "Loaded " + std::to_string(rels.size()) + " relation(s)");
+    const std::string ts = utc_now();
+    const size_t len = mps_results_json(&it, rels.data(), rels.size(), ts.c_str(), nullptr, 0);
+    std::string obj(len + 1, '\0');
+    mps_results_json(&it, rels.data(), rels.size(), ts.c_str(), &obj[0], obj.size());
+    obj.resize(len);
+    append_results(obj);
+    log_line("INFO", "Finished work item " + std::to_string(w + 1) + "/" + std::to_string(nitems) + ": " +
+                         std::to_string(rels.size()) + " relations, " + stats);
+}
+
+// Every item through the batch engine at once; outputs are still written in worktodo order.
+void run_batched(mps_gpu* gpu, const std::vector<mps_work_item>& items) {
+    mps_batch_opts o{};
+    o.use_graphs = 1;
+    mps_batch* b = mps_batch_create(gpu, &o);
+    if (b == nullptr) fatal(std::string("batch engine: ") + mps_gpu_last_error(gpu));
+    std::vector<int64_t> ids;
+    for (size_t w = 0; w < items.size(); ++w) ids.push_back(mps_batch_submit(b, &items[w], w));
+    log_line("INFO", "Submitted " + std::to_string(items.size()) + " work item(s) to the batch engine");
+    std::map<uint64_t, mps_item_done> done;
+    std::vector<mps_item_done> buf(256);
+    for (size_t w = 0; w < items.size(); ++w) {
+        while (done.find((uint64_t)ids[w]) == done.end()) {
+            size_t n = 0;
+            mps_batch_poll(b, buf.data(), buf.size(), &n);
+            for (size_t k = 0; k < n; ++k) done[buf[k].id] = buf[k];
+            if (n == 0) std::this_thread::sleep_for(std::chrono::milliseconds(1));
+        }
+        const mps_item_done& d = done[(uint64_t)ids[w]];
+        if (d.status != MPS_OK)
+            fatal(std::string("GPU worker failed on work item ") + std::to_string(w + 1) + ": status " +
+                  std::to_string(d.status) + " " + mps_batch_last_error(b));
+        std::vector<mps_relation> rels(d.relations + 1);
+        size_t n = 0;
+        if (mps_batch_relations(b, d.id, rels.data(), rels.size(), &n) != MPS_OK) fatal("batch relations read failed");
+        rels.resize(n);
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "submitted at %.3f s, admitted at %.3f s, done at %.3f s", d.t_submit, d.t_admit,
+                      d.t_done);
+        write_item(items[w], w, items.size(), rels, msg);
+    }
+    mps_batch_destroy(b);
+}
+
 }  // namespace
 
 int main() {
@@ -236,7 +297,23 @@ int main() {
     const unsigned long long chunk = std::max(1ULL, total / (unsigned long long)std::max(1, cfg.cpu_count));
 
     std::remove("relations.p");
-    for (size_t w = 0; w < items.size(); ++w) {
+    if (cfg.gpu_autotune) {
+        for (const mps_work_item& it : items) {
+            const mps_item di{it.b, it.c, it.N0, it.N1, it.cls_mod, it.cls_res};
+            mps_tuning t{};
+            double best = 0, dflt = 0;
+            if (mps_gpu_autotune(gpu, &di, it.P0, 1ull << 22, &t, &best, &dflt) == MPS_OK) {
+                char msg[200];
+                std::snprintf(msg, sizeof(msg), "Autotuned b=%lld c=%lld: waves %u, mcap %u, cmax %u, load %u (%.3f ms, default %.3f ms)",
+                              (long long)it.b, (long long)it.c, t.waves, t.mcap, t.cmax, t.load, best, dflt);
+                log_line("INFO", msg);
+            }
+        }
+    }
+    if (cfg.gpu_batch) {
+        run_batched(gpu, items);
+    }
+    for (size_t w = 0; w < items.size() && !cfg.gpu_batch; ++w) {
         const mps_work_item& it = items[w];
         log_line("INFO", "Processing work item " + std::to_string(w + 1) + "/" + std::to_string(items.size()) +
                              " b=" + std::to_string(it.b) + " c=" + std::to_string(it.c));
@@ -251,25 +328,10 @@ int main() {
         }
         if (rc != MPS_OK) fatal(std::string("GPU worker failed on this item: ") + mps_gpu_last_error(gpu));
         rels.resize(n);
-        {
-            FILE* f = std::fopen("relations.p", "wb");
-            if (!f) fatal("Failed to create relations.p");
-            for (const mps_relation& r : rels)
-                std::fprintf(f, "%llu, %llu, %llu\n", (unsigned long long)r.p, (unsigned long long)r.M,
-                             (unsigned long long)r.q);
-            std::fclose(f);
-        }
-        log_line("INFO", "Loaded " + std::to_string(n) + " relation(s)");
-        const std::string ts = utc_now();
-        const size_t len = mps_results_json(&it, rels.data(), n, ts.c_str(), nullptr, 0);
-        std::string obj(len + 1, '\0');
-        mps_results_json(&it, rels.data(), n, ts.c_str(), &obj[0], obj.size());
-        obj.resize(len);
-        append_results(obj);
-        char msg[256];
-        std::snprintf(msg, sizeof(msg), "Finished work item %zu: %zu relations, %llu chunks, %.3f s (device calls %.3f s)",
-                      w + 1, n, (unsigned long long)st.chunks, st.wall_s, st.device_call_s);
-        log_line("INFO", msg);
+        char msg[160];
+        std::snprintf(msg, sizeof(msg), "%llu chunks, %.3f s (device calls %.3f s)", (unsigned long long)st.chunks,
+                      st.wall_s, st.device_call_s);
+        write_item(it, w, items.size(), rels, msg);
     }
     const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     log_line("INFO", "Siever completed successfully, total elapsed " + std::to_string((long long)el) + " s");

@@ -54,3 +54,28 @@ def test_cli_two_items_append_results(tmp_path, oracle):
         oit = oracle.make_item(*full)
         rels, _ = oracle.process_range(oit, full[2], full[3], 1 << 20, 8)
         assert obj["prp"] == json.loads(oracle.results_json(oit, rels, "t"))["prp"]
+
+
+@pytest.mark.parametrize("mode", ["GPU_batch=1\n", "GPU_autotune=1\n", "GPU_batch=1\nGPU_autotune=1\n"])
+def test_cli_batch_and_autotune_modes_match_sequential(tmp_path, mode):
+    """GPU_batch=1 (every item through the batch engine) and GPU_autotune=1 give the same
+    relations.p and results.json objects, in worktodo order, as the plain sequential run."""
+    items = [(10, 111, 10 ** 9, 10 ** 9 + 2 ** 21, 0, 2_000_000, 6, 4), (3, -2, 10 ** 9, 10 ** 9 + 2 ** 20, 0, 500_000),
+             (2, -25, 10 ** 10, 10 ** 10 + 2 ** 21, 3_600_000, 5_000_000, 2, 1)]
+    lines = ["Factor=" + ",".join(map(str, it)) for it in items]
+    base = "CPU_count=0\nMemoryAllowed=8\nUseGPU=1\nGPU_sieve=1\nGPU_classes=1\n"
+    seq, alt = tmp_path / "seq", tmp_path / "alt"
+    seq.mkdir()
+    alt.mkdir()
+    print(f"cli mode {mode!r}", flush=True)
+    r1 = run_siever(seq, lines, base)
+    r2 = run_siever(alt, lines, base + mode)
+    assert r1.returncode == 0 and r2.returncode == 0, (r1.stderr[-1000:], r2.stderr[-1000:])
+    assert (seq / "relations.p").read_text() == (alt / "relations.p").read_text()
+    a = json.loads((seq / "results.json").read_text())
+    b = json.loads((alt / "results.json").read_text())
+    assert len(a) == len(b) == 3
+    for x, y in zip(a, b):
+        x.pop("Timestamp", None)
+        y.pop("Timestamp", None)
+        assert x == y
