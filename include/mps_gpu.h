@@ -31,8 +31,9 @@ typedef struct mps_item {
 /* Status codes.  DECLINED: this chunk must take the caller's CPU path (reference: sieve_filter
  * returning false for a range, table or buffer limit, Gpu.hpp:858, :875, :925).  ERROR: a device
  * error; the GPU sieve stays disabled for the rest of the process (sv_fail, Gpu.hpp:982-987).
- * CAPACITY: the output buffer is too small; *out_n holds the count needed. */
-enum { MPS_OK = 0, MPS_DECLINED = 1, MPS_ERROR = 2, MPS_CAPACITY = 3 };
+ * CAPACITY: the output buffer is too small; *out_n holds the count needed.  STOPPED: the caller
+ * asked the run to stop (mps_run_opts.stop); the chunks that finished are reported. */
+enum { MPS_OK = 0, MPS_DECLINED = 1, MPS_ERROR = 2, MPS_CAPACITY = 3, MPS_STOPPED = 4 };
 
 typedef struct mps_gpu mps_gpu;
 

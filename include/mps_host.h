@@ -41,9 +41,19 @@ typedef struct mps_relation {
     uint64_t q;
 } mps_relation;
 
+/* Called once per finished chunk [lo, hi) with its relations (sorted by p), from the feeder
+ * thread that ran it (feeder = 0 .. feeder_threads - 1): the hook of the reference's per-worker
+ * core_<id>.p files (Worker.hpp:294-297), which checkpointing relies on. */
+typedef void (*mps_chunk_cb)(void* ctx, int feeder, uint64_t lo, uint64_t hi, const mps_relation* rels, size_t n);
+
 typedef struct mps_run_opts {
     uint64_t chunk_ints; /* integers per chunk (Worker.hpp:152-157); 0 = 2^26 */
     int feeder_threads;  /* GPU_threads (Config.hpp:23): chunks in flight; 0 = 2 */
+    mps_chunk_cb on_chunk;       /* may be NULL */
+    void* ctx;
+    const volatile int* stop;    /* when non-NULL and *stop != 0, no further chunk starts
+                                    (shutdown_requested, Worker.hpp:324) and the run returns
+                                    MPS_STOPPED */
 } mps_run_opts;
 
 typedef struct mps_run_stats {
@@ -59,7 +69,9 @@ typedef struct mps_run_stats {
 
 /* Runs [P0, P1) of one item through the GPU worker and the exact re-check.  Relations are
  * returned sorted by p.  Returns MPS_OK, MPS_DECLINED (some chunk was declined; the relations
- * of the other chunks are still returned), MPS_ERROR or MPS_CAPACITY (*nrel = count needed). */
+ * of the other chunks are still returned), MPS_STOPPED (the stop flag was raised; up to cap
+ * relations of the finished chunks are returned, *nrel = their count), MPS_ERROR or MPS_CAPACITY
+ * (*nrel = count needed). */
 int mps_run_item(mps_gpu* g, const mps_work_item* item, const mps_run_opts* opts, mps_relation* rels,
                  size_t cap, size_t* nrel, mps_run_stats* stats);
 

@@ -155,12 +155,21 @@ int mps_run_item(mps_gpu* g, const mps_work_item* item, const mps_run_opts* opts
     std::mutex mu;
     ChunkResult total;
     const double t0 = now_s();
-    auto feeder = [&]() {
+    const mps_chunk_cb on_chunk = opts ? opts->on_chunk : nullptr;
+    void* const ctx = opts ? opts->ctx : nullptr;
+    const volatile int* const stop = opts ? opts->stop : nullptr;
+    std::atomic<bool> stopped{false};
+    auto feeder = [&](int id) {
         std::vector<mps_dev_relation> buf(1 << 14);
         ChunkResult mine;
         for (size_t k; (k = next.fetch_add(1)) < chunks.size();) {
+            if (stop && *stop) {  // no further chunk starts (Worker.hpp:324); chunk k stays undone
+                stopped = true;
+                break;
+            }
             ChunkResult r;
             process_chunk(g, item, chunks[k].first, chunks[k].second, buf, r);
+            if (on_chunk && r.status == MPS_OK) on_chunk(ctx, id, chunks[k].first, chunks[k].second, r.rels.data(), r.rels.size());
             mine.rels.insert(mine.rels.end(), r.rels.begin(), r.rels.end());
             mine.dev_s += r.dev_s;
             mine.chk_s += r.chk_s;
@@ -180,8 +189,8 @@ int mps_run_item(mps_gpu* g, const mps_work_item* item, const mps_run_opts* opts
         total.prime_flagged += mine.prime_flagged;
     };
     std::vector<std::thread> th;
-    for (int i = 1; i < feeders; ++i) th.emplace_back(feeder);
-    feeder();
+    for (int i = 1; i < feeders; ++i) th.emplace_back(feeder, i);
+    feeder(0);
     for (auto& t : th) t.join();
     std::sort(total.rels.begin(), total.rels.end(),
               [](const mps_relation& a, const mps_relation& b) { return a.p < b.p; });
@@ -199,6 +208,7 @@ int mps_run_item(mps_gpu* g, const mps_work_item* item, const mps_run_opts* opts
         stats->relations = total.rels.size();
     }
     if (total.status != MPS_OK) return total.status;
+    if (stopped) return MPS_STOPPED;
     return total.rels.size() > cap ? MPS_CAPACITY : MPS_OK;
 }
 

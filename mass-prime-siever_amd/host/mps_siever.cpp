@@ -6,6 +6,7 @@
 // the device declines stops the run (exit 1), as GPU_strict=1 does in the reference.
 #include <algorithm>
 #include <cctype>
+#include <csignal>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -13,6 +14,7 @@
 #include <ctime>
 #include <fstream>
 #include <map>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <thread>
@@ -217,6 +219,123 @@ void append_results(const std::string& obj) {  // Results.hpp:109-139, as a comp
     f << out << '\n';
 }
 
+// ---- checkpointing through per-feeder core files (Worker.hpp:294-297, :369-463)
+
+// Raised by SIGINT / SIGTERM / SIGHUP (Siever.cpp:67-90): no further chunk starts, the finished
+// chunks stay in core_<id>.p and the next run resumes from them.
+volatile sig_atomic_t g_stop = 0;
+static_assert(sizeof(sig_atomic_t) == sizeof(int), "the stop flag is read as an int");
+
+extern "C" void on_shutdown_signal(int) { g_stop = 1; }
+
+std::string core_path(int id) { return "core_" + std::to_string(id) + ".p"; }
+
+// Core files in id order; like the reference's scan, a missing id ends the scan only when the
+// next id is missing too.
+std::vector<int> core_ids() {
+    std::vector<int> ids;
+    for (int id = 0;; ++id) {
+        if (std::ifstream(core_path(id))) {
+            ids.push_back(id);
+        } else if (!std::ifstream(core_path(id + 1))) {
+            break;
+        }
+    }
+    return ids;
+}
+
+bool parse_relation(const std::string& line, mps_relation& r) {
+    unsigned long long p = 0, M = 0, q = 0;
+    if (std::sscanf(line.c_str(), "%llu, %llu, %llu", &p, &M, &q) != 3) return false;
+    r.p = p;
+    r.M = M;
+    r.q = q;
+    return true;
+}
+
+// read_checkpoint: the resume point is the smallest last-line p over the core files; lines at or
+// above it are dropped (their chunks run again).  Returns the relations kept, or 0 when there is no
+// checkpoint ahead of P0 (the item then starts from P0).
+uint64_t read_checkpoint(mps_work_item& it) {
+    uint64_t new_p0 = it.P1;
+    bool any = false;
+    const std::vector<int> ids = core_ids();
+    for (int id : ids) {
+        std::ifstream in(core_path(id));
+        std::string line, last;
+        while (std::getline(in, line))
+            if (!line.empty()) last = line;
+        mps_relation r;
+        if (!last.empty() && parse_relation(last, r) && r.p < new_p0) {
+            new_p0 = r.p;
+            any = true;
+        }
+    }
+    if (!any) {
+        log_line("INFO", "No checkpoint found, starting from original P0.");
+        return 0;
+    }
+    if (new_p0 <= it.P0) {
+        log_line("INFO", "Checkpoint not ahead of current P0, starting from original P0.");
+        return 0;
+    }
+    uint64_t kept_total = 0;
+    for (int id : ids) {
+        std::vector<std::string> kept;
+        {
+            std::ifstream in(core_path(id));
+            std::string line;
+            mps_relation r;
+            while (std::getline(in, line))
+                if (!line.empty() && parse_relation(line, r) && r.p < new_p0) kept.push_back(line);
+        }
+        std::ofstream out(core_path(id), std::ios::trunc);
+        if (!out) fatal("Failed to trim checkpoint file: " + core_path(id));
+        for (const std::string& l : kept) out << l << '\n';
+        kept_total += kept.size();
+    }
+    it.P0 = new_p0;
+    log_line("INFO", "Checkpoint recovered: new P0 = " + std::to_string(new_p0) + ", relations restored = " +
+                         std::to_string(kept_total));
+    return kept_total;
+}
+
+struct CoreSink {
+    int first_id;  // core_<first_id + feeder>.p, as core_<cpu_count + g>.p (Worker.hpp:340)
+    std::mutex mu[64];
+};
+
+// Appends one finished chunk's relations to its feeder's core file.
+extern "C" void append_core(void* ctx, int feeder, uint64_t, uint64_t, const mps_relation* rels, size_t n) {
+    if (n == 0) return;
+    CoreSink* sink = static_cast<CoreSink*>(ctx);
+    std::lock_guard<std::mutex> l(sink->mu[feeder & 63]);
+    FILE* f = std::fopen(core_path(sink->first_id + feeder).c_str(), "ab");
+    if (!f) fatal("Failed to append to " + core_path(sink->first_id + feeder));
+    for (size_t i = 0; i < n; ++i)
+        std::fprintf(f, "%llu, %llu, %llu\n", (unsigned long long)rels[i].p, (unsigned long long)rels[i].M,
+                     (unsigned long long)rels[i].q);
+    std::fclose(f);
+}
+
+// merge_core_files + load_relations (Worker.hpp:465-515): every core file's relations, sorted by p.
+std::vector<mps_relation> load_core_relations() {
+    std::vector<mps_relation> rels;
+    for (int id : core_ids()) {
+        std::ifstream in(core_path(id));
+        std::string line;
+        mps_relation r;
+        while (std::getline(in, line))
+            if (!line.empty() && parse_relation(line, r)) rels.push_back(r);
+    }
+    std::sort(rels.begin(), rels.end(), [](const mps_relation& a, const mps_relation& b) { return a.p < b.p; });
+    return rels;
+}
+
+void remove_core_files() {
+    for (int id : core_ids()) std::remove(core_path(id).c_str());
+}
+
 // relations.p, the results.json object and the log line of one finished item.
 void write_item(const mps_work_item& it, size_t w, size_t nitems, const std::vector<mps_relation>& rels,
                 const std::string& stats) {
@@ -273,6 +392,14 @@ void run_batched(mps_gpu* gpu, const std::vector<mps_work_item>& items) {
 }  // namespace
 
 int main() {
+    {
+        struct sigaction action {};
+        action.sa_handler = on_shutdown_signal;
+        sigemptyset(&action.sa_mask);
+        sigaction(SIGINT, &action, nullptr);
+        sigaction(SIGTERM, &action, nullptr);
+        sigaction(SIGHUP, &action, nullptr);
+    }
     const auto t0 = std::chrono::steady_clock::now();
     g_log = std::fopen("siever.log", "a");
     log_line("INFO", "Mass Prime Siever (MI355X worker) starting");
@@ -313,28 +440,40 @@ int main() {
     if (cfg.gpu_batch) {
         run_batched(gpu, items);
     }
+    CoreSink sink;
+    sink.first_id = std::max(0, cfg.cpu_count);
+    bool stopped = false;
     for (size_t w = 0; w < items.size() && !cfg.gpu_batch; ++w) {
+        if (g_stop) {
+            log_line("INFO", "Shutdown requested, stopping before next work item.");
+            stopped = true;
+            break;
+        }
         const mps_work_item& it = items[w];
         log_line("INFO", "Processing work item " + std::to_string(w + 1) + "/" + std::to_string(items.size()) +
                              " b=" + std::to_string(it.b) + " c=" + std::to_string(it.c));
-        mps_run_opts opts{chunk, cfg.gpu_threads};
+        std::remove("relations.p");
+        mps_work_item run = it;  // resumes from the checkpoint, if any
+        read_checkpoint(run);
+        mps_run_opts opts{chunk, cfg.gpu_threads, append_core, &sink, (const volatile int*)&g_stop};
         mps_run_stats st{};
-        std::vector<mps_relation> rels(1 << 16);
         size_t n = 0;
-        int rc = mps_run_item(gpu, &it, &opts, rels.data(), rels.size(), &n, &st);
-        if (rc == MPS_CAPACITY) {
-            rels.resize(n);
-            rc = mps_run_item(gpu, &it, &opts, rels.data(), rels.size(), &n, &st);
+        const int rc = mps_run_item(gpu, &run, &opts, nullptr, 0, &n, &st);
+        if (rc == MPS_STOPPED) {
+            log_line("INFO", "Shutdown: checkpoint preserved in core_*.p files");
+            stopped = true;
+            break;
         }
-        if (rc != MPS_OK) fatal(std::string("GPU worker failed on this item: ") + mps_gpu_last_error(gpu));
-        rels.resize(n);
+        if (rc != MPS_OK && rc != MPS_CAPACITY) fatal(std::string("GPU worker failed on this item: ") + mps_gpu_last_error(gpu));
+        const std::vector<mps_relation> rels = load_core_relations();
         char msg[160];
         std::snprintf(msg, sizeof(msg), "%llu chunks, %.3f s (device calls %.3f s)", (unsigned long long)st.chunks,
                       st.wall_s, st.device_call_s);
         write_item(it, w, items.size(), rels, msg);
+        remove_core_files();
     }
     const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    log_line("INFO", "Siever completed successfully, total elapsed " + std::to_string((long long)el) + " s");
+    if (!stopped) log_line("INFO", "Siever completed successfully, total elapsed " + std::to_string((long long)el) + " s");
     mps_gpu_destroy(gpu);
     if (g_log) std::fclose(g_log);
     return 0;

@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libmps.so")
 
-MPS_OK, MPS_DECLINED, MPS_ERROR, MPS_CAPACITY = 0, 1, 2, 3
+MPS_OK, MPS_DECLINED, MPS_ERROR, MPS_CAPACITY, MPS_STOPPED = 0, 1, 2, 3, 4
 
 
 class Item(ctypes.Structure):
@@ -43,7 +43,8 @@ HOST_DECIDES = 0xFFFFFFFFFFFFFFFF
 
 
 class RunOpts(ctypes.Structure):
-    _fields_ = [("chunk_ints", ctypes.c_uint64), ("feeder_threads", ctypes.c_int)]
+    _fields_ = [("chunk_ints", ctypes.c_uint64), ("feeder_threads", ctypes.c_int), ("on_chunk", ctypes.c_void_p),
+                ("ctx", ctypes.c_void_p), ("stop", ctypes.c_void_p)]
 
 
 class RunStats(ctypes.Structure):
@@ -324,7 +325,7 @@ class GpuSiever:
             rels = (Relation * cap)()
             n = ctypes.c_size_t()
             st = RunStats()
-            opts = RunOpts(chunk_ints, feeder_threads)
+            opts = RunOpts(chunk_ints, feeder_threads, None, None, None)
             rc = lib().mps_run_item(self.h, ctypes.byref(item), ctypes.byref(opts), rels, cap, ctypes.byref(n),
                                     ctypes.byref(st))
             if rc == MPS_CAPACITY:

@@ -79,3 +79,66 @@ def test_cli_batch_and_autotune_modes_match_sequential(tmp_path, mode):
         x.pop("Timestamp", None)
         y.pop("Timestamp", None)
         assert x == y
+
+
+def test_cli_resumes_from_core_files(tmp_path, oracle):
+    """read_checkpoint (Worker.hpp:369-463): the run resumes from the smallest last-line p of the
+    core files, drops the lines at or above it, and the merged relations equal a full run's."""
+    item = (10, 111, 10 ** 9, 10 ** 9 + 2 ** 22, 0, 2_000_000, 6, 4)
+    oit = oracle.make_item(*item)
+    full, _ = oracle.process_range(oit, item[2], item[3], 1 << 20, 8)
+    assert len(full) > 8
+    k = len(full) // 2
+    lines = [f"{p}, {M}, {q}" for p, M, q in full]
+    (tmp_path / "core_0.p").write_text("\n".join(lines[:k + 1]) + "\n")         # last p = full[k]
+    (tmp_path / "core_1.p").write_text("\n".join(lines[k + 1:k + 4]) + "\n")    # later p: trimmed away
+    r = run_siever(tmp_path, ["Factor=" + ",".join(map(str, item))],
+                   "CPU_count=0\nMemoryAllowed=1\nUseGPU=1\nGPU_sieve=1\nGPU_classes=1\n")
+    assert r.returncode == 0, r.stderr[-2000:]
+    log = (tmp_path / "siever.log").read_text()
+    assert f"Checkpoint recovered: new P0 = {full[k][0]}, relations restored = {k}" in log
+    assert sorted((tmp_path / "relations.p").read_text().split()) == sorted(" ".join(lines).split())
+    res = json.loads((tmp_path / "results.json").read_text())
+    assert res[0]["P0"] == item[2]
+    assert res[0]["prp"] == json.loads(oracle.results_json(oit, full, "t"))["prp"]
+    assert not list(tmp_path.glob("core_*.p"))
+
+
+def test_cli_sigterm_keeps_checkpoint_and_resumes(tmp_path):
+    """SIGTERM (Siever.cpp:67-90): no further chunk starts, core_*.p keep the finished chunks, exit
+    0; the next run resumes and its relations equal an uninterrupted run's."""
+    import signal
+    import time
+    item = "Factor=2,-25,1000000000,9589934592,3600000,5000000,2,1"  # 2^33 integers, 128 chunks
+    ini = "CPU_count=0\nMemoryAllowed=64\nUseGPU=1\nGPU_sieve=1\nGPU_classes=1\n"
+    full_dir, cut_dir = tmp_path / "full", tmp_path / "cut"
+    full_dir.mkdir()
+    cut_dir.mkdir()
+    r = run_siever(full_dir, [item], ini)
+    assert r.returncode == 0, r.stderr[-2000:]
+    (cut_dir / "worktodo.txt").write_text(item + "\n")
+    (cut_dir / "prime.ini").write_text(ini)
+    proc = subprocess.Popen([BIN], cwd=cut_dir, stdin=subprocess.DEVNULL, stdout=subprocess.PIPE,
+                            stderr=subprocess.PIPE, text=True)
+    log = cut_dir / "siever.log"
+    t0 = time.time()
+    while time.time() - t0 < 120 and not (log.exists() and "Processing work item" in log.read_text()):
+        time.sleep(0.01)
+    time.sleep(0.3)
+    proc.send_signal(signal.SIGTERM)
+    out, err = proc.communicate(timeout=120)
+    assert proc.returncode == 0, err[-2000:]
+    text = log.read_text()
+    assert "Shutdown: checkpoint preserved in core_*.p files" in text
+    assert list(cut_dir.glob("core_*.p"))
+    assert not (cut_dir / "results.json").exists()
+    r = run_siever(cut_dir, [item], ini)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "Checkpoint recovered" in (cut_dir / "siever.log").read_text()
+    assert sorted((cut_dir / "relations.p").read_text().split("\n")) == \
+        sorted((full_dir / "relations.p").read_text().split("\n"))
+    a = json.loads((cut_dir / "results.json").read_text())
+    b = json.loads((full_dir / "results.json").read_text())
+    a[0].pop("Timestamp")
+    b[0].pop("Timestamp")
+    assert a == b
