@@ -51,8 +51,11 @@ void mps_gpu_set_classes(mps_gpu* g, int on);
 /* Which items use the per-prime residue classes when classes are on (no reference counterpart;
  * the reference's rule is gpu_classes, Math.hpp:324-337):
  *   0 = as the reference: an even class modulus cm with lcm(cm, 24) <= 48 and N1 - N0 >= 4096 lcm;
- *   1 = any cm with lcm(cm, 24) <= 48 and the same span (default).
- * Relations are identical under both; only the device work differs. */
+ *   1 = any cm with lcm(cm, 24) <= 48 and the same span (default);
+ *   2 = order mode: every item (classes on, lcm(cm, 367567200) < 2^32) gets one exponent class per
+ *       prime, n == n* (mod L), from the power residues of b and -c modulo gcd(lcm(cm, 367567200),
+ *       p - 1) (Pohlig-Hellman on the device); other items as under 1.
+ * Relations are identical under all three; only the device work differs. */
 int mps_gpu_set_class_policy(mps_gpu* g, int policy);
 
 /* GpuSiever::sieve_filter (Gpu.hpp:1064-1067, OpenClSiever::sieve_filter Gpu.hpp:857-979).
@@ -129,16 +132,19 @@ int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap, 
  * mps_gpu_set_tuning a zero field keeps the handle's default. */
 typedef struct mps_tuning {
     uint32_t waves;   /* wavefronts that share one BSGS entry: 1, 2 or 4 (0 = by item) */
-    uint32_t mcap;    /* plain mode: largest baby-step count, 256..4096 (default 2048) */
+    uint32_t mcap;    /* plain and order mode: largest baby-step count, 256..4096 (default 2048) */
     uint32_t cmax;    /* class mode: largest baby-step count, 256..2048 (default 1024) */
     uint32_t load;    /* keys per 8-slot LDS table bucket the table is sized for: 2 or 4 (default 2) */
+    uint32_t policy;  /* class policy for this kind of item (see mps_gpu_set_class_policy), 0 = the
+                         handle's */
 } mps_tuning;
 
-/* Autotuner.  Items are keyed by class mode and kind of class modulus, p magnitude (log2 P0 in
- * steps of 6) and exponent span per class modulus (log2 in steps of 2).  For the key of `item`
- * whose P range starts at P0, times every candidate shape (waves 1/2/4 x load 2/4 x the baby-step
- * cap of its mode: 1024/2048/4096 plain, 512/1024/2048 class mode; shapes whose table exceeds
- * 64 KB of LDS are skipped) on one pass over the first 2 * sample_odds integers from P0 (device
+/* Autotuner.  Items are keyed by whether classes are on, kind of class modulus, p magnitude
+ * (log2 P0 in steps of 6) and exponent span per class modulus (log2 in steps of 2).  For the key
+ * of `item` whose P range starts at P0, times every candidate shape (with classes on: class
+ * policy 1 or 2; waves 1/2/4 x load 2/4 x the baby-step cap of the resulting mode: 1024/2048/4096
+ * plain and order mode, 512/1024/2048 class mode; shapes whose table exceeds 64 KB of LDS are
+ * skipped) on one pass over the first 2 * sample_odds integers from P0 (device
  * time of the pass chain, best of two runs), requires every candidate's records to agree with the
  * default's (the same smallest solution wherever both decided a number; a number may be left to
  * the host under one shape only) and installs the fastest for the key.  best, best_ms, default_ms

@@ -174,7 +174,7 @@ void free_slot(Slot& s) {
     hipHostFree(s.h_out);
 }
 
-PassChain slot_chain(const mps_batch* b, const Slot& s, FieldKind fk, bool qmode, const TableGeom& tg) {
+PassChain slot_chain(const mps_batch* b, const Slot& s, FieldKind fk, uint32_t mode, const TableGeom& tg) {
     PassChain c{};
     c.d_desc = s.d_desc;
     c.h_desc = s.h_desc;
@@ -190,24 +190,24 @@ PassChain slot_chain(const mps_batch* b, const Slot& s, FieldKind fk, bool qmode
     c.grid_stage = b->grid_stage;
     c.grid_bsgs = bsgs_persistent_grid(fk, tg, b->g->num_cu);
     c.fk = fk;
-    c.qmode = qmode;
+    c.mode = mode;
     c.tg = tg;
     return c;
 }
 
 // Puts one pass on the slot's stream: a replay of the variant's graph (built from explicit
 // nodes on first use), or the chain's kernels one by one.
-bool launch_pass(mps_batch* b, Slot& s, FieldKind fk, bool qmode, const TableGeom& tg) {
+bool launch_pass(mps_batch* b, Slot& s, FieldKind fk, uint32_t mode, const TableGeom& tg) {
     if (!b->o.use_graphs) {
-        BCHK(enqueue_pass_chain(s.stream, slot_chain(b, s, fk, qmode, tg)), "pass launch");
+        BCHK(enqueue_pass_chain(s.stream, slot_chain(b, s, fk, mode, tg)), "pass launch");
         return true;
     }
-    const uint64_t key = (uint64_t)fk | (uint64_t)qmode << 1 | (uint64_t)tg.waves << 2 | (uint64_t)tg.nbbits << 8;
+    const uint64_t key = (uint64_t)fk | (uint64_t)mode << 1 | (uint64_t)tg.waves << 4 | (uint64_t)tg.nbbits << 8;
     auto it = s.graphs.find(key);
     if (it == s.graphs.end()) {
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
-        BCHK(build_pass_graph(slot_chain(b, s, fk, qmode, tg), &graph), "graph build");
+        BCHK(build_pass_graph(slot_chain(b, s, fk, mode, tg), &graph), "graph build");
         const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         BCHK(ie, "graph instantiate");
@@ -351,7 +351,7 @@ int run_chunk(mps_batch* b, Slot& s, const Task& t, std::vector<mps_relation>& r
     s.h_desc->ip = it->ip;
     s.h_desc->tg = it->tg;
     const double t0 = now_s();
-    if (!launch_pass(b, s, fk, it->ip.qmode != 0, it->tg)) return MPS_ERROR;
+    if (!launch_pass(b, s, fk, it->ip.qmode, it->tg)) return MPS_ERROR;
     hipError_t e = hipStreamSynchronize(s.stream);
     if (e != hipSuccess) return batch_fail(b, e, "pass" + chunk_name(t)), MPS_ERROR;
     const uint32_t n_out = s.h_cnt->n_out;

@@ -94,6 +94,14 @@ bool class_mode(const mps_item* it, bool classes_on, int class_policy, uint64_t&
     return (class_policy == 1 || (cm & 1) == 0) && Q <= 48 && span / Q >= 4096;
 }
 
+// Order mode (class policy 2, classes on): one class per prime from the power residues of b and t
+// modulo Qo = lcm(cm, kOrderQ), when Qo fits 32 bits.
+bool order_mode(const mps_item* it, bool classes_on, int class_policy, uint64_t& Qo) {
+    const uint64_t cm = it->cls_mod == 0 ? 1 : it->cls_mod;
+    Qo = cm / gcd_u64(cm, kOrderQ) * kOrderQ;
+    return classes_on && class_policy == 2 && it->N0 >= 0 && it->N1 > it->N0 && Qo < (1ull << 32);
+}
+
 bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t cmax,
                  uint32_t waves, ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why) {
     std::memset(&ip, 0, sizeof(ip));
@@ -112,17 +120,45 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
     ip.cm_r32 = cm < 65536 ? (uint32_t)((1ull << 32) % cm) : 0;
     ip.mcap = mcap;
     ip.lanes = L;
-    uint64_t Q = 0;
-    const bool qmode = class_mode(it, classes_on, class_policy, Q);
+    uint64_t Q = 0, Qo = 0;
+    const bool omode = order_mode(it, classes_on, class_policy, Qo);
+    const bool qmode = !omode && class_mode(it, classes_on, class_policy == 2 ? 1 : class_policy, Q);
     if (qmode) {
         ip.Qmax = Q;
         ip.n0modQ = (uint32_t)(N0 % Q);
         ip.Q_r32 = (uint32_t)((1ull << 32) % Q);
     }
-    ip.qmode = qmode ? 1 : 0;
+    ip.qmode = omode ? kModeOrder : qmode ? kModeClass : kModePlain;
     m_max = 1;
     csz.assign(49 * 32, make_uint2(1, L));
-    if (qmode) {
+    if (omode) {
+        ip.Qo = (uint32_t)Qo;
+        ip.n0modQo = (uint32_t)(N0 % Qo);
+        uint64_t q = Qo;
+        for (uint32_t r = 2; q > 1 && ip.nopr < 10; ++r) {
+            if (q % r) continue;
+            uint32_t ra = 1;
+            while (q % r == 0) {
+                q /= r;
+                ra *= r;
+            }
+            if (ra >= 65536) {
+                why = "class modulus with a prime power above 2^16";
+                return false;
+            }
+            ip.opr[ip.nopr] = r;
+            ip.oprpow[ip.nopr++] = ra;
+        }
+        if (q > 1) {
+            why = "class modulus with more than 10 prime factors";
+            return false;
+        }
+        // an entry's class has at most kcount members: m <= sqrt(kcount) in whole rounds, <= mcap
+        uint64_t m = 1;
+        while (m * m < ip.kcount) ++m;
+        m = (m + L - 1) / L * L;
+        m_max = std::max<uint64_t>(1, std::min<uint64_t>(m, mcap));
+    } else if (qmode) {
         const uint64_t span = N1 - N0;
         for (uint32_t qp = 1; qp <= 48; ++qp) {
             if (ip.Qmax % qp != 0 || qp % cm != 0) continue;
@@ -159,19 +195,18 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
 
 uint32_t tune_key(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on) {
     const uint64_t cm = item->cls_mod == 0 ? 1 : item->cls_mod;
-    uint64_t Q = 0;
-    const bool qmode = class_mode(item, classes_on, g->class_policy, Q);
     const uint64_t span = item->N1 > item->N0 && item->N0 >= 0 ? (uint64_t)(item->N1 - item->N0) : 0;
     const uint32_t kind = cm == 1 ? 0u : (cm & 1) ? 2u : 1u;
     const uint32_t pbits = 64 - __builtin_clzll(P0 | 1), sbits = 64 - __builtin_clzll(span / cm | 1);
-    return (qmode ? 1u : 0u) | kind << 1 | (pbits / 6) << 4 | (sbits / 2) << 8 | (g->field_mode == 0 ? 1u << 16 : 0u);
+    return (classes_on ? 1u : 0u) | kind << 2 | (pbits / 6) << 4 | (sbits / 2) << 10 |
+           (g->field_mode == 0 ? 1u << 16 : 0u);
 }
 
 // The candidate shape the autotuner is timing, seen only by the autotuning thread.
 thread_local const mps_tuning* t_candidate = nullptr;
 
 mps_tuning item_tuning(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on) {
-    mps_tuning t{g->waves, g->mcap, kClassMaxM, 2};
+    mps_tuning t{g->waves, g->mcap, kClassMaxM, 2, (uint32_t)g->class_policy};
     std::lock_guard<std::mutex> l(g->tune_mu);
     const mps_tuning* o = nullptr;
     if (t_candidate) {
@@ -185,6 +220,7 @@ mps_tuning item_tuning(const mps_gpu* g, const mps_item* item, uint64_t P0, bool
         if (o->mcap) t.mcap = o->mcap;
         if (o->cmax) t.cmax = o->cmax;
         if (o->load) t.load = o->load;
+        if (o->policy) t.policy = o->policy;
     }
     return t;
 }
@@ -192,15 +228,15 @@ mps_tuning item_tuning(const mps_gpu* g, const mps_item* item, uint64_t P0, bool
 bool plan_item(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on, Plan& pl, std::string& why) {
     const mps_tuning t = item_tuning(g, item, P0, classes_on);
     uint32_t waves = t.waves;
+    const int policy = (int)t.policy;
     if (waves == 0) {
         // measured on MI355X: two wavefronts per entry for the class tables (m <= 1024) and short
         // plain ranges; four when plain-mode entries run up to 2048 baby steps
-        if (!item_params(item, classes_on, g->class_policy, t.mcap, t.cmax, 2, pl.ip, pl.m_max, pl.csz, why))
-            return false;
-        waves = pl.m_max > 1024 ? 4 : 2;
+        if (!item_params(item, classes_on, policy, t.mcap, t.cmax, 2, pl.ip, pl.m_max, pl.csz, why)) return false;
+        // order-mode entries are mostly short (their classes are narrow), whatever the item's cap
+        waves = pl.m_max > 1024 && pl.ip.qmode != kModeOrder ? 4 : 2;
     }
-    if (!item_params(item, classes_on, g->class_policy, t.mcap, t.cmax, waves, pl.ip, pl.m_max, pl.csz, why))
-        return false;
+    if (!item_params(item, classes_on, policy, t.mcap, t.cmax, waves, pl.ip, pl.m_max, pl.csz, why)) return false;
     pl.tg = table_geom(pl.m_max, waves, t.load);
     if (bsgs_lds_bytes(pl.tg) > kMaxBsgsLds) {
         why = "BSGS table would exceed the LDS budget";
@@ -314,7 +350,7 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (stop_stage == 1 || n_cand == 0) return true;
 
     HIPCHK(hipEventRecord(g->ev[2], s), "event");
-    launch_setup(s, fk, ip.qmode != 0, d, g->pb, blocks_for(g, n_cand, 256));
+    launch_setup(s, fk, ip.qmode, d, g->pb, blocks_for(g, n_cand, 256));
     HIPCHK(hipGetLastError(), "setup launch");
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
     if (!read_counts(g)) return false;
@@ -425,7 +461,7 @@ void mps_gpu_set_classes(mps_gpu* g, int on) {
 }
 
 int mps_gpu_set_class_policy(mps_gpu* g, int policy) {
-    if (g == nullptr || (policy != 0 && policy != 1)) return MPS_ERROR;
+    if (g == nullptr || policy < 0 || policy > 2) return MPS_ERROR;
     std::lock_guard<std::mutex> lock(g->mu);
     g->class_policy = policy;
     return MPS_OK;
@@ -709,7 +745,8 @@ namespace {
 
 bool valid_tuning(const mps_tuning& t) {
     return (t.waves == 0 || t.waves == 1 || t.waves == 2 || t.waves == 4) && (t.mcap == 0 || (t.mcap >= 256 && t.mcap <= 4096)) &&
-           (t.cmax == 0 || (t.cmax >= 256 && t.cmax <= 2048)) && (t.load == 0 || t.load == 2 || t.load == 4);
+           (t.cmax == 0 || (t.cmax >= 256 && t.cmax <= 2048)) && (t.load == 0 || t.load == 2 || t.load == 4) &&
+           t.policy <= 2;
 }
 
 double device_ms(const mps_gpu* g) { return g->st.ms_sieve + g->st.ms_filter + g->st.ms_setup + g->st.ms_bsgs; }
@@ -770,7 +807,7 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
     };
     // the defaults (no table entry for the key) give the reference records and time
     const uint32_t key = tune_key(g, item, P0, g->classes_on);
-    force(true, mps_tuning{g->waves, g->mcap, kClassMaxM, 2});
+    force(true, mps_tuning{g->waves, g->mcap, kClassMaxM, 2, (uint32_t)g->class_policy});
     std::vector<uint64_t> ref, rec;
     double ms = 0, t_default = 1e300;
     int st = MPS_OK;
@@ -783,14 +820,23 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
         g->st = keep;
         return st;
     }
-    uint64_t Q = 0;
-    const bool qmode = class_mode(item, g->classes_on, g->class_policy, Q);
-    mps_tuning win{0, 0, 0, 0};
+    mps_tuning win{0, 0, 0, 0, 0};
     double t_win = t_default;
+    // with classes on, both class policies that change the setup are candidates; policy 2 only
+    // where the item has an order-mode reduction (otherwise it runs as policy 1)
+    std::vector<uint32_t> policies{(uint32_t)g->class_policy};
+    uint64_t Q = 0, Qo = 0;
+    if (g->classes_on) {
+        policies = {1u};
+        if (order_mode(item, true, 2, Qo)) policies.push_back(2u);
+    }
+    for (uint32_t policy : policies) {
+    const bool qmode = !order_mode(item, g->classes_on, (int)policy, Qo) &&
+                       class_mode(item, g->classes_on, policy == 2 ? 1 : (int)policy, Q);
     for (uint32_t waves : {1u, 2u, 4u}) {
         for (uint32_t load : {2u, 4u}) {
             for (uint32_t cap : {0u, 1u, 2u}) {
-                mps_tuning t{waves, qmode ? g->mcap : 1024u << cap, qmode ? 512u << cap : kClassMaxM, load};
+                mps_tuning t{waves, qmode ? g->mcap : 1024u << cap, qmode ? 512u << cap : kClassMaxM, load, policy};
                 Plan pl;
                 std::string why;
                 force(true, t);
@@ -810,7 +856,11 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
                     win = t;
                 }
             }
+            if (st != MPS_OK) break;
         }
+        if (st != MPS_OK) break;
+    }
+    if (st != MPS_OK) break;
     }
     force(false, mps_tuning{});
     g->st = keep;

@@ -14,6 +14,10 @@ constexpr uint32_t kEntWords = 72;                // u64 words per BSGS entry (5
 constexpr uint32_t kMaxClasses = 24;              // classes per prime in class mode
 constexpr uint32_t kClassMaxM = 1024;             // default largest baby-step count in class mode
 constexpr size_t kMaxBsgsLds = 64 * 1024;          // LDS of one BSGS workgroup at most
+// Setup modes (ItemParams.qmode).
+constexpr uint32_t kModePlain = 0, kModeClass = 1, kModeOrder = 2;
+// Order mode: the power-residue modulus every p - 1 is split by, 2^5 3^3 5^2 7 11 13 17.
+constexpr uint64_t kOrderQ = 367567200ull;
 constexpr uint64_t kHostDecides = ~0ull;          // M of a record the host must decide exactly
 
 // Entry layout (u64 words).  B is the baby-step base, Bm = B^m the giant-step base.
@@ -31,9 +35,11 @@ enum EntryWord : uint32_t {
     E_MLO = 24,   // Bm^k, k = 0..7
     E_MHI = 32,   // Bm^(8k), k = 0..7
     E_NB = 40,    // plain: n_start; class mode: N0
-    E_META = 41,  // qp (bits 0..7) | stride 2 (bit 8) | odd k (bit 9) | class mode (bit 10)
+    E_META = 41,  // qp (bits 0..7) | stride 2 (bit 8) | odd k (bit 9) | class mode (bit 10) |
+                  // order mode (bit 11: n = n_start + k * L, L in E_DJ)
     E_KLIM = 42,  // plain: jcount (solutions k = g*m - i must lie in [0, jcount))
-    E_DJ = 43,    // class mode: dj of the kept classes, 8 bits each, 3 words (n_j = N0 + dj)
+    E_DJ = 43,    // class mode: dj of the kept classes, 8 bits each, 3 words (n_j = N0 + dj);
+                  // order mode: the entry's exponent modulus L
     E_BL = 46,    // B^L, L = lanes of the BSGS workgroup (the baby-step stride of one lane)
     E_BML = 47,   // Bm^L (the giant-step stride of one lane)
     E_G = 48,     // giant start per class G_j = b^(n_j), j < s
@@ -58,7 +64,7 @@ struct ItemParams {
     uint64_t kcount;        // in-class n in [n_start, N1) (plain mode)
     uint64_t Qmax;          // class mode: lcm(cm, 24)
     uint32_t n0modQ;        // N0 mod Qmax
-    uint32_t qmode;         // 1 = per-prime classes
+    uint32_t qmode;         // setup mode: 0 plain, 1 per-prime classes, 2 per-prime order (kModeOrder)
     uint32_t cm_r32;        // 2^32 mod cm when cm < 2^16
     uint32_t Q_r32;         // 2^32 mod Qmax
     uint32_t mcap;          // plain mode: largest baby-step count (bounds the LDS table)
@@ -68,12 +74,17 @@ struct ItemParams {
     uint64_t pjc[4];
     // class mode sizes: csz[qp * 32 + s] = (m, giant steps per class)
     const uint2* csz;
+    // order mode: Qo = lcm(cm, kOrderQ), its distinct prime factors and their powers in Qo,
+    // and N0 mod Qo
+    uint32_t Qo, nopr, n0modQo;
+    uint32_t opr[10], oprpow[10];
 };
 
 // BSGS table geometry of a pass: 2^nbbits buckets of 8 slots; a slot holds a fingerprint of
 // fpbits bits over a baby-step index of ibits bits.
 struct TableGeom {
     uint32_t nbbits, fpbits, ibits, waves;
+    uint32_t load;  // keys per bucket the table is sized for; an entry uses the buckets its m needs
 };
 // Everything a pass's kernels need besides the buffers, resident in device memory so that a
 // captured graph of the pass chain can be replayed for any pass by rewriting this descriptor.
@@ -102,7 +113,7 @@ void launch_reset(hipStream_t s, Counters* cnt, uint32_t n_surv0);
 void launch_sieve(hipStream_t s, const PassDesc* d, uint64_t nodd, const uint32_t* sp, const uint32_t* sp_r32,
                   uint32_t nsp, uint32_t nsmall, const PassBuffers& pb, uint32_t grid_override);
 void launch_filter(hipStream_t s, const PassDesc* d, const PassBuffers& pb, uint32_t grid);
-void launch_setup(hipStream_t s, FieldKind fk, bool qmode, const PassDesc* d, const PassBuffers& pb, uint32_t grid);
+void launch_setup(hipStream_t s, FieldKind fk, uint32_t mode, const PassDesc* d, const PassBuffers& pb, uint32_t grid);
 TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load);
 constexpr uint32_t kBsgsWaves = 4;                // default wavefronts of one BSGS workgroup (one entry)
 size_t bsgs_lds_bytes(const TableGeom& tg);
@@ -127,7 +138,7 @@ struct PassChain {
     PassBuffers pb;
     uint32_t grid_sieve, grid_stage, grid_bsgs;
     FieldKind fk;
-    bool qmode;
+    uint32_t mode;  // setup mode of the item (kModePlain / kModeClass / kModeOrder)
     TableGeom tg;
 };
 // Enqueues the chain on stream s.

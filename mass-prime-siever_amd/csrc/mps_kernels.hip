@@ -270,7 +270,7 @@ __global__ __launch_bounds__(256) void k_filter(const PassDesc* __restrict__ d, 
         if (i < n) {
             idx = pb.vals ? i : pb.surv[i];
             p = pb.vals ? pb.vals[i] : lo + 2 * (uint64_t)idx;
-            if (ip.qmode)
+            if (ip.qmode == kModeClass)
                 filter_class(p, ip, keep, flag2, info);
             else
                 filter_plain(p, ip, keep, flag2, info);
@@ -325,8 +325,112 @@ __device__ __forceinline__ typename F::T lane_stride_pow(const F& f, typename F:
     return r;
 }
 
-template <class F, bool QMODE>
+// Inverse of a modulo a small m (m <= 2^16) by stepping through the multiples of a; at most m
+// steps, 0 when gcd(a, m) != 1.
+__device__ __forceinline__ uint32_t inv_small(uint32_t a, uint32_t m) {
+    if (m <= 1) return 0;
+    a %= m;
+    uint32_t acc = a;
+    for (uint32_t u = 1; u < m; ++u) {
+        if (acc == 1u) return u;
+        acc += a;
+        if (acc >= m) acc -= m;
+    }
+    return 0;
+}
+
+// Quotient of an exact division x / d (d odd): x times the inverse of d modulo 2^64.
+__device__ __forceinline__ uint64_t exact_div_odd(uint64_t x, uint64_t d) {
+    uint64_t inv = d;  // d * d == 1 (mod 8): correct to 3 bits, doubled by each Newton step
+    for (int k = 0; k < 5; ++k) inv *= 2 - d * inv;
+    return x * inv;
+}
+
+// Order mode: with g = gcd(Qo, p - 1) and e = (p - 1) / g, b^n == t forces (b^e)^n == t^e in the
+// group of g-th roots of unity.  For each prime power r^a of g (Pohlig-Hellman), x = (b^e)^(g/r^a)
+// and y = (t^e)^(g/r^a) give n == k_r (mod d_r), d_r = ord(x), or no solution when y is not a
+// power of x.  With n == cres (mod cm) the admissible n form one class n == n* (mod L); returns its
+// first member n_start >= N0 and count K below N1 (false: the prime has no solution).  All moduli
+// divide Qo < 2^32; only the exponent e needs 64-bit arithmetic.
+template <class F>
+__device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::T bm, typename F::T tm,
+                                            const ItemParams& ip, uint64_t& L, uint64_t& nst, uint64_t& K,
+                                            unsigned long long& work) {
+    typedef typename F::T T;
+    // g from the valuations of p - 1 at the primes of Qo (prime powers in Qo are below 2^16)
+    uint32_t g = 1;
+    uint32_t gpow[10];
+    for (uint32_t j = 0; j < ip.nopr; ++j) {
+        const uint32_t r = ip.opr[j];
+        uint32_t ra = 1;
+        if (r == 2) {
+            const uint32_t v = __builtin_ctzll(p - 1);
+            ra = v >= 31 ? ip.oprpow[j] : min(1u << v, ip.oprpow[j]);
+        } else {
+            while (ra < ip.oprpow[j]) {
+                const uint32_t dd = ra * r, t = 65536u % dd;  // 2^32 mod dd from (2^16 mod dd)^2
+                if (mod_small(p - 1, dd, t * t % dd) != 0) break;
+                ra = dd;
+            }
+        }
+        gpow[j] = ra;
+        g *= ra;
+    }
+    const uint32_t tz = __builtin_ctz(g);
+    const uint64_t e = exact_div_odd((p - 1) >> tz, g >> tz);
+    const T Be = fpow(f, bm, e), Te = fpow(f, tm, e);
+    work += 2 * (64 - __builtin_clzll(e));
+    const uint64_t k1 = f.key(f.one());
+    uint32_t kst = 0, dt = 1;  // n == kst (mod dt)
+    for (uint32_t j = 0; j < ip.nopr; ++j) {
+        const uint32_t ra = gpow[j];
+        if (ra == 1) continue;
+        const uint32_t h = g / ra;
+        const T x = fpow(f, Be, h), y = fpow(f, Te, h);
+        work += 2 * (32 - __builtin_clz(h));
+        // the order dr of x is the first k >= 1 with x^k == 1; for a prime p, x^ra == b^(p-1) == 1,
+        // so dr divides ra.  A number where that fails is composite (a base-2 Fermat pseudoprime
+        // that reached setup) and has no relation: it is dropped, and the coprimality the CRT
+        // below relies on holds for every number that gets that far.
+        const uint64_t ky = f.key(y);
+        T xk = f.one();
+        uint32_t kf = ~0u, dr = 0;
+        for (uint32_t k = 0; k <= ra; ++k) {
+            const uint64_t kx = f.key(xk);
+            if (k > 0 && kx == k1) {
+                dr = k;
+                break;
+            }
+            if (kf == ~0u && kx == ky) kf = k;
+            xk = f.mul(xk, x);
+        }
+        work += dr;
+        if (dr == 0 || ra % dr != 0) return false;  // composite
+        if (kf == ~0u) return false;
+        // n == kf (mod dr) with n == kst (mod dt); dr and dt are coprime (different primes)
+        const uint32_t tcr = (uint32_t)((uint64_t)((kf + dr - kst % dr) % dr) * inv_small(dt % dr, dr) % dr);
+        kst += dt * tcr;
+        dt *= dr;
+    }
+    // with the class n == cres (mod cm); cm divides Qo, so L = lcm(dt, cm) divides Qo too
+    const uint32_t cm = (uint32_t)ip.cm, cr = (uint32_t)(ip.cres % ip.cm);
+    const uint32_t g0 = gcd32(dt, cm);
+    if (kst % g0 != cr % g0) return false;
+    const uint32_t m2 = cm / g0;
+    const uint32_t diff = ((cr + cm - kst % cm) % cm) / g0 % m2;
+    const uint32_t u = m2 == 1 ? 0 : (uint32_t)((uint64_t)diff * inv_small((dt / g0) % m2, m2) % m2);
+    const uint32_t Lw = dt * m2;
+    const uint32_t nstar = (uint32_t)(((uint64_t)kst + (uint64_t)dt * u) % Lw);
+    const uint32_t off = (nstar + Lw - ip.n0modQo % Lw) % Lw;
+    L = Lw;
+    nst = ip.N0 + off;
+    K = nst < ip.N1 ? (uint64_t)((uint32_t)(ip.N1 - 1 - nst) / Lw) + 1 : 0;
+    return K > 0;
+}
+
+template <class F, int MODE>
 __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, PassBuffers pb) {
+    constexpr bool QMODE = MODE == kModeClass;
     const uint64_t lo = d->lo;
     const ItemParams& ip = d->ip;  // read in place: uniform fields load as scalars, tables by index
     typedef typename F::T T;
@@ -340,6 +444,7 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
         bool keep = false;
         uint64_t p = 0, m = 0, ng = 0, jc = 0;
         uint32_t idx = 0, info = 0, s = 1, mask = 0, dmin = 0;
+        uint64_t oL = 0, onst = 0;  // order mode: exponent modulus and first exponent
         ClassSplit cs;
         F f;
         if (i < n) {
@@ -383,6 +488,17 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                         const uint2 sz = ip.csz[cs.qp * 32 + s];
                         m = sz.x;
                         ng = sz.y;
+                        keep = true;
+                    }
+                } else if (MODE == kModeOrder) {
+                    uint64_t K = 0;
+                    if (order_class(f, p, bm, tm, ip, oL, onst, K, w_setup)) {
+                        jc = K < p ? K : p;  // exponents repeat with period dividing p - 1
+                        m = isqrt_ceil64(jc);
+                        m = (m + ip.lanes - 1) / ip.lanes * ip.lanes;  // whole rounds of the workgroup
+                        if (m > ip.mcap) m = ip.mcap;
+                        if (m == 0) m = 1;
+                        ng = (jc + m - 1) / m + 1;
                         keep = true;
                     }
                 } else {
@@ -459,6 +575,14 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                 E[E_DJ] = djw[0];
                 E[E_DJ + 1] = djw[1];
                 E[E_DJ + 2] = djw[2];
+            } else if (MODE == kModeOrder) {
+                B = fpow(f, bm, oL);
+                E[E_G] = F::to_bits(fpow(f, bm, onst));
+                w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
+                E[E_NB] = onst;
+                E[E_META] = 1u << 11;
+                E[E_KLIM] = jc;
+                E[E_DJ] = oL;
             } else {
                 B = fpow(f, bm, ip.cm);
                 T G0 = fpow(f, bm, ip.n_start);
@@ -685,8 +809,13 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         be.B64 = F::from_bits(E[E_B64]);
         const F& f = be.f;
 
+        // the entry uses as many buckets as its m needs (at most the launch's table)
+        uint32_t nbe = tg.nbbits;
+        while (nbe > 4 && ((uint64_t)tg.load << (nbe - 1)) >= be.m) --nbe;
+        const uint32_t nbk_e = 1u << nbe;
+        Tb.nbmask = nbk_e - 1u;
         uint4* fp4 = reinterpret_cast<uint4*>(Tb.fp);
-        for (uint32_t i = tid; i < nbk; i += L) {
+        for (uint32_t i = tid; i < nbk_e; i += L) {
             fp4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
             Tb.cnt[i] = 0u;
         }
@@ -712,7 +841,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             const bool qmode = (be.meta >> 10) & 1u;
             const uint64_t stride = qmode ? 1 : ((be.meta >> 8) & 1u) + 1;
             const uint64_t parity = qmode ? 0 : (be.meta >> 9) & 1u;
-            const uint64_t cmul = qmode ? be.qp : item_cm;
+            const uint64_t cmul = qmode ? be.qp : ((be.meta >> 11) & 1u) ? E[E_DJ] : item_cm;
             T mstep;
             const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(E[E_BM64]), E_BML, tid, mstep);
             uint64_t best = ~0ull;
@@ -801,18 +930,21 @@ void launch_filter(hipStream_t s, const PassDesc* d, const PassBuffers& pb, uint
     hipLaunchKernelGGL(k_filter, dim3(grid), dim3(256), 0, s, d, pb);
 }
 
-void launch_setup(hipStream_t s, FieldKind fk, bool qmode, const PassDesc* d, const PassBuffers& pb, uint32_t grid) {
-    if (fk == FIELD_F64) {
-        if (qmode)
-            hipLaunchKernelGGL((k_setup<F64Field, true>), dim3(grid), dim3(256), 0, s, d, pb);
-        else
-            hipLaunchKernelGGL((k_setup<F64Field, false>), dim3(grid), dim3(256), 0, s, d, pb);
-    } else {
-        if (qmode)
-            hipLaunchKernelGGL((k_setup<MontField, true>), dim3(grid), dim3(256), 0, s, d, pb);
-        else
-            hipLaunchKernelGGL((k_setup<MontField, false>), dim3(grid), dim3(256), 0, s, d, pb);
-    }
+template <class F>
+const void* setup_kernel_f(uint32_t mode) {
+    return mode == kModeOrder   ? reinterpret_cast<const void*>(&k_setup<F, kModeOrder>)
+           : mode == kModeClass ? reinterpret_cast<const void*>(&k_setup<F, kModeClass>)
+                                : reinterpret_cast<const void*>(&k_setup<F, kModePlain>);
+}
+
+const void* setup_kernel(FieldKind fk, uint32_t mode) {
+    return fk == FIELD_F64 ? setup_kernel_f<F64Field>(mode) : setup_kernel_f<MontField>(mode);
+}
+
+void launch_setup(hipStream_t s, FieldKind fk, uint32_t mode, const PassDesc* d, const PassBuffers& pb, uint32_t grid) {
+    PassBuffers b = pb;
+    void* args[] = {&d, &b};
+    hipLaunchKernel(setup_kernel(fk, mode), dim3(grid), dim3(256), args, 0, s);
 }
 
 // Table geometry for baby-step counts up to m_max: at most `load` keys per 8-slot bucket on
@@ -824,6 +956,7 @@ TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load) {
     tg.nbbits = 4;
     while (((uint64_t)load << tg.nbbits) < m_max) ++tg.nbbits;
     tg.waves = (uint32_t)waves;
+    tg.load = load;
     return tg;
 }
 
@@ -875,13 +1008,6 @@ const void* bsgs_kernel(FieldKind fk, uint32_t waves, uint32_t& threads) {
                         : reinterpret_cast<const void*>(&k_bsgs<MontField, 4>);
 }
 
-const void* setup_kernel(FieldKind fk, bool qmode) {
-    if (fk == FIELD_F64)
-        return qmode ? reinterpret_cast<const void*>(&k_setup<F64Field, true>)
-                     : reinterpret_cast<const void*>(&k_setup<F64Field, false>);
-    return qmode ? reinterpret_cast<const void*>(&k_setup<MontField, true>)
-                 : reinterpret_cast<const void*>(&k_setup<MontField, false>);
-}
 
 hipError_t enqueue_pass_chain(hipStream_t s, const PassChain& c) {
     hipError_t e = hipMemcpyAsync(c.d_desc, c.h_desc, sizeof(PassDesc), hipMemcpyHostToDevice, s);
@@ -889,7 +1015,7 @@ hipError_t enqueue_pass_chain(hipStream_t s, const PassChain& c) {
     launch_reset(s, c.pb.cnt, 0);
     launch_sieve(s, c.d_desc, 0, c.sp, c.sp_r32, c.nsp, c.nsmall, c.pb, c.grid_sieve);
     launch_filter(s, c.d_desc, c.pb, c.grid_stage);
-    launch_setup(s, c.fk, c.qmode, c.d_desc, c.pb, c.grid_stage);
+    launch_setup(s, c.fk, c.mode, c.d_desc, c.pb, c.grid_stage);
     launch_bsgs(s, c.fk, c.tg, c.d_desc, c.pb, 0, c.grid_bsgs);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -947,7 +1073,7 @@ hipError_t build_pass_graph(const PassChain& c, hipGraph_t* out) {
     if (e == hipSuccess)
         e = add_kernel(g, prev, reinterpret_cast<const void*>(&k_sieve), c.grid_sieve, kSieveThreads, 0, a_sieve);
     if (e == hipSuccess) e = add_kernel(g, prev, reinterpret_cast<const void*>(&k_filter), c.grid_stage, 256, 0, a_stage);
-    if (e == hipSuccess) e = add_kernel(g, prev, setup_kernel(c.fk, c.qmode), c.grid_stage, 256, 0, a_stage);
+    if (e == hipSuccess) e = add_kernel(g, prev, setup_kernel(c.fk, c.mode), c.grid_stage, 256, 0, a_stage);
     if (e == hipSuccess)
         e = add_kernel(g, prev, bsgs, c.grid_bsgs, bsgs_threads, (uint32_t)bsgs_lds_bytes(c.tg), a_stage);
     if (e == hipSuccess) e = add_copy(g, prev, c.h_cnt, c.pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost);

@@ -431,8 +431,8 @@ int main() {
             double best = 0, dflt = 0;
             if (mps_gpu_autotune(gpu, &di, it.P0, 1ull << 22, &t, &best, &dflt) == MPS_OK) {
                 char msg[200];
-                std::snprintf(msg, sizeof(msg), "Autotuned b=%lld c=%lld: waves %u, mcap %u, cmax %u, load %u (%.3f ms, default %.3f ms)",
-                              (long long)it.b, (long long)it.c, t.waves, t.mcap, t.cmax, t.load, best, dflt);
+                std::snprintf(msg, sizeof(msg), "Autotuned b=%lld c=%lld: class policy %u, waves %u, mcap %u, cmax %u, load %u (%.3f ms, default %.3f ms)",
+                              (long long)it.b, (long long)it.c, t.policy, t.waves, t.mcap, t.cmax, t.load, best, dflt);
                 log_line("INFO", msg);
             }
         }

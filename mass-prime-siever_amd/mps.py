@@ -66,7 +66,7 @@ class GpuStats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """mps_tuning (include/mps_gpu.h): the BSGS shape of one kind of item."""
     _fields_ = [("waves", ctypes.c_uint32), ("mcap", ctypes.c_uint32), ("cmax", ctypes.c_uint32),
-                ("load", ctypes.c_uint32)]
+                ("load", ctypes.c_uint32), ("policy", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -351,7 +351,7 @@ class GpuSiever:
         return t.as_dict(), key.value
 
     def set_tuning(self, item, **kw):
-        t = Tuning(kw.get("waves", 0), kw.get("mcap", 0), kw.get("cmax", 0), kw.get("load", 0))
+        t = Tuning(*(kw.get(n, 0) for n, _ in Tuning._fields_))
         if lib().mps_gpu_set_tuning(self.h, ctypes.byref(device_item(item)), item.P0, ctypes.byref(t)) != MPS_OK:
             raise ValueError(f"tuning {kw}")
 

@@ -141,3 +141,18 @@ def test_declined_ranges(gpu, mps):
         assert st[c] == mps.MPS_OK and eng.relations(c) == []
     finally:
         eng.close()
+
+
+def test_batch_order_mode_matches_oracle(gpu, mps, want):
+    """The batch engine under class policy 2 (order mode, its own setup kernel variant and graphs)."""
+    gpu.set_class_policy(2)
+    eng = mps.Batch(gpu, slots=4, pass_odds=PASS_ODDS)
+    try:
+        ids = [eng.submit(mps.work_item(*it)) for it in ITEMS]
+        eng.wait_all()
+        assert all(d["status"] == mps.MPS_OK for d in eng.poll())
+        for i, it in zip(ids, ITEMS):
+            assert eng.relations(i) == want[it], it
+    finally:
+        eng.close()
+        gpu.set_class_policy(1)

@@ -90,3 +90,41 @@ def test_device_relations_exact(gpu, mps, oracle, it):
     assert decided <= want_pm
     assert {p for p, _ in want_pm} <= {p for p, _ in decided} | host
     assert len(host) <= max(2, len(want) // 10)
+
+
+@pytest.mark.parametrize("it", ITEMS)
+def test_order_mode_relations_match_oracle(gpu, mps, oracle, it):
+    """Class policy 2 (one exponent class per prime from the power residues of b and -c, solved
+    by Pohlig-Hellman on the device) gives the oracle's relations."""
+    b, c, P0, P1, N0, N1, mod, res = it
+    print(f"order mode {it}", flush=True)
+    gpu.set_class_policy(2)
+    try:
+        rels, st = gpu.run_item(mps.work_item(b, c, P0, P1, N0, N1, mod, res), chunk_ints=1 << 20)
+    finally:
+        gpu.set_class_policy(1)
+    want, _ = oracle.process_range(oracle.make_item(b, c, P0, P1, N0, N1, mod, res), P0, P1, 1 << 20, 8)
+    assert rels == want
+    assert st.declined == 0
+
+
+# base-2 Fermat pseudoprimes near 1e12 (composites that pass the device's first primality test)
+PSEUDOPRIMES = [1000254486421, 1000001633401, 1000133402761]
+
+
+@pytest.mark.parametrize("policy", [1, 2])
+@pytest.mark.parametrize("psp", PSEUDOPRIMES)
+def test_pseudoprime_candidates(gpu, mps, oracle, policy, psp):
+    """A base-2 pseudoprime reaches the setup stage; it must be dropped or left to the host, never
+    stall the kernel, and the relations around it must be the oracle's."""
+    assert pow(2, psp - 1, psp) == 1
+    P0, P1 = psp - 2**19, psp + 2**19
+    item = (10, 111, P0, P1, 0, 10_000_000, 6, 4)
+    print(f"pseudoprime {psp} policy {policy}", flush=True)
+    gpu.set_class_policy(policy)
+    try:
+        rels, _ = gpu.run_item(mps.work_item(*item), chunk_ints=1 << 21)
+    finally:
+        gpu.set_class_policy(1)
+    want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 8)
+    assert rels == want

@@ -100,3 +100,73 @@ def test_entries_match_oracle_decisions(gpu, mps, oracle, it, field, policy):
     assert len(got) == len(ent), "an entry was written twice"
     assert set(got) == set(want)
     assert got == want
+
+
+ORDER_Q = 367567200  # 2^5 3^3 5^2 7 11 13 17
+
+
+def order_class(p, b, c, cm, cres, N0, N1):
+    """The exponent class the device must find in order mode, restated from the mathematics: with
+    g = gcd(lcm(cm, ORDER_Q), p - 1), every solution n of b^n == -c (mod p) has (b^e)^n == (-c)^e,
+    e = (p-1)/g; per prime power of g this fixes n modulo the order of the projected b^e (or rules
+    p out), and with n == cres (mod cm) the admissible n are one class mod L.  Returns
+    (L, first n >= N0, count below N1) or None."""
+    import math
+    import sympy
+    Qo = cm // math.gcd(cm, ORDER_Q) * ORDER_Q
+    g = math.gcd(Qo, p - 1)
+    e = (p - 1) // g
+    Be, Te = pow(b % p, e, p), pow((-c) % p, e, p)
+    residues = []  # (k_r, d_r)
+    for r in sympy.primefactors(g):
+        ra = r ** sympy.multiplicity(r, g)
+        x, y = pow(Be, g // ra, p), pow(Te, g // ra, p)
+        d = next(k for k in range(1, ra + 1) if pow(x, k, p) == 1)
+        ks = [k for k in range(d) if pow(x, k, p) == y]
+        if not ks:
+            return None
+        residues.append((ks[0], d))
+    residues.append((cres % cm, cm))
+    # brute-force CRT over the small moduli (no shared code with the device's extended Euclid)
+    L = 1
+    for _, d in residues:
+        L = L * d // math.gcd(L, d)
+    sols = [n for n in range(L) if all(n % d == k % d for k, d in residues)]
+    if not sols:
+        return None
+    first = N0 + (sols[0] - N0) % L
+    K = (N1 - 1 - first) // L + 1 if first < N1 else 0
+    return (L, first, K) if K > 0 else None
+
+
+ORDER_ITEMS = [
+    (10, 111, 10**12, 10**12 + 2**19, 0, 10_000_000, 6, 4),
+    (2, -25, 10**9, 10**9 + 2**19, 3_600_000, 5_000_000, 2, 1),
+    (3, -2, 4 * 10**18, 4 * 10**18 + 2**18, 0, 10_000_000, 1, 0),
+    (7, 3, 10**6 + 1, 10**6 + 2**19, 12345, 412345, 1, 0),
+]
+
+
+@pytest.mark.parametrize("it", ORDER_ITEMS)
+def test_order_mode_entries_match_power_residues(gpu, mps, oracle, it):
+    """Under class policy 2 the prime entries are exactly the primes whose power-residue class is
+    non-empty, and each entry's baby/giant split covers the class."""
+    import sympy
+    b, c, lo, hi, N0, N1, mod, res = it
+    print(f"order entries {it}", flush=True)
+    gpu.set_class_policy(2)
+    try:
+        ent = gpu.debug_stage(2, lo, hi, mps.work_item(b, c, lo, hi, N0, N1, mod, res))
+    finally:
+        gpu.set_class_policy(1)
+    got = {int(p): (int(m), int(ng)) for p, m, ng, _ in ent.tolist() if sympy.isprime(int(p))}
+    want = {}
+    for x in oracle.sieve_primes(lo, hi).tolist():
+        if x < 1000 or b % x == 0 or c % x == 0:
+            continue
+        oc = order_class(x, b, c, max(1, mod), res, N0, N1)
+        if oc:
+            want[x] = min(oc[2], x)
+    assert set(got) == set(want)
+    for p, (m, ng) in got.items():
+        assert m * (ng - 1) >= want[p], p

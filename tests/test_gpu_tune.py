@@ -1,6 +1,7 @@
 """The autotuner and the BSGS shapes it picks from (include/mps_gpu.h mps_gpu_autotune,
-mps_gpu_set_tuning): every shape gives the oracle's relations, the tuner installs the fastest for
-the item's key and only for that key, and invalid shapes are rejected."""
+mps_gpu_set_tuning): every shape, under class policy 1 (class mode) and 2 (order mode), gives the
+oracle's relations, the tuner installs the fastest for the item's key and only for that key, and
+invalid shapes are rejected."""
 import pytest
 
 pytestmark = pytest.mark.gpu
@@ -23,9 +24,10 @@ def clean_tuning(gpu):
 
 
 @pytest.mark.parametrize("it", [CLASS_ITEM, PLAIN_ITEM, SHORT_ITEM])
-@pytest.mark.parametrize("shape", [dict(waves=1, load=4, cmax=512, mcap=1024),
-                                   dict(waves=4, load=2, cmax=2048, mcap=2048),
-                                   dict(waves=2, load=4, cmax=1024, mcap=4096)])
+@pytest.mark.parametrize("shape", [dict(waves=1, load=4, cmax=512, mcap=1024, policy=1),
+                                   dict(waves=4, load=2, cmax=2048, mcap=2048, policy=2),
+                                   dict(waves=2, load=4, cmax=1024, mcap=4096, policy=1),
+                                   dict(waves=1, load=2, cmax=512, mcap=1024, policy=2)])
 def test_every_shape_gives_oracle_relations(gpu, mps, oracle, it, shape):
     print(f"shape {shape} item {it}", flush=True)
     w = mps.work_item(*it)
@@ -54,6 +56,6 @@ def test_autotune_installs_fastest_for_key(gpu, mps, oracle):
 
 def test_invalid_shapes_rejected(gpu, mps):
     w = mps.work_item(*CLASS_ITEM)
-    for bad in (dict(waves=3), dict(load=3), dict(cmax=4096), dict(mcap=100)):
+    for bad in (dict(waves=3), dict(load=3), dict(cmax=4096), dict(mcap=100), dict(policy=3)):
         with pytest.raises(ValueError):
             gpu.set_tuning(w, **bad)
