@@ -142,8 +142,8 @@ typedef struct mps_tuning {
 /* Autotuner.  Items are keyed by whether classes are on, kind of class modulus, p magnitude
  * (log2 P0 in steps of 6) and exponent span per class modulus (log2 in steps of 2).  For the key
  * of `item` whose P range starts at P0, times every candidate shape (with classes on: class
- * policy 1 or 2; waves 1/2/4 x load 2/4 x the baby-step cap of the resulting mode: 1024/2048/4096
- * plain and order mode, 512/1024/2048 class mode; shapes whose table exceeds 64 KB of LDS are
+ * policy 1 or 2; waves 1/2/4 x load 2/4 x the baby-step cap of the resulting mode: 512 to 4096
+ * plain and order mode, 256 to 2048 class mode; shapes whose table exceeds 64 KB of LDS are
  * skipped) on one pass over the first 2 * sample_odds integers from P0 (device
  * time of the pass chain, best of two runs), requires every candidate's records to agree with the
  * default's (the same smallest solution wherever both decided a number; a number may be left to

@@ -835,8 +835,8 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
                        class_mode(item, g->classes_on, policy == 2 ? 1 : (int)policy, Q);
     for (uint32_t waves : {1u, 2u, 4u}) {
         for (uint32_t load : {2u, 4u}) {
-            for (uint32_t cap : {0u, 1u, 2u}) {
-                mps_tuning t{waves, qmode ? g->mcap : 1024u << cap, qmode ? 512u << cap : kClassMaxM, load, policy};
+            for (uint32_t cap : {0u, 1u, 2u, 3u}) {
+                mps_tuning t{waves, qmode ? g->mcap : 512u << cap, qmode ? 256u << cap : kClassMaxM, load, policy};
                 Plan pl;
                 std::string why;
                 force(true, t);

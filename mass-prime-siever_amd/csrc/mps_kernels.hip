@@ -352,10 +352,14 @@ __device__ __forceinline__ uint64_t exact_div_odd(uint64_t x, uint64_t d) {
 // power of x.  With n == cres (mod cm) the admissible n form one class n == n* (mod L); returns its
 // first member n_start >= N0 and count K below N1 (false: the prime has no solution).  All moduli
 // divide Qo < 2^32; only the exponent e needs 64-bit arithmetic.
+// With need_prime, p is first put to a Fermat test to base b, b^(p-1) = (b^e)^g == 1, which costs
+// the bits of g instead of those of p (b mod p != 0: the filter hands p | b to the host); a number
+// that fails it sets composite.
 template <class F>
 __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::T bm, typename F::T tm,
-                                            const ItemParams& ip, uint64_t& L, uint64_t& nst, uint64_t& K,
-                                            unsigned long long& work) {
+                                            const ItemParams& ip, bool need_prime, uint64_t& L, uint64_t& nst,
+                                            uint64_t& K, unsigned long long& work, unsigned long long& fermat_work,
+                                            bool& composite) {
     typedef typename F::T T;
     // g from the valuations of p - 1 at the primes of Qo (prime powers in Qo are below 2^16)
     uint32_t g = 1;
@@ -381,6 +385,13 @@ __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::
     const T Be = fpow(f, bm, e), Te = fpow(f, tm, e);
     work += 2 * (64 - __builtin_clzll(e));
     const uint64_t k1 = f.key(f.one());
+    if (need_prime) {
+        fermat_work += 2 * (32 - __builtin_clz(g));
+        if (f.key(fpow(f, Be, g)) != k1) {
+            composite = true;
+            return false;
+        }
+    }
     uint32_t kst = 0, dt = 1;  // n == kst (mod dt)
     for (uint32_t j = 0; j < ip.nopr; ++j) {
         const uint32_t ra = gpow[j];
@@ -453,8 +464,10 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
             info = (uint32_t)(cd >> 32);
             p = pb.vals ? pb.vals[idx] : lo + 2 * (uint64_t)idx;
             f.init(p);
-            if (pb.need_prime) w_fermat += bitlen64(p - 1);
-            if (pb.need_prime && !fermat2(f, p)) {
+            // order mode runs its Fermat test inside order_class, on the power b^e it needs anyway
+            const bool fermat_here = pb.need_prime && MODE != kModeOrder;
+            if (fermat_here) w_fermat += bitlen64(p - 1);
+            if (fermat_here && !fermat2(f, p)) {
                 ++fermat_fail;
             } else {
                 const uint64_t bmod = mod_signed(ip.b, p), t = mod_signed(-ip.c, p);
@@ -492,7 +505,11 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                     }
                 } else if (MODE == kModeOrder) {
                     uint64_t K = 0;
-                    if (order_class(f, p, bm, tm, ip, oL, onst, K, w_setup)) {
+                    bool composite = false;
+                    const bool found = order_class(f, p, bm, tm, ip, pb.need_prime, oL, onst, K, w_setup, w_fermat,
+                                                   composite);
+                    fermat_fail += composite ? 1u : 0u;
+                    if (found) {
                         jc = K < p ? K : p;  // exponents repeat with period dividing p - 1
                         m = isqrt_ceil64(jc);
                         m = (m + ip.lanes - 1) / ip.lanes * ip.lanes;  // whole rounds of the workgroup

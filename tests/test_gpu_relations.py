@@ -128,3 +128,22 @@ def test_pseudoprime_candidates(gpu, mps, oracle, policy, psp):
         gpu.set_class_policy(1)
     want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 8)
     assert rels == want
+
+
+@pytest.mark.parametrize("policy", [1, 2])
+@pytest.mark.parametrize("psp", [1000254486421, 1000001376901])
+def test_pseudoprime_candidates_base2_item(gpu, mps, oracle, policy, psp):
+    """With b = 2 the order-mode Fermat test (to base b) is the base-2 test, so these pseudoprimes
+    (p = 5 mod 8: (2/p) = -1, kept by the filter when cm = 1) pass it and reach the discrete logs,
+    which must drop them; the relations around them must be the oracle's."""
+    assert pow(2, psp - 1, psp) == 1 and psp % 8 == 5
+    P0, P1 = psp - 2**19, psp + 2**19
+    item = (2, 3, P0, P1, 0, 10_000_000, 1, 0)
+    print(f"pseudoprime {psp} b=2 policy {policy}", flush=True)
+    gpu.set_class_policy(policy)
+    try:
+        rels, _ = gpu.run_item(mps.work_item(*item), chunk_ints=1 << 21)
+    finally:
+        gpu.set_class_policy(1)
+    want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 8)
+    assert rels == want
