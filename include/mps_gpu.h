@@ -63,8 +63,8 @@ int mps_gpu_set_class_policy(mps_gpu* g, int policy);
  * rule out: each prime p of the range for which some in-class n in [N0, N1) has p | b^n + c,
  * plus the rare numbers the device leaves to the host (p < 1000, p | b, an order of the baby
  * step below the baby-step count, an entry that did not fit).  The device verifies every table
- * match exactly and tests primality with the 12-base Miller-Rabin test, so the list holds no
- * false positives or pseudoprimes otherwise.  Order is unspecified; the caller sorts, dedups,
+ * match exactly and tests primality with the 12-base Miller-Rabin test (a kernel of its own over
+ * the pass's records), so the list holds no false positives or pseudoprimes otherwise.  Order is unspecified; the caller sorts, dedups,
  * range-checks and runs the exact check, as the reference's caller does (Worker.hpp:223-227).
  * Precondition of the GPU path: hi < 2^62 and lo > 3 (Worker.hpp:209). */
 int mps_gpu_sieve_filter(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, uint64_t* out,

@@ -365,6 +365,10 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (!read_counts(g)) return false;
     g->st.ms_bsgs += elapsed(g->ev[3], g->ev[4]);
     g->st.n_bsgs_launches++;
+    if (g->h_cnt->n_out != 0) {  // the record copy that follows on the stream waits for it
+        launch_prime_records(s, g->pb);
+        HIPCHK(hipGetLastError(), "record test launch");
+    }
     return true;
 }
 
@@ -528,7 +532,11 @@ int sieve_records(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, st
                 hip_fail(g, hipErrorUnknown, "record read");
                 return MPS_ERROR;
             }
-            rec.insert(rec.end(), g->h_out, g->h_out + 2 * n_out);
+            for (uint32_t i = 0; i < n_out; ++i) {
+                if (g->h_out[2 * i] == 0) continue;  // composite (k_prime_records)
+                rec.push_back(g->h_out[2 * i]);
+                rec.push_back(g->h_out[2 * i + 1]);
+            }
         }
     }
     g->st.field = (int)fk;

@@ -118,6 +118,10 @@ TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load);
 constexpr uint32_t kBsgsWaves = 4;                // default wavefronts of one BSGS workgroup (one entry)
 size_t bsgs_lds_bytes(const TableGeom& tg);
 
+// Miller-Rabin over the pass's relation records (after launch_bsgs, same stream): composite
+// numbers are replaced by 0.
+constexpr uint32_t kPrimeRecordBlocks = 8;
+void launch_prime_records(hipStream_t s, const PassBuffers& pb);
 void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const PassDesc* d, const PassBuffers& pb,
                  uint32_t n_ent, uint32_t grid);
 // Workgroups of this BSGS variant resident on the device at once (cus compute units): the grid

@@ -785,8 +785,6 @@ __device__ __forceinline__ typename F::T thread_pow(const F& f, const uint64_t* 
     return r;
 }
 
-__device__ __forceinline__ bool mr_prime(uint64_t n);
-
 // One workgroup of WB wavefronts per entry: the table lives in LDS once per entry and every lane
 // of the group takes baby steps i == tid and giant steps g == tid (mod 64 * WB).
 template <class F, int WB>
@@ -891,9 +889,8 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             __syncthreads();
             if (tid == 0) {
                 for (int w = 1; w < WB; ++w) best = red[w] < best ? red[w] : best;
-                // deterministic Miller-Rabin: only primes reach the host
-                if (best != ~0ull && (pb.out_index || mr_prime(p)))
-                    put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, best);
+                // primality of the number is decided by k_prime_records, which runs next
+                if (best != ~0ull) put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, best);
             }
         }
         __syncthreads();
@@ -929,6 +926,24 @@ __device__ __forceinline__ bool mr_prime(uint64_t n) {
         if (composite) return false;
     }
     return true;
+}
+
+// The relation records of a pass through the deterministic 12-base Miller-Rabin test: a record
+// whose number is composite (a Fermat pseudoprime with a BSGS solution) gets number 0, which every
+// reader of the records drops.  Records the host decides, and records that carry an index instead
+// of the number, stay as they are.  A kernel of its own so that k_bsgs does not carry the test's
+// 64-bit arithmetic in its register budget (the test runs for a handful of numbers per pass).
+__global__ __launch_bounds__(256) void k_prime_records(PassBuffers pb) {
+    if (pb.out_index) return;
+    const uint32_t n = min(pb.cnt->n_out, pb.out_cap);
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        if (pb.out[2 * (uint64_t)i + 1] == kHostDecides) continue;
+        if (!mr_prime(pb.out[2 * (uint64_t)i])) pb.out[2 * (uint64_t)i] = 0;
+    }
+}
+
+void launch_prime_records(hipStream_t s, const PassBuffers& pb) {
+    hipLaunchKernelGGL(k_prime_records, dim3(kPrimeRecordBlocks), dim3(256), 0, s, pb);
 }
 
 // ------------------------------------------------------------------------------------- launchers
@@ -1034,6 +1049,7 @@ hipError_t enqueue_pass_chain(hipStream_t s, const PassChain& c) {
     launch_filter(s, c.d_desc, c.pb, c.grid_stage);
     launch_setup(s, c.fk, c.mode, c.d_desc, c.pb, c.grid_stage);
     launch_bsgs(s, c.fk, c.tg, c.d_desc, c.pb, 0, c.grid_bsgs);
+    launch_prime_records(s, c.pb);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(c.h_cnt, c.pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, s);
@@ -1083,6 +1099,7 @@ hipError_t build_pass_graph(const PassChain& c, hipGraph_t* out) {
     void* a_reset[] = {&cnt, &zero};
     void* a_sieve[] = {&d, &sp, &sp_r32, &nsp, &nsmall, &pb};
     void* a_stage[] = {&d, &pb};
+    void* a_records[] = {&pb};
     uint32_t bsgs_threads = 0;
     const void* bsgs = bsgs_kernel(c.fk, c.tg.waves, bsgs_threads);
     if (e == hipSuccess) e = add_copy(g, prev, c.d_desc, c.h_desc, sizeof(PassDesc), hipMemcpyHostToDevice);
@@ -1093,6 +1110,8 @@ hipError_t build_pass_graph(const PassChain& c, hipGraph_t* out) {
     if (e == hipSuccess) e = add_kernel(g, prev, setup_kernel(c.fk, c.mode), c.grid_stage, 256, 0, a_stage);
     if (e == hipSuccess)
         e = add_kernel(g, prev, bsgs, c.grid_bsgs, bsgs_threads, (uint32_t)bsgs_lds_bytes(c.tg), a_stage);
+    if (e == hipSuccess)
+        e = add_kernel(g, prev, reinterpret_cast<const void*>(&k_prime_records), kPrimeRecordBlocks, 256, 0, a_records);
     if (e == hipSuccess) e = add_copy(g, prev, c.h_cnt, c.pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost);
     if (e == hipSuccess)
         e = add_copy(g, prev, c.h_out, c.pb.out, c.out_words * sizeof(uint64_t), hipMemcpyDeviceToHost);
