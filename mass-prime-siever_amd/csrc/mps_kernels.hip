@@ -725,24 +725,30 @@ struct BsgsEntry {
     }
 };
 
-// Every exact match of giant value v (canonical key xv, giant index g) in the table: the
-// solution k = g*m - i, when in range, gives n = nj + (k*stride + parity)*cm; keeps the smallest.
-template <class F>
-__device__ __forceinline__ void giant_hits(const BsgsEntry<F>& be, const BsgsTable& T, uint32_t bucket, uint32_t fp,
-                                           uint64_t xv, uint32_t g, uint64_t nj, uint64_t Kj, uint64_t stride,
-                                           uint64_t parity, uint64_t cm, uint64_t& best) {
+// Fingerprint matches of the giant steps, verified after the giant loop (kept out of the loop so
+// that the exact check's registers do not count against the loop's occupancy).  A match holds the
+// giant value's key, the giant index g, the baby index i and the class j; more than kMaxHits
+// matches in one entry (a short cycle of the giant values, say) leave the number to the host.
+constexpr uint32_t kMaxHits = 32;
+
+struct HitList {
+    uint4 h[kMaxHits];
+    uint32_t n;
+    unsigned long long best;
+};
+
+// Every slot of the chain from bucket whose fingerprint is fp goes to the hit list.
+__device__ __forceinline__ void collect_hits(const BsgsTable& T, uint32_t bucket, uint32_t fp, uint64_t xv, uint32_t g,
+                                             uint32_t j, HitList& hl) {
     for (uint32_t b = bucket;; b = (b + 1) & T.nbmask) {
         const uint4 w = read_bucket(T, b);
         uint32_t mm = match_mask(w, fp);
         while (mm) {
             const uint32_t k = __builtin_ctz(mm);
             mm &= mm - 1;
-            const uint32_t i = T.idx[b * 8 + k];
-            if (be.f.key(be.baby(i)) != xv) continue;
-            const int64_t kk = (int64_t)g * be.m - (int64_t)i;
-            if (kk < 0 || (uint64_t)kk >= Kj) continue;
-            const uint64_t n = nj + ((uint64_t)kk * stride + parity) * cm;
-            if (n < best) best = n;
+            const uint32_t slot = atomicAdd(&hl.n, 1u);
+            if (slot < kMaxHits)
+                hl.h[slot] = make_uint4((uint32_t)xv, (uint32_t)(xv >> 32), g, (uint32_t)T.idx[b * 8 + k] | (j << 16));
         }
         if (w.w >= 0xFFFF0000u) break;  // slot 7 empty: the chain ends here
     }
@@ -794,7 +800,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
     typedef typename F::T T;
     constexpr uint32_t L = 64 * WB;
     extern __shared__ uint32_t lds_tab[];
-    __shared__ uint64_t red[WB];
+    __shared__ HitList hits;
     const uint32_t tid = threadIdx.x;
     const uint32_t nbk = 1u << tg.nbbits;
     BsgsTable Tb;
@@ -834,6 +840,10 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             fp4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
             Tb.cnt[i] = 0u;
         }
+        if (tid == 0) {
+            hits.n = 0;
+            hits.best = ~0ull;
+        }
         __syncthreads();
 
         // baby steps t * B^i, i == tid (mod L)
@@ -853,23 +863,9 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             if (tid == 0) put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, kHostDecides);
         } else {
             // giant steps G_j * (B^m)^g, g == tid (mod L), g < ng
-            const bool qmode = (be.meta >> 10) & 1u;
-            const uint64_t stride = qmode ? 1 : ((be.meta >> 8) & 1u) + 1;
-            const uint64_t parity = qmode ? 0 : (be.meta >> 9) & 1u;
-            const uint64_t cmul = qmode ? be.qp : ((be.meta >> 11) & 1u) ? E[E_DJ] : item_cm;
             T mstep;
             const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(E[E_BM64]), E_BML, tid, mstep);
-            uint64_t best = ~0ull;
             for (uint32_t j = 0; j < be.s; ++j) {
-                uint64_t nj, Kj;
-                if (qmode) {
-                    const uint32_t d = (uint32_t)(E[E_DJ + (j >> 3)] >> (8 * (j & 7))) & 0xFFu;
-                    nj = be.nb + d;
-                    Kj = (item_N1 - nj + be.qp - 1) / be.qp;
-                } else {
-                    nj = be.nb;
-                    Kj = be.klim;
-                }
                 T v = f.mul(F::from_bits(E[E_G + j]), Mt);
                 for (uint32_t g = tid; g < be.ng; g += L) {
                     const uint64_t x = f.key(v);
@@ -877,20 +873,41 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                     hash_key(x, Tb.nbmask, fp, b);
                     const uint4 w = read_bucket(Tb, b);
                     v = f.mul(v, mstep);  // the next giant value while the LDS read is in flight
-                    if (bucket_needs_look(w, fp)) giant_hits(be, Tb, b, fp, x, g, nj, Kj, stride, parity, cmul, best);
+                    if (bucket_needs_look(w, fp)) collect_hits(Tb, b, fp, x, g, j, hits);
                 }
             }
-            // smallest solution over the workgroup
-            for (int d = 32; d >= 1; d >>= 1) {
-                const uint64_t o = __shfl_xor((unsigned long long)best, d, 64);
-                best = o < best ? o : best;
-            }
-            if (lane_id() == 0) red[tid >> 6] = best;
             __syncthreads();
-            if (tid == 0) {
-                for (int w = 1; w < WB; ++w) best = red[w] < best ? red[w] : best;
+            // exact check of every match: the solution k = g*m - i, when in range, gives
+            // n = nj + (k*stride + parity)*cm; the smallest over the workgroup by an LDS minimum
+            const uint32_t nh = hits.n;
+            if (nh > kMaxHits) {
+                if (tid == 0) put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, kHostDecides);
+            } else {
+                const bool qmode = (be.meta >> 10) & 1u;
+                const uint64_t stride = qmode ? 1 : ((be.meta >> 8) & 1u) + 1;
+                const uint64_t parity = qmode ? 0 : (be.meta >> 9) & 1u;
+                const uint64_t cmul = qmode ? be.qp : ((be.meta >> 11) & 1u) ? E[E_DJ] : item_cm;
+                for (uint32_t h = tid; h < nh; h += L) {
+                    const uint4 c = hits.h[h];
+                    const uint32_t i = c.w & 0xFFFFu, j = c.w >> 16;
+                    if (f.key(be.baby(i)) != ((uint64_t)c.x | (uint64_t)c.y << 32)) continue;
+                    uint64_t nj, Kj;
+                    if (qmode) {
+                        const uint32_t dj = (uint32_t)(E[E_DJ + (j >> 3)] >> (8 * (j & 7))) & 0xFFu;
+                        nj = be.nb + dj;
+                        Kj = (item_N1 - nj + be.qp - 1) / be.qp;
+                    } else {
+                        nj = be.nb;
+                        Kj = be.klim;
+                    }
+                    const int64_t kk = (int64_t)c.z * be.m - (int64_t)i;
+                    if (kk < 0 || (uint64_t)kk >= Kj) continue;
+                    atomicMin(&hits.best, (unsigned long long)(nj + ((uint64_t)kk * stride + parity) * cmul));
+                }
+                __syncthreads();
                 // primality of the number is decided by k_prime_records, which runs next
-                if (best != ~0ull) put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, best);
+                if (tid == 0 && hits.best != ~0ull)
+                    put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, hits.best);
             }
         }
         __syncthreads();
