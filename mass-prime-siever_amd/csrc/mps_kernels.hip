@@ -916,46 +916,44 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
 
 // Deterministic Miller-Rabin with the 12 bases 2..37 (exact for n < 2^64, is_prime64
 // Math.hpp:209-251), in Montgomery arithmetic; n odd, n < 2^62.
-__device__ __forceinline__ bool mr_prime(uint64_t n) {
-    const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
-    for (int k = 0; k < 12; ++k) {
-        if (n == bases[k]) return true;
-        if (n % bases[k] == 0) return false;
-    }
+// One round of the deterministic 12-base Miller-Rabin test (bases 2..37, exact for n < 2^64,
+// is_prime64 in Math.hpp): false when base a proves the odd number n > 37 composite (a divides n,
+// or a is a strong witness).
+__device__ __forceinline__ bool mr_round(uint64_t n, uint32_t a) {
+    if (n % a == 0) return false;
     MontField f;
     f.init(n);
-    const uint64_t rr = f.r2();
     const uint64_t one = f.one(), mone = n - one;  // Montgomery forms of 1 and -1
     uint64_t d = n - 1;
-    int s = 0;
-    while ((d & 1) == 0) {
-        d >>= 1;
-        ++s;
+    const int s = __builtin_ctzll(d);
+    d >>= s;
+    uint64_t x = fpow(f, f.from(a, f.r2()), d);
+    if (x == one || x == mone) return true;
+    for (int r = 1; r < s; ++r) {
+        x = f.mul(x, x);
+        if (x == mone) return true;
     }
-    for (int k = 0; k < 12; ++k) {
-        uint64_t x = fpow(f, f.from(bases[k], rr), d);
-        if (x == one || x == mone) continue;
-        bool composite = true;
-        for (int r = 1; r < s && composite; ++r) {
-            x = f.mul(x, x);
-            if (x == mone) composite = false;
-        }
-        if (composite) return false;
-    }
-    return true;
+    return false;
 }
 
-// The relation records of a pass through the deterministic 12-base Miller-Rabin test: a record
-// whose number is composite (a Fermat pseudoprime with a BSGS solution) gets number 0, which every
-// reader of the records drops.  Records the host decides, and records that carry an index instead
-// of the number, stay as they are.  A kernel of its own so that k_bsgs does not carry the test's
-// 64-bit arithmetic in its register budget (the test runs for a handful of numbers per pass).
+// The relation records of a pass through the 12-base Miller-Rabin test: a record whose number is
+// composite (a Fermat pseudoprime with a BSGS solution) gets number 0, which every reader of the
+// records drops.  Records the host decides (which include every number below 1000), and records
+// that carry an index instead of the number, stay as they are.  A kernel of its own so that k_bsgs
+// does not carry the test's 64-bit arithmetic in its register budget; 16 lanes per record run its
+// 12 rounds side by side (a round is a chain of dependent products: one lane per round keeps the
+// kernel short).
 __global__ __launch_bounds__(256) void k_prime_records(PassBuffers pb) {
     if (pb.out_index) return;
+    const uint32_t bases[12] = {2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37};
     const uint32_t n = min(pb.cnt->n_out, pb.out_cap);
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        if (pb.out[2 * (uint64_t)i + 1] == kHostDecides) continue;
-        if (!mr_prime(pb.out[2 * (uint64_t)i])) pb.out[2 * (uint64_t)i] = 0;
+    const uint32_t k = threadIdx.x & 15u, grp = (threadIdx.x >> 4) & 3u;
+    for (uint32_t r0 = blockIdx.x * 16; r0 < n; r0 += gridDim.x * 16) {  // uniform over the block
+        const uint32_t i = r0 + (threadIdx.x >> 4);
+        const bool live = i < n && pb.out[2 * (uint64_t)i + 1] != kHostDecides;
+        const bool composite = live && k < 12 && !mr_round(pb.out[2 * (uint64_t)i], bases[k]);
+        const uint64_t votes = __ballot(composite);
+        if (live && k == 0 && ((votes >> (16 * grp)) & 0xFFFFull) != 0) pb.out[2 * (uint64_t)i] = 0;
     }
 }
 

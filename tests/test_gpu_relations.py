@@ -147,3 +147,24 @@ def test_pseudoprime_candidates_base2_item(gpu, mps, oracle, policy, psp):
         gpu.set_class_policy(1)
     want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 8)
     assert rels == want
+
+
+@pytest.mark.parametrize("policy", [1, 2])
+def test_pseudoprime_with_a_solution_is_dropped(gpu, mps, oracle, policy):
+    """A base-2 pseudoprime q that divides b^n + c for an in-class n (c = -(2^1000003 mod q)) gets
+    through the Fermat tests and has a BSGS solution: only the Miller-Rabin test over the records
+    removes it.  The relations must be the oracle's, which has no composite."""
+    q = 1000254486421
+    assert pow(2, q - 1, q) == 1 and q % 8 == 5
+    c = -pow(2, 1000003, q)
+    P0, P1 = q - 2**19, q + 2**19
+    item = (2, c, P0, P1, 1_000_000, 1_100_000, 1, 0)
+    print(f"pseudoprime with solution, policy {policy}", flush=True)
+    gpu.set_class_policy(policy)
+    try:
+        rels, _ = gpu.run_item(mps.work_item(*item), chunk_ints=1 << 21)
+    finally:
+        gpu.set_class_policy(1)
+    want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 8)
+    assert q not in [r[0] for r in rels]
+    assert rels == want
