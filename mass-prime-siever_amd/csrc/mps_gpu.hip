@@ -350,7 +350,9 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (stop_stage == 1 || n_cand == 0) return true;
 
     HIPCHK(hipEventRecord(g->ev[2], s), "event");
-    launch_setup(s, fk, ip.qmode, d, g->pb, blocks_for(g, n_cand, 256));
+    // class and order mode sort tiles of kSetupTile candidates: one workgroup per tile
+    const uint32_t setup_grid = ip.qmode == kModePlain ? blocks_for(g, n_cand, 256) : (n_cand + kSetupTile - 1) / kSetupTile;
+    launch_setup(s, fk, ip.qmode, d, g->pb, setup_grid);
     HIPCHK(hipGetLastError(), "setup launch");
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
     if (!read_counts(g)) return false;

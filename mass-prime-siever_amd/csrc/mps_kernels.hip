@@ -355,15 +355,10 @@ __device__ __forceinline__ uint64_t exact_div_odd(uint64_t x, uint64_t d) {
 // With need_prime, p is first put to a Fermat test to base b, b^(p-1) = (b^e)^g == 1, which costs
 // the bits of g instead of those of p (b mod p != 0: the filter hands p | b to the host); a number
 // that fails it sets composite.
-template <class F>
-__device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::T bm, typename F::T tm,
-                                            const ItemParams& ip, bool need_prime, uint64_t& L, uint64_t& nst,
-                                            uint64_t& K, unsigned long long& work, unsigned long long& fermat_work,
-                                            bool& composite) {
-    typedef typename F::T T;
-    // g from the valuations of p - 1 at the primes of Qo (prime powers in Qo are below 2^16)
+// g = gcd(Qo, p - 1) from the valuations of p - 1 at the primes of Qo (prime powers in Qo are
+// below 2^16); gpow[j] = the power of the j-th prime in g.
+__device__ __forceinline__ uint32_t order_g(uint64_t p, const ItemParams& ip, uint32_t (&gpow)[10]) {
     uint32_t g = 1;
-    uint32_t gpow[10];
     for (uint32_t j = 0; j < ip.nopr; ++j) {
         const uint32_t r = ip.opr[j];
         uint32_t ra = 1;
@@ -380,6 +375,17 @@ __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::
         gpow[j] = ra;
         g *= ra;
     }
+    return g;
+}
+
+template <class F>
+__device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::T bm, typename F::T tm,
+                                            const ItemParams& ip, bool need_prime, uint64_t& L, uint64_t& nst,
+                                            uint64_t& K, unsigned long long& work, unsigned long long& fermat_work,
+                                            bool& composite) {
+    typedef typename F::T T;
+    uint32_t gpow[10];
+    const uint32_t g = order_g(p, ip, gpow);
     const uint32_t tz = __builtin_ctz(g);
     const uint64_t e = exact_div_odd((p - 1) >> tz, g >> tz);
     const T Be = fpow(f, bm, e), Te = fpow(f, tm, e);
@@ -446,183 +452,227 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
     const ItemParams& ip = d->ip;  // read in place: uniform fields load as scalars, tables by index
     typedef typename F::T T;
     __shared__ uint32_t sh[5];
+    // class and order mode: each tile of kSetupTile candidates is processed in the order of a cost
+    // key (the size of g), so the lanes of a wavefront run exponent loops of like length
+    constexpr bool SORT = MODE != kModePlain;
+    constexpr uint32_t kTile = SORT ? kSetupTile : 256;
+    __shared__ uint16_t s_order[SORT ? kSetupTile : 1];
+    __shared__ uint32_t s_bin[32];
     const uint32_t n = min(pb.cnt->n_cand, pb.cand_cap);
     const uint32_t lane = lane_id();
     unsigned long long w_fermat = 0, w_setup = 0, w_bsgs = 0;
     uint32_t fermat_fail = 0;
-    for (uint32_t i0 = blockIdx.x * 256; i0 < n; i0 += gridDim.x * 256) {
-        const uint32_t i = i0 + threadIdx.x;
-        bool keep = false;
-        uint64_t p = 0, m = 0, ng = 0, jc = 0;
-        uint32_t idx = 0, info = 0, s = 1, mask = 0, dmin = 0;
-        uint64_t oL = 0, onst = 0;  // order mode: exponent modulus and first exponent
-        ClassSplit cs;
-        F f;
-        if (i < n) {
-            const uint64_t cd = pb.cand[i];
-            idx = (uint32_t)cd;
-            info = (uint32_t)(cd >> 32);
-            p = pb.vals ? pb.vals[idx] : lo + 2 * (uint64_t)idx;
-            f.init(p);
-            // order mode runs its Fermat test inside order_class, on the power b^e it needs anyway
-            const bool fermat_here = pb.need_prime && MODE != kModeOrder;
-            if (fermat_here) w_fermat += bitlen64(p - 1);
-            if (fermat_here && !fermat2(f, p)) {
-                ++fermat_fail;
-            } else {
+    for (uint32_t t0 = blockIdx.x * kTile; t0 < n; t0 += gridDim.x * kTile) {
+        if (SORT) {
+            if (threadIdx.x < 32) s_bin[threadIdx.x] = 0;
+            __syncthreads();
+            uint32_t keys[kTile / 256];
+            for (uint32_t q = 0; q < kTile / 256; ++q) {
+                const uint32_t i = t0 + q * 256 + threadIdx.x;
+                uint32_t key = 31;  // past the end: last
+                if (i < n) {
+                    const uint32_t ix = (uint32_t)pb.cand[i];
+                    const uint64_t pk = pb.vals ? pb.vals[ix] : lo + 2 * (uint64_t)ix;
+                    if (MODE == kModeOrder) {
+                        uint32_t gp[10];
+                        key = min(32u - __builtin_clz(order_g(pk, ip, gp)), 30u);  // bits of g
+                    } else {
+                        ClassSplit cs;
+                        class_split(pk, ip, cs);
+                        key = min(cs.g >> 1, 30u);
+                    }
+                }
+                keys[q] = key;
+                atomicAdd(&s_bin[key], 1u);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t tot = 0;
+                for (int b = 0; b < 32; ++b) {
+                    const uint32_t c = s_bin[b];
+                    s_bin[b] = tot;
+                    tot += c;
+                }
+            }
+            __syncthreads();
+            for (uint32_t q = 0; q < kTile / 256; ++q) s_order[atomicAdd(&s_bin[keys[q]], 1u)] = (uint16_t)(q * 256 + threadIdx.x);
+            __syncthreads();
+        }
+        for (uint32_t q = 0; q < kTile / 256; ++q) {
+            const uint32_t i = t0 + (SORT ? (uint32_t)s_order[q * 256 + threadIdx.x] : q * 256 + threadIdx.x);
+            bool keep = false;
+            uint64_t p = 0, m = 0, ng = 0, jc = 0;
+            uint32_t idx = 0, info = 0, s = 1, mask = 0, dmin = 0;
+            uint64_t oL = 0, onst = 0;  // order mode: exponent modulus and first exponent
+            ClassSplit cs;
+            F f;
+            if (i < n) {
+                const uint64_t cd = pb.cand[i];
+                idx = (uint32_t)cd;
+                info = (uint32_t)(cd >> 32);
+                p = pb.vals ? pb.vals[idx] : lo + 2 * (uint64_t)idx;
+                f.init(p);
+                // order mode runs its Fermat test inside order_class, on the power b^e it needs anyway
+                const bool fermat_here = pb.need_prime && MODE != kModeOrder;
+                if (fermat_here) w_fermat += bitlen64(p - 1);
+                if (fermat_here && !fermat2(f, p)) {
+                    ++fermat_fail;
+                } else {
+                    const uint64_t bmod = mod_signed(ip.b, p), t = mod_signed(-ip.c, p);
+                    const T rr = f.r2();
+                    const T bm = f.from(bmod, rr), tm = f.from(t, rr);
+                    if (QMODE) {
+                        class_split(p, ip, cs);
+                        mask = info;
+                        if (cs.g > 2) {
+                            // class j needs t^e == (b^e)^(n_j mod g), e = (p-1)/g
+                            const uint64_t e = (p - 1) / cs.g;
+                            const T u = fpow(f, tm, e), w = fpow(f, bm, e);
+                            w_setup += 2 * bitlen64(e) + cs.g;
+                            const uint64_t ku = f.key(u);
+                            const uint32_t n0g = ip.n0modQ % cs.g;
+                            uint32_t keepm = 0;
+                            T wk = f.one();
+                            for (uint32_t k = 0; k < cs.g; ++k) {
+                                if (f.key(wk) == ku)
+                                    for (uint32_t j = 0; j < cs.ncls; ++j)
+                                        if ((n0g + cs.dj(j)) % cs.g == k) keepm |= 1u << j;
+                                wk = f.mul(wk, w);
+                            }
+                            mask &= keepm;
+                        }
+                        if (mask != 0) {
+                            s = __popc(mask);
+                            dmin = cs.qp;
+                            for (uint32_t j = 0; j < cs.ncls; ++j)
+                                if (((mask >> j) & 1u) && cs.dj(j) < dmin) dmin = cs.dj(j);
+                            const uint2 sz = ip.csz[cs.qp * 32 + s];
+                            m = sz.x;
+                            ng = sz.y;
+                            keep = true;
+                        }
+                    } else if (MODE == kModeOrder) {
+                        uint64_t K = 0;
+                        bool composite = false;
+                        const bool found = order_class(f, p, bm, tm, ip, pb.need_prime, oL, onst, K, w_setup, w_fermat,
+                                                       composite);
+                        fermat_fail += composite ? 1u : 0u;
+                        if (found) {
+                            jc = K < p ? K : p;  // exponents repeat with period dividing p - 1
+                            m = isqrt_ceil64(jc);
+                            m = (m + ip.lanes - 1) / ip.lanes * ip.lanes;  // whole rounds of the workgroup
+                            if (m > ip.mcap) m = ip.mcap;
+                            if (m == 0) m = 1;
+                            ng = (jc + m - 1) / m + 1;
+                            keep = true;
+                        }
+                    } else {
+                        const uint64_t g = gcd64(ip.cm, pm1_mod(p, ip.cm, ip.cm_r32));
+                        bool ok = true;
+                        if (g > 2) {
+                            // B = b^cm is a g-th power, so T = t*b^-n_start must be one: t^e == G0^e
+                            const uint64_t e = (p - 1) / g;
+                            const T G0 = fpow(f, bm, ip.n_start);
+                            ok = f.key(fpow(f, tm, e)) == f.key(fpow(f, G0, e));
+                            w_setup += 2 * bitlen64(e) + bitlen64(ip.n_start);
+                        }
+                        const uint32_t stride = (info & 1u) ? 2 : 1, parity = (info >> 1) & 1u;
+                        const uint64_t kc = ip.kcount < p ? ip.kcount : p;
+                        if (ok && kc > parity) {
+                            const uint32_t combo = (stride - 1) * 2 + parity;
+                            if (kc == ip.kcount) {
+                                jc = ip.pjc[combo];
+                                m = ip.pm[combo];
+                                ng = ip.png[combo];
+                            } else {  // kcount capped at p (p below the exponent count)
+                                jc = (kc - parity + stride - 1) / stride;
+                                m = isqrt_ceil64(jc);
+                                if (m > ip.mcap) m = ip.mcap;
+                                if (m == 0) m = 1;
+                                ng = (jc + m - 1) / m + 1;  // g = 0 .. ceil(jcount/m)
+                            }
+                            keep = true;
+                        }
+                    }
+                }
+            }
+            const uint32_t slot = block_append<4>(keep, &pb.cnt->n_ent, sh);
+            if (keep && slot >= pb.ent_cap) {
+                const uint32_t o = atomicAdd(&pb.cnt->n_out, 1u);  // no room: the host decides this one
+                put_record(pb, o, pb.out_index ? idx : p, kHostDecides);
+                keep = false;
+            }
+            if (keep) {
+                uint64_t* E = pb.ent + (uint64_t)slot * kEntWords;
                 const uint64_t bmod = mod_signed(ip.b, p), t = mod_signed(-ip.c, p);
                 const T rr = f.r2();
                 const T bm = f.from(bmod, rr), tm = f.from(t, rr);
+                T B;
+                E[E_P] = p;
+                E[E_AUX] = f.aux();
+                E[E_ONE] = F::to_bits(f.one());
+                E[E_T] = F::to_bits(tm);
+                E[E_MG] = m | (ng << 32);
+                E[E_SSRC] = (uint64_t)s | ((uint64_t)((QMODE ? mask : info) & 0xFFFFFFu) << 8) | ((uint64_t)idx << 32);
                 if (QMODE) {
-                    class_split(p, ip, cs);
-                    mask = info;
-                    if (cs.g > 2) {
-                        // class j needs t^e == (b^e)^(n_j mod g), e = (p-1)/g
-                        const uint64_t e = (p - 1) / cs.g;
-                        const T u = fpow(f, tm, e), w = fpow(f, bm, e);
-                        w_setup += 2 * bitlen64(e) + cs.g;
-                        const uint64_t ku = f.key(u);
-                        const uint32_t n0g = ip.n0modQ % cs.g;
-                        uint32_t keepm = 0;
-                        T wk = f.one();
-                        for (uint32_t k = 0; k < cs.g; ++k) {
-                            if (f.key(wk) == ku)
-                                for (uint32_t j = 0; j < cs.ncls; ++j)
-                                    if ((n0g + cs.dj(j)) % cs.g == k) keepm |= 1u << j;
-                            wk = f.mul(wk, w);
-                        }
-                        mask &= keepm;
+                    B = fpow(f, bm, cs.qp);
+                    E[E_NB] = ip.N0;
+                    E[E_META] = cs.qp | (1u << 10);
+                    E[E_KLIM] = 0;
+                    // G_j = b^(N0 + dj) = b^(N0 + dmin) * b^(dj - dmin)
+                    const T base = fpow(f, bm, ip.N0 + dmin);
+                    w_setup += 2 * bitlen64(ip.N0 + dmin) + 12;
+                    uint32_t o = 0;
+                    uint64_t djw[3] = {0, 0, 0};
+                    for (uint32_t j = 0; j < cs.ncls; ++j) {
+                        if (!((mask >> j) & 1u)) continue;
+                        const uint32_t d = cs.dj(j);
+                        E[E_G + o] = F::to_bits(f.mul(base, fpow(f, bm, d - dmin)));
+                        if (o < 8)
+                            djw[0] |= (uint64_t)d << (8 * o);
+                        else if (o < 16)
+                            djw[1] |= (uint64_t)d << (8 * (o - 8));
+                        else
+                            djw[2] |= (uint64_t)d << (8 * (o - 16));
+                        w_setup += 12;
+                        ++o;
                     }
-                    if (mask != 0) {
-                        s = __popc(mask);
-                        dmin = cs.qp;
-                        for (uint32_t j = 0; j < cs.ncls; ++j)
-                            if (((mask >> j) & 1u) && cs.dj(j) < dmin) dmin = cs.dj(j);
-                        const uint2 sz = ip.csz[cs.qp * 32 + s];
-                        m = sz.x;
-                        ng = sz.y;
-                        keep = true;
-                    }
+                    E[E_DJ] = djw[0];
+                    E[E_DJ + 1] = djw[1];
+                    E[E_DJ + 2] = djw[2];
                 } else if (MODE == kModeOrder) {
-                    uint64_t K = 0;
-                    bool composite = false;
-                    const bool found = order_class(f, p, bm, tm, ip, pb.need_prime, oL, onst, K, w_setup, w_fermat,
-                                                   composite);
-                    fermat_fail += composite ? 1u : 0u;
-                    if (found) {
-                        jc = K < p ? K : p;  // exponents repeat with period dividing p - 1
-                        m = isqrt_ceil64(jc);
-                        m = (m + ip.lanes - 1) / ip.lanes * ip.lanes;  // whole rounds of the workgroup
-                        if (m > ip.mcap) m = ip.mcap;
-                        if (m == 0) m = 1;
-                        ng = (jc + m - 1) / m + 1;
-                        keep = true;
-                    }
+                    B = fpow(f, bm, oL);
+                    E[E_G] = F::to_bits(fpow(f, bm, onst));
+                    w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
+                    E[E_NB] = onst;
+                    E[E_META] = 1u << 11;
+                    E[E_KLIM] = jc;
+                    E[E_DJ] = oL;
                 } else {
-                    const uint64_t g = gcd64(ip.cm, pm1_mod(p, ip.cm, ip.cm_r32));
-                    bool ok = true;
-                    if (g > 2) {
-                        // B = b^cm is a g-th power, so T = t*b^-n_start must be one: t^e == G0^e
-                        const uint64_t e = (p - 1) / g;
-                        const T G0 = fpow(f, bm, ip.n_start);
-                        ok = f.key(fpow(f, tm, e)) == f.key(fpow(f, G0, e));
-                        w_setup += 2 * bitlen64(e) + bitlen64(ip.n_start);
+                    B = fpow(f, bm, ip.cm);
+                    T G0 = fpow(f, bm, ip.n_start);
+                    w_setup += 2 * bitlen64(ip.n_start) + 2 * bitlen64(ip.cm);
+                    if (info & 1u) {  // stride 2: k = 2j + parity
+                        if (info & 2u) G0 = f.mul(G0, B);
+                        B = f.mul(B, B);
                     }
-                    const uint32_t stride = (info & 1u) ? 2 : 1, parity = (info >> 1) & 1u;
-                    const uint64_t kc = ip.kcount < p ? ip.kcount : p;
-                    if (ok && kc > parity) {
-                        const uint32_t combo = (stride - 1) * 2 + parity;
-                        if (kc == ip.kcount) {
-                            jc = ip.pjc[combo];
-                            m = ip.pm[combo];
-                            ng = ip.png[combo];
-                        } else {  // kcount capped at p (p below the exponent count)
-                            jc = (kc - parity + stride - 1) / stride;
-                            m = isqrt_ceil64(jc);
-                            if (m > ip.mcap) m = ip.mcap;
-                            if (m == 0) m = 1;
-                            ng = (jc + m - 1) / m + 1;  // g = 0 .. ceil(jcount/m)
-                        }
-                        keep = true;
-                    }
+                    E[E_G] = F::to_bits(G0);
+                    E[E_NB] = ip.n_start;
+                    E[E_META] = ((info & 1u) ? (1u << 8) : 0u) | ((info & 2u) ? (1u << 9) : 0u);
+                    E[E_KLIM] = jc;
                 }
+                const T Bm = fpow(f, B, m);
+                w_setup += 2 * bitlen64(m);
+                const T B64 = write_pow_tables(f, B, E + E_BLO, w_setup);
+                const T Bm64 = write_pow_tables(f, Bm, E + E_MLO, w_setup);
+                E[E_B64] = F::to_bits(B64);
+                E[E_BM64] = F::to_bits(Bm64);
+                E[E_BL] = F::to_bits(lane_stride_pow(f, B64, ip.lanes));
+                E[E_BML] = F::to_bits(lane_stride_pow(f, Bm64, ip.lanes));
+                w_bsgs += m + (uint64_t)s * ng;
             }
         }
-        const uint32_t slot = block_append<4>(keep, &pb.cnt->n_ent, sh);
-        if (keep && slot >= pb.ent_cap) {
-            const uint32_t o = atomicAdd(&pb.cnt->n_out, 1u);  // no room: the host decides this one
-            put_record(pb, o, pb.out_index ? idx : p, kHostDecides);
-            keep = false;
-        }
-        if (keep) {
-            uint64_t* E = pb.ent + (uint64_t)slot * kEntWords;
-            const uint64_t bmod = mod_signed(ip.b, p), t = mod_signed(-ip.c, p);
-            const T rr = f.r2();
-            const T bm = f.from(bmod, rr), tm = f.from(t, rr);
-            T B;
-            E[E_P] = p;
-            E[E_AUX] = f.aux();
-            E[E_ONE] = F::to_bits(f.one());
-            E[E_T] = F::to_bits(tm);
-            E[E_MG] = m | (ng << 32);
-            E[E_SSRC] = (uint64_t)s | ((uint64_t)((QMODE ? mask : info) & 0xFFFFFFu) << 8) | ((uint64_t)idx << 32);
-            if (QMODE) {
-                B = fpow(f, bm, cs.qp);
-                E[E_NB] = ip.N0;
-                E[E_META] = cs.qp | (1u << 10);
-                E[E_KLIM] = 0;
-                // G_j = b^(N0 + dj) = b^(N0 + dmin) * b^(dj - dmin)
-                const T base = fpow(f, bm, ip.N0 + dmin);
-                w_setup += 2 * bitlen64(ip.N0 + dmin) + 12;
-                uint32_t o = 0;
-                uint64_t djw[3] = {0, 0, 0};
-                for (uint32_t j = 0; j < cs.ncls; ++j) {
-                    if (!((mask >> j) & 1u)) continue;
-                    const uint32_t d = cs.dj(j);
-                    E[E_G + o] = F::to_bits(f.mul(base, fpow(f, bm, d - dmin)));
-                    if (o < 8)
-                        djw[0] |= (uint64_t)d << (8 * o);
-                    else if (o < 16)
-                        djw[1] |= (uint64_t)d << (8 * (o - 8));
-                    else
-                        djw[2] |= (uint64_t)d << (8 * (o - 16));
-                    w_setup += 12;
-                    ++o;
-                }
-                E[E_DJ] = djw[0];
-                E[E_DJ + 1] = djw[1];
-                E[E_DJ + 2] = djw[2];
-            } else if (MODE == kModeOrder) {
-                B = fpow(f, bm, oL);
-                E[E_G] = F::to_bits(fpow(f, bm, onst));
-                w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
-                E[E_NB] = onst;
-                E[E_META] = 1u << 11;
-                E[E_KLIM] = jc;
-                E[E_DJ] = oL;
-            } else {
-                B = fpow(f, bm, ip.cm);
-                T G0 = fpow(f, bm, ip.n_start);
-                w_setup += 2 * bitlen64(ip.n_start) + 2 * bitlen64(ip.cm);
-                if (info & 1u) {  // stride 2: k = 2j + parity
-                    if (info & 2u) G0 = f.mul(G0, B);
-                    B = f.mul(B, B);
-                }
-                E[E_G] = F::to_bits(G0);
-                E[E_NB] = ip.n_start;
-                E[E_META] = ((info & 1u) ? (1u << 8) : 0u) | ((info & 2u) ? (1u << 9) : 0u);
-                E[E_KLIM] = jc;
-            }
-            const T Bm = fpow(f, B, m);
-            w_setup += 2 * bitlen64(m);
-            const T B64 = write_pow_tables(f, B, E + E_BLO, w_setup);
-            const T Bm64 = write_pow_tables(f, Bm, E + E_MLO, w_setup);
-            E[E_B64] = F::to_bits(B64);
-            E[E_BM64] = F::to_bits(Bm64);
-            E[E_BL] = F::to_bits(lane_stride_pow(f, B64, ip.lanes));
-            E[E_BML] = F::to_bits(lane_stride_pow(f, Bm64, ip.lanes));
-            w_bsgs += m + (uint64_t)s * ng;
-        }
+        if (SORT) __syncthreads();  // s_order and s_bin are rewritten by the next tile
     }
     // one atomic per counter and workgroup
     __shared__ unsigned long long red[4][4];
