@@ -429,7 +429,7 @@ int main() {
             const mps_item di{it.b, it.c, it.N0, it.N1, it.cls_mod, it.cls_res};
             mps_tuning t{};
             double best = 0, dflt = 0;
-            if (mps_gpu_autotune(gpu, &di, it.P0, 1ull << 22, &t, &best, &dflt) == MPS_OK) {
+            if (mps_gpu_autotune(gpu, &di, it.P0, 1ull << 24, &t, &best, &dflt) == MPS_OK) {
                 char msg[200];
                 std::snprintf(msg, sizeof(msg), "Autotuned b=%lld c=%lld: class policy %u, waves %u, mcap %u, cmax %u, load %u (%.3f ms, default %.3f ms)",
                               (long long)it.b, (long long)it.c, t.policy, t.waves, t.mcap, t.cmax, t.load, best, dflt);

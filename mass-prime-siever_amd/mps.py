@@ -335,7 +335,7 @@ class GpuSiever:
                 raise RuntimeError(f"run_item status {rc}: {self.last_error()}")
             return [(rels[i].p, rels[i].M, rels[i].q) for i in range(n.value)], st
 
-    def autotune(self, item, sample_odds=1 << 22):
+    def autotune(self, item, sample_odds=1 << 24):
         """Times the BSGS shapes on a sample pass of a work item and installs the fastest for its
         key; returns (tuning dict, best ms, default ms)."""
         t, best, dflt = Tuning(), ctypes.c_double(), ctypes.c_double()

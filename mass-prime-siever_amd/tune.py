@@ -1,7 +1,7 @@
 """Autotuner driver: picks the BSGS shape (wavefronts per entry, baby-step caps, LDS table load)
 for every kind of item in a worktodo.txt file and writes a tuning table that a GpuSiever loads.
 
-    python tune.py worktodo.txt --out tuning.json [--sample-odds 4194304]
+    python tune.py worktodo.txt --out tuning.json [--sample-odds 16777216]
 
 Kinds are the library's tuning keys (class mode and class-modulus kind, p magnitude, exponent
 span); one item per key is timed (include/mps_gpu.h mps_gpu_autotune).  The table lists, per key,
@@ -41,7 +41,7 @@ def parse_worktodo(text):
     return out
 
 
-def autotune_items(g, items, sample_odds=1 << 22, log=None):
+def autotune_items(g, items, sample_odds=1 << 24, log=None):
     """Tunes one representative item per key; returns the table records."""
     mps = _mps()
     reps = {}
@@ -80,7 +80,7 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("worktodo")
     ap.add_argument("--out", required=True)
-    ap.add_argument("--sample-odds", type=int, default=1 << 22)
+    ap.add_argument("--sample-odds", type=int, default=1 << 24)
     args = ap.parse_args()
     mps = _mps()
     g = mps.GpuSiever()
