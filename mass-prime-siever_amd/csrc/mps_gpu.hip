@@ -102,6 +102,20 @@ bool order_mode(const mps_item* it, bool classes_on, int class_policy, uint64_t&
     return classes_on && class_policy == 2 && it->N0 >= 0 && it->N1 > it->N0 && Qo < (1ull << 32);
 }
 
+JacobiPre jacobi_pre(int64_t x) {
+    JacobiPre J{0, 0, 0, 0};
+    if (x == 0) return J;
+    uint64_t ax = x < 0 ? 0ull - (uint64_t)x : (uint64_t)x;
+    const uint32_t e = (uint32_t)__builtin_ctzll(ax);
+    ax >>= e;
+    if (ax >= 65536) return J;
+    J.a = (uint32_t)ax;
+    J.r32 = (uint32_t)((1ull << 32) % ax);
+    J.neg = x < 0 ? 1u : 0u;
+    J.e_odd = e & 1u;
+    return J;
+}
+
 bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t cmax,
                  uint32_t waves, ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why) {
     std::memset(&ip, 0, sizeof(ip));
@@ -117,6 +131,8 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
     ip.cres = it->cls_res;
     ip.n_start = N0 + ((it->cls_res % cm) + cm - (N0 % cm)) % cm;
     ip.kcount = ip.n_start >= N1 ? 0 : (N1 - ip.n_start + cm - 1) / cm;
+    ip.jb = jacobi_pre(it->b);
+    ip.jt = jacobi_pre(it->c == INT64_MIN ? 0 : -it->c);
     ip.cm_r32 = cm < 65536 ? (uint32_t)((1ull << 32) % cm) : 0;
     ip.mcap = mcap;
     ip.lanes = L;
@@ -335,27 +351,31 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     if (g->pb.vals == nullptr) launch_sieve(s, d, pc, g->d_sp, g->d_sp_r32, g->nsp, g->nsmall, g->pb, 0);
     HIPCHK(hipGetLastError(), "sieve launch");
     HIPCHK(hipEventRecord(g->ev[1], s), "event");
-    if (!read_counts(g)) return false;
-    const uint32_t n_surv = std::min(g->h_cnt->n_surv, g->pb.surv_cap);
-    g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
-    if (stop_stage == 0 || n_surv == 0) return true;
-
-    HIPCHK(hipEventRecord(g->ev[1], s), "event");
-    launch_filter(s, d, g->pb, blocks_for(g, n_surv, 256));
+    if (stop_stage == 0) {
+        if (!read_counts(g)) return false;
+        g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
+        return true;
+    }
+    // The filter and setup kernels read their element counts on the device, so their grids are
+    // sized for the largest count and nothing is read back until the BSGS grid needs the entry
+    // count.
+    launch_filter(s, d, g->pb, blocks_for(g, pc, 256));
     HIPCHK(hipGetLastError(), "filter launch");
     HIPCHK(hipEventRecord(g->ev[2], s), "event");
-    if (!read_counts(g)) return false;
-    g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
-    const uint32_t n_cand = std::min(g->h_cnt->n_cand, g->pb.cand_cap);
-    if (stop_stage == 1 || n_cand == 0) return true;
-
-    HIPCHK(hipEventRecord(g->ev[2], s), "event");
-    // class and order mode sort tiles of kSetupTile candidates: one workgroup per tile
-    const uint32_t setup_grid = ip.qmode == kModePlain ? blocks_for(g, n_cand, 256) : (n_cand + kSetupTile - 1) / kSetupTile;
+    if (stop_stage == 1) {
+        if (!read_counts(g)) return false;
+        g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
+        g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
+        return true;
+    }
+    // class and order mode sort tiles of kSetupTile candidates, a workgroup a tile at a time
+    const uint32_t setup_grid = blocks_for(g, pc, ip.qmode == kModePlain ? 256 : kSetupTile);
     launch_setup(s, fk, ip.qmode, d, g->pb, setup_grid);
     HIPCHK(hipGetLastError(), "setup launch");
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
     if (!read_counts(g)) return false;
+    g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
+    g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
     g->st.ms_setup += elapsed(g->ev[2], g->ev[3]);
     const uint32_t n_ent = std::min(g->h_cnt->n_ent, g->pb.ent_cap);
     if (stop_stage == 2 || n_ent == 0) return true;

@@ -57,8 +57,18 @@ struct Counters {
 };
 
 // Per-item constants for the filter and setup stages.
+// Jacobi symbol (x / p) of a fixed item value x = 2^e * a * sign, prepared on the host: with
+// 0 < a < 2^16 the device needs p mod a (from r32 = 2^32 mod a) and a 16-bit Jacobi loop; a == 0
+// (|x| of 2^16 or more, or x == 0) takes the general path.
+struct JacobiPre {
+    uint32_t a, r32;
+    uint32_t neg;    // x < 0: the factor (-1/p)
+    uint32_t e_odd;  // e odd: the factor (2/p)
+};
+
 struct ItemParams {
     int64_t b, c;
+    JacobiPre jb, jt;       // for x = b and x = -c
     uint64_t N0, N1;        // N0 clamped at 0
     uint64_t cm, cres;      // class modulus (>= 1) and residue
     uint64_t n_start;       // first in-class n >= N0 (plain mode)

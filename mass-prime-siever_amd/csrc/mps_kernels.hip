@@ -174,6 +174,18 @@ __device__ __forceinline__ uint64_t pm1_mod(uint64_t p, uint64_t d, uint32_t r32
     return (p - 1) % d;
 }
 
+// (x / p) for x = b or -c of the item (x given as well, for the general path).
+__device__ __forceinline__ int jacobi_item(const JacobiPre& J, int64_t x, uint64_t p) {
+    if (J.a == 0) return jacobi_signed(x, p);
+    int s = 1;
+    const uint32_t p8 = (uint32_t)p & 7u;
+    if (J.neg && (p8 & 3u) == 3u) s = -s;                    // (-1/p)
+    if (J.e_odd && (p8 == 3u || p8 == 5u)) s = -s;           // (2/p)
+    if (J.a == 1) return s;
+    if ((J.a & 3u) == 3u && (p8 & 3u) == 3u) s = -s;         // reciprocity
+    return s * jacobi32(mod_small(p, J.a, J.r32), J.a);
+}
+
 __device__ __forceinline__ uint64_t gcd64(uint64_t a, uint64_t b) {
     while (b) {
         const uint64_t t = a % b;
@@ -197,9 +209,10 @@ __device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, b
     if (mod_signed(ip.b, p) == 0) { flag2 = true; return; }
     if (mod_signed(ip.c, p) == 0) return;  // t = -c == 0: b^n == 0 impossible
     if (ip.kcount == 0) return;
-    const uint64_t g = gcd64(ip.cm, pm1_mod(p, ip.cm, ip.cm_r32));
-    const int jb = jacobi_signed(ip.b, p);
-    const int jt = jacobi_signed(-ip.c, p);
+    const uint64_t pm = pm1_mod(p, ip.cm, ip.cm_r32);
+    const uint64_t g = ip.cm < (1ull << 32) ? gcd32((uint32_t)ip.cm, (uint32_t)pm) : gcd64(ip.cm, pm);
+    const int jb = jacobi_item(ip.jb, ip.b, p);
+    const int jt = jacobi_item(ip.jt, -ip.c, p);
     const int lT = jt * ((ip.n_start & 1) ? jb : 1);
     if (g == 2) {
         keep = lT == 1;
@@ -241,8 +254,8 @@ __device__ __forceinline__ void filter_class(uint64_t p, const ItemParams& ip, b
     ClassSplit cs;
     class_split(p, ip, cs);
     const uint64_t span = ip.N1 - ip.N0;
-    const int jb = jacobi_signed(ip.b, p);
-    const int jt = jacobi_signed(-ip.c, p);
+    const int jb = jacobi_item(ip.jb, ip.b, p);
+    const int jt = jacobi_item(ip.jt, -ip.c, p);
     const uint32_t n0odd = (uint32_t)(ip.N0 & 1);
     uint32_t mask = 0;
     for (uint32_t j = 0; j < cs.ncls; ++j) {
