@@ -169,6 +169,16 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
             why = "class modulus with more than 10 prime factors";
             return false;
         }
+        for (uint32_t j = 0; j < ip.nopr; ++j) {
+            const uint64_t qr = ip.oprpow[j], M = Qo / qr;
+            uint64_t inv = 1;  // M^-1 mod qr by search (qr < 2^16)
+            while ((M % qr) * inv % qr != 1 % qr) ++inv;
+            ip.ocrt[j] = (uint32_t)(M * inv % Qo);
+            uint64_t cr = 1;
+            while (cm % (cr * ip.opr[j]) == 0) cr *= ip.opr[j];
+            ip.ocm[j] = (uint32_t)cr;
+            ip.ocres[j] = (uint32_t)(ip.cres % cr);
+        }
         // an entry's class has at most kcount members: m <= sqrt(kcount) in whole rounds, <= mcap
         uint64_t m = 1;
         while (m * m < ip.kcount) ++m;

@@ -86,9 +86,11 @@ struct ItemParams {
     // class mode sizes: csz[qp * 32 + s] = (m, giant steps per class)
     const uint2* csz;
     // order mode: Qo = lcm(cm, kOrderQ), its distinct prime factors and their powers in Qo,
-    // and N0 mod Qo
+    // and N0 mod Qo; per prime: the CRT basis element of Qo (1 modulo the prime's power, 0 modulo
+    // the others), the prime's part of cm and cres modulo that part
     uint32_t Qo, nopr, n0modQo;
     uint32_t opr[10], oprpow[10];
+    uint32_t ocrt[10], ocm[10], ocres[10];
 };
 
 // BSGS table geometry of a pass: 2^nbbits buckets of 8 slots; a slot holds a fingerprint of

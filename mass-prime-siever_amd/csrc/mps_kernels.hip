@@ -58,6 +58,34 @@ __device__ __forceinline__ uint32_t block_append(bool keep, uint32_t* counter, u
     return slot;
 }
 
+// Block-wide reservation of c slots per thread (c may be 0): every thread of the workgroup must
+// call it.  One atomic per call and workgroup; returns this thread's first slot.  sh needs NW + 1
+// words of LDS.
+template <int NW>
+__device__ __forceinline__ uint32_t block_reserve(uint32_t c, uint32_t* counter, uint32_t* sh) {
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t x = c;  // inclusive scan over the wavefront
+    for (int dd = 1; dd < 64; dd <<= 1) {
+        const uint32_t y = __shfl_up(x, dd, 64);
+        if (lane >= (uint32_t)dd) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (int i = 0; i < NW; ++i) {
+            const uint32_t t = sh[i];
+            sh[i] = tot;
+            tot += t;
+        }
+        sh[NW] = tot ? atomicAdd(counter, tot) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = sh[NW] + sh[w] + x - c;
+    __syncthreads();
+    return base;
+}
+
 __device__ __forceinline__ unsigned long long wave_sum64(unsigned long long v) {
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
     return v;
@@ -275,24 +303,42 @@ __global__ __launch_bounds__(256) void k_filter(const PassDesc* __restrict__ d, 
     const uint32_t n = min(pb.cnt->n_surv, pb.surv_cap);
     const uint32_t lane = lane_id();
     uint32_t nflag2 = 0;
-    for (uint32_t i0 = blockIdx.x * 256; i0 < n; i0 += gridDim.x * 256) {
-        const uint32_t i = i0 + threadIdx.x;
-        bool keep = false, flag2 = false;
-        uint32_t info = 0, idx = 0;
-        uint64_t p = 0;
-        if (i < n) {
-            idx = pb.vals ? i : pb.surv[i];
-            p = pb.vals ? pb.vals[i] : lo + 2 * (uint64_t)idx;
-            if (ip.qmode == kModeClass)
-                filter_class(p, ip, keep, flag2, info);
-            else
-                filter_plain(p, ip, keep, flag2, info);
+    // kFilterPer survivors per thread between two reservations of candidate slots: one atomic
+    // on the candidate counter per 256 * kFilterPer survivors (same-address atomics serialise)
+    constexpr uint32_t kFilterPer = 4;
+    for (uint32_t i0 = blockIdx.x * 256 * kFilterPer; i0 < n; i0 += gridDim.x * 256 * kFilterPer) {
+        bool kept[kFilterPer];
+        uint64_t val[kFilterPer];
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kFilterPer; ++k) {
+            const uint32_t i = i0 + k * 256 + threadIdx.x;
+            bool keep = false, flag2 = false;
+            uint32_t info = 0, idx = 0;
+            uint64_t p = 0;
+            if (i < n) {
+                idx = pb.vals ? i : pb.surv[i];
+                p = pb.vals ? pb.vals[i] : lo + 2 * (uint64_t)idx;
+                if (ip.qmode == kModeClass)
+                    filter_class(p, ip, keep, flag2, info);
+                else
+                    filter_plain(p, ip, keep, flag2, info);
+            }
+            kept[k] = keep;
+            val[k] = (uint64_t)idx | ((uint64_t)info << 32);
+            c += keep ? 1u : 0u;
+            const uint32_t oslot = wave_append(flag2, &pb.cnt->n_out);
+            if (flag2) put_record(pb, oslot, pb.out_index ? idx : p, kHostDecides);
+            nflag2 += flag2 ? 1u : 0u;
         }
-        const uint32_t slot = block_append<4>(keep, &pb.cnt->n_cand, sh);
-        if (keep && slot < pb.cand_cap) pb.cand[slot] = (uint64_t)idx | ((uint64_t)info << 32);
-        const uint32_t oslot = wave_append(flag2, &pb.cnt->n_out);
-        if (flag2) put_record(pb, oslot, pb.out_index ? idx : p, kHostDecides);
-        nflag2 += flag2 ? 1u : 0u;
+        uint32_t slot = block_reserve<4>(c, &pb.cnt->n_cand, sh);
+#pragma unroll
+        for (uint32_t k = 0; k < kFilterPer; ++k) {
+            if (kept[k]) {
+                if (slot < pb.cand_cap) pb.cand[slot] = val[k];
+                ++slot;
+            }
+        }
     }
     nflag2 = (uint32_t)wave_sum64(nflag2);
     if (lane == 0 && nflag2 != 0) atomicAdd(&pb.cnt->n_flag2, nflag2);
@@ -338,20 +384,6 @@ __device__ __forceinline__ typename F::T lane_stride_pow(const F& f, typename F:
     return r;
 }
 
-// Inverse of a modulo a small m (m <= 2^16) by stepping through the multiples of a; at most m
-// steps, 0 when gcd(a, m) != 1.
-__device__ __forceinline__ uint32_t inv_small(uint32_t a, uint32_t m) {
-    if (m <= 1) return 0;
-    a %= m;
-    uint32_t acc = a;
-    for (uint32_t u = 1; u < m; ++u) {
-        if (acc == 1u) return u;
-        acc += a;
-        if (acc >= m) acc -= m;
-    }
-    return 0;
-}
-
 // Quotient of an exact division x / d (d odd): x times the inverse of d modulo 2^64.
 __device__ __forceinline__ uint64_t exact_div_odd(uint64_t x, uint64_t d) {
     uint64_t inv = d;  // d * d == 1 (mod 8): correct to 3 bits, doubled by each Newton step
@@ -364,28 +396,34 @@ __device__ __forceinline__ uint64_t exact_div_odd(uint64_t x, uint64_t d) {
 // and y = (t^e)^(g/r^a) give n == k_r (mod d_r), d_r = ord(x), or no solution when y is not a
 // power of x.  With n == cres (mod cm) the admissible n form one class n == n* (mod L); returns its
 // first member n_start >= N0 and count K below N1 (false: the prime has no solution).  All moduli
-// divide Qo < 2^32; only the exponent e needs 64-bit arithmetic.
+// divide Qo < 2^32; only the exponent e needs 64-bit arithmetic.  g and its exponents (gexp, 4
+// bits per prime of Qo) come from order_g.
 // With need_prime, p is first put to a Fermat test to base b, b^(p-1) = (b^e)^g == 1, which costs
 // the bits of g instead of those of p (b mod p != 0: the filter hands p | b to the host); a number
 // that fails it sets composite.
 // g = gcd(Qo, p - 1) from the valuations of p - 1 at the primes of Qo (prime powers in Qo are
-// below 2^16); gpow[j] = the power of the j-th prime in g.
-__device__ __forceinline__ uint32_t order_g(uint64_t p, const ItemParams& ip, uint32_t (&gpow)[10]) {
+// below 2^16); returns g and, 4 bits per prime, the exponent of the j-th prime in g.
+__device__ __forceinline__ uint32_t order_g(uint64_t p, const ItemParams& ip, uint64_t& gexp) {
     uint32_t g = 1;
+    gexp = 0;
     for (uint32_t j = 0; j < ip.nopr; ++j) {
         const uint32_t r = ip.opr[j];
-        uint32_t ra = 1;
+        uint32_t ra = 1, e = 0;
         if (r == 2) {
             const uint32_t v = __builtin_ctzll(p - 1);
-            ra = v >= 31 ? ip.oprpow[j] : min(1u << v, ip.oprpow[j]);
+            while (ra < ip.oprpow[j] && e < v) {
+                ra *= 2;
+                ++e;
+            }
         } else {
             while (ra < ip.oprpow[j]) {
                 const uint32_t dd = ra * r, t = 65536u % dd;  // 2^32 mod dd from (2^16 mod dd)^2
                 if (mod_small(p - 1, dd, t * t % dd) != 0) break;
                 ra = dd;
+                ++e;
             }
         }
-        gpow[j] = ra;
+        gexp |= (uint64_t)e << (4 * j);
         g *= ra;
     }
     return g;
@@ -393,12 +431,10 @@ __device__ __forceinline__ uint32_t order_g(uint64_t p, const ItemParams& ip, ui
 
 template <class F>
 __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::T bm, typename F::T tm,
-                                            const ItemParams& ip, bool need_prime, uint64_t& L, uint64_t& nst,
-                                            uint64_t& K, unsigned long long& work, unsigned long long& fermat_work,
-                                            bool& composite) {
+                                            const ItemParams& ip, uint32_t g, uint64_t gexp, bool need_prime,
+                                            uint64_t& L, uint64_t& nst, uint64_t& K, unsigned long long& work,
+                                            unsigned long long& fermat_work, bool& composite) {
     typedef typename F::T T;
-    uint32_t gpow[10];
-    const uint32_t g = order_g(p, ip, gpow);
     const uint32_t tz = __builtin_ctz(g);
     const uint64_t e = exact_div_odd((p - 1) >> tz, g >> tz);
     const T Be = fpow(f, bm, e), Te = fpow(f, tm, e);
@@ -411,51 +447,66 @@ __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::
             return false;
         }
     }
-    uint32_t kst = 0, dt = 1;  // n == kst (mod dt)
+    // per prime r of Qo: n == k_r (mod d_r) from the discrete log, and n == cres (mod cm_r); both
+    // moduli are powers of r, so they agree or there is no solution, and the finer one holds.
+    // The CRT basis of Qo then gives n* modulo L = lcm(prod d_r, cm) without an inverse.
+    uint64_t acc = 0;
+    uint32_t Lw = 1;
     for (uint32_t j = 0; j < ip.nopr; ++j) {
-        const uint32_t ra = gpow[j];
-        if (ra == 1) continue;
-        const uint32_t h = g / ra;
-        const T x = fpow(f, Be, h), y = fpow(f, Te, h);
-        work += 2 * (32 - __builtin_clz(h));
-        // the order dr of x is the first k >= 1 with x^k == 1; for a prime p, x^ra == b^(p-1) == 1,
-        // so dr divides ra.  A number where that fails is composite (a base-2 Fermat pseudoprime
-        // that reached setup) and has no relation: it is dropped, and the coprimality the CRT
-        // below relies on holds for every number that gets that far.
-        const uint64_t ky = f.key(y);
-        T xk = f.one();
-        uint32_t kf = ~0u, dr = 0;
-        for (uint32_t k = 0; k <= ra; ++k) {
-            const uint64_t kx = f.key(xk);
-            if (k > 0 && kx == k1) {
-                dr = k;
-                break;
+        const uint32_t ej = (uint32_t)(gexp >> (4 * j)) & 15u;
+        uint32_t dr = 1, kf = 0;
+        if (ej != 0) {
+            uint32_t ra = 1;
+            for (uint32_t k = 0; k < ej; ++k) ra *= ip.opr[j];
+            const uint32_t h = g / ra;
+            const T x = fpow(f, Be, h), y = fpow(f, Te, h);
+            work += 2 * (32 - __builtin_clz(h));
+            // the order dr of x is the first k >= 1 with x^k == 1; for a prime p, x^ra == b^(p-1) == 1,
+            // so dr divides ra.  A number where that fails is composite (a Fermat pseudoprime that
+            // reached setup) and has no relation: it is dropped.
+            const uint64_t ky = f.key(y);
+            T xk = f.one();
+            uint32_t kd = ~0u;
+            dr = 0;
+            for (uint32_t k = 0; k <= ra; ++k) {
+                const uint64_t kx = f.key(xk);
+                if (k > 0 && kx == k1) {
+                    dr = k;
+                    break;
+                }
+                if (kd == ~0u && kx == ky) kd = k;
+                xk = f.mul(xk, x);
             }
-            if (kf == ~0u && kx == ky) kf = k;
-            xk = f.mul(xk, x);
+            work += dr;
+            if (dr == 0 || ra % dr != 0) return false;  // composite
+            if (kd == ~0u) return false;                // -c is not a power of b modulo p
+            kf = kd;
         }
-        work += dr;
-        if (dr == 0 || ra % dr != 0) return false;  // composite
-        if (kf == ~0u) return false;
-        // n == kf (mod dr) with n == kst (mod dt); dr and dt are coprime (different primes)
-        const uint32_t tcr = (uint32_t)((uint64_t)((kf + dr - kst % dr) % dr) * inv_small(dt % dr, dr) % dr);
-        kst += dt * tcr;
-        dt *= dr;
+        const uint32_t cmj = ip.ocm[j], crj = ip.ocres[j];
+        uint32_t cj, lj;
+        if (dr >= cmj) {
+            if (kf % cmj != crj) return false;
+            cj = kf;
+            lj = dr;
+        } else {
+            if (crj % dr != kf) return false;
+            cj = crj;
+            lj = cmj;
+        }
+        acc += (uint64_t)cj * ip.ocrt[j];
+        Lw *= lj;
     }
-    // with the class n == cres (mod cm); cm divides Qo, so L = lcm(dt, cm) divides Qo too
-    const uint32_t cm = (uint32_t)ip.cm, cr = (uint32_t)(ip.cres % ip.cm);
-    const uint32_t g0 = gcd32(dt, cm);
-    if (kst % g0 != cr % g0) return false;
-    const uint32_t m2 = cm / g0;
-    const uint32_t diff = ((cr + cm - kst % cm) % cm) / g0 % m2;
-    const uint32_t u = m2 == 1 ? 0 : (uint32_t)((uint64_t)diff * inv_small((dt / g0) % m2, m2) % m2);
-    const uint32_t Lw = dt * m2;
-    const uint32_t nstar = (uint32_t)(((uint64_t)kst + (uint64_t)dt * u) % Lw);
+    const uint32_t nstar = (uint32_t)(acc % Lw);
     const uint32_t off = (nstar + Lw - ip.n0modQo % Lw) % Lw;
     L = Lw;
     nst = ip.N0 + off;
-    K = nst < ip.N1 ? (uint64_t)((uint32_t)(ip.N1 - 1 - nst) / Lw) + 1 : 0;
-    return K > 0;
+    if (nst >= ip.N1) {
+        K = 0;
+        return false;
+    }
+    const uint64_t rest = ip.N1 - 1 - nst;
+    K = (rest < (1ull << 32) ? (uint64_t)((uint32_t)rest / Lw) : rest / Lw) + 1;
+    return true;
 }
 
 template <class F, int MODE>
@@ -471,6 +522,8 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
     constexpr uint32_t kTile = SORT ? kSetupTile : 256;
     __shared__ uint16_t s_order[SORT ? kSetupTile : 1];
     __shared__ uint32_t s_bin[32];
+    __shared__ uint64_t s_gexp[MODE == kModeOrder ? kSetupTile : 1];  // order mode: exponents of g
+    __shared__ uint32_t s_g[MODE == kModeOrder ? kSetupTile : 1];
     const uint32_t n = min(pb.cnt->n_cand, pb.cand_cap);
     const uint32_t lane = lane_id();
     unsigned long long w_fermat = 0, w_setup = 0, w_bsgs = 0;
@@ -487,8 +540,11 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                     const uint32_t ix = (uint32_t)pb.cand[i];
                     const uint64_t pk = pb.vals ? pb.vals[ix] : lo + 2 * (uint64_t)ix;
                     if (MODE == kModeOrder) {
-                        uint32_t gp[10];
-                        key = min(32u - __builtin_clz(order_g(pk, ip, gp)), 30u);  // bits of g
+                        uint64_t gx;
+                        const uint32_t gv = order_g(pk, ip, gx);
+                        s_gexp[q * 256 + threadIdx.x] = gx;
+                        s_g[q * 256 + threadIdx.x] = gv;
+                        key = min(32u - __builtin_clz(gv), 30u);  // bits of g
                     } else {
                         ClassSplit cs;
                         class_split(pk, ip, cs);
@@ -512,7 +568,8 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
             __syncthreads();
         }
         for (uint32_t q = 0; q < kTile / 256; ++q) {
-            const uint32_t i = t0 + (SORT ? (uint32_t)s_order[q * 256 + threadIdx.x] : q * 256 + threadIdx.x);
+            const uint32_t li = SORT ? (uint32_t)s_order[q * 256 + threadIdx.x] : q * 256 + threadIdx.x;
+            const uint32_t i = t0 + li;
             bool keep = false;
             uint64_t p = 0, m = 0, ng = 0, jc = 0;
             uint32_t idx = 0, info = 0, s = 1, mask = 0, dmin = 0;
@@ -567,8 +624,8 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                     } else if (MODE == kModeOrder) {
                         uint64_t K = 0;
                         bool composite = false;
-                        const bool found = order_class(f, p, bm, tm, ip, pb.need_prime, oL, onst, K, w_setup, w_fermat,
-                                                       composite);
+                        const bool found = order_class(f, p, bm, tm, ip, s_g[li], s_gexp[li], pb.need_prime, oL, onst,
+                                                       K, w_setup, w_fermat, composite);
                         fermat_fail += composite ? 1u : 0u;
                         if (found) {
                             jc = K < p ? K : p;  // exponents repeat with period dividing p - 1
