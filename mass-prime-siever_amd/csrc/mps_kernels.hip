@@ -786,7 +786,6 @@ constexpr uint32_t kEmpty16 = 0xFFFFu;
 
 struct BsgsTable {
     uint16_t* fp;    // [nbk * 8]
-    uint16_t* idx;   // [nbk * 8]
     uint32_t* cnt;   // [nbk]
     uint32_t nbmask;
 };
@@ -797,12 +796,11 @@ __device__ __forceinline__ void hash_key(uint64_t x, uint32_t nbmask, uint32_t& 
     bucket = ((lo >> 16) ^ lo) & nbmask;
 }
 
-__device__ __forceinline__ void tab_insert(const BsgsTable& T, uint32_t bucket, uint32_t fp, uint32_t i) {
+__device__ __forceinline__ void tab_insert(const BsgsTable& T, uint32_t bucket, uint32_t fp) {
     for (;;) {
         const uint32_t old = atomicAdd(&T.cnt[bucket], 1u);
         if (old < 8) {
             T.fp[bucket * 8 + old] = (uint16_t)fp;
-            T.idx[bucket * 8 + old] = (uint16_t)i;
             return;
         }
         bucket = (bucket + 1) & T.nbmask;
@@ -836,20 +834,16 @@ struct BsgsEntry {
     T t, B64;
     uint32_t m, ng, s, meta, qp;
     uint64_t nb, klim;
-    // exact baby value t * B^i
-    __device__ __forceinline__ T baby(uint32_t i) const {
-        const uint32_t r = i & 63;
-        T x = f.mul(F::from_bits(E[E_BLO + (r & 7)]), F::from_bits(E[E_BHI + (r >> 3)]));
-        if (i >> 6) x = f.mul(x, fpow(f, B64, i >> 6));
-        return f.mul(t, x);
-    }
 };
 
-// Fingerprint matches of the giant steps, verified after the giant loop (kept out of the loop so
-// that the exact check's registers do not count against the loop's occupancy).  A match holds the
-// giant value's key, the giant index g, the baby index i and the class j; more than kMaxHits
-// matches in one entry (a short cycle of the giant values, say) leave the number to the host.
+// Giant steps whose fingerprint is in the table, verified after the giant loop (kept out of the
+// loop so that the exact check's registers do not count against the loop's occupancy).  A hit
+// holds the giant value's key, the giant index g and the class j; the table keeps no baby
+// indices, so the check runs the baby steps again and compares every baby value with every hit's
+// key exactly.  More than kMaxHits hits in one entry (a short cycle of the giant values, say)
+// leave the number to the host.
 constexpr uint32_t kMaxHits = 32;
+constexpr uint32_t kCycleFlag = 1u << 30;  // hit count of an entry whose baby values repeat
 
 struct HitList {
     uint4 h[kMaxHits];
@@ -857,43 +851,18 @@ struct HitList {
     unsigned long long best;
 };
 
-// Every slot of the chain from bucket whose fingerprint is fp goes to the hit list.
-__device__ __forceinline__ void collect_hits(const BsgsTable& T, uint32_t bucket, uint32_t fp, uint64_t xv, uint32_t g,
-                                             uint32_t j, HitList& hl) {
+// A giant value whose fingerprint is somewhere in the chain from bucket goes to the hit list once.
+__device__ __forceinline__ void collect_hit(const BsgsTable& T, uint32_t bucket, uint32_t fp, uint64_t xv, uint32_t g,
+                                            uint32_t j, HitList& hl) {
     for (uint32_t b = bucket;; b = (b + 1) & T.nbmask) {
         const uint4 w = read_bucket(T, b);
-        uint32_t mm = match_mask(w, fp);
-        while (mm) {
-            const uint32_t k = __builtin_ctz(mm);
-            mm &= mm - 1;
+        if (match_mask(w, fp) != 0) {
             const uint32_t slot = atomicAdd(&hl.n, 1u);
-            if (slot < kMaxHits)
-                hl.h[slot] = make_uint4((uint32_t)xv, (uint32_t)(xv >> 32), g, (uint32_t)T.idx[b * 8 + k] | (j << 16));
+            if (slot < kMaxHits) hl.h[slot] = make_uint4((uint32_t)xv, (uint32_t)(xv >> 32), g, j);
+            return;
         }
-        if (w.w >= 0xFFFF0000u) break;  // slot 7 empty: the chain ends here
+        if (w.w >= 0xFFFF0000u) return;  // slot 7 empty: the chain ends here
     }
-}
-
-// True when t * B^r == t for some 0 < r < m, i.e. the order of B is below m: then the giant steps
-// might not reach the smallest solution and the host decides the prime.
-template <class F>
-__device__ __forceinline__ bool has_cycle(const BsgsEntry<F>& be, const BsgsTable& T) {
-    const uint64_t xt = be.f.key(be.t);
-    uint32_t fp, b;
-    hash_key(xt, T.nbmask, fp, b);
-    bool cycle = false;
-    for (;; b = (b + 1) & T.nbmask) {
-        const uint4 w = read_bucket(T, b);
-        uint32_t mm = match_mask(w, fp);
-        while (mm) {
-            const uint32_t k = __builtin_ctz(mm);
-            mm &= mm - 1;
-            const uint32_t i = T.idx[b * 8 + k];
-            if (i != 0 && be.f.key(be.baby(i)) == xt) cycle = true;
-        }
-        if (w.w >= 0xFFFF0000u) break;
-    }
-    return cycle;
 }
 
 // x^tid for tid < 64 * WB from the entry's tables: x^(tid & 63) * (x^64)^(tid >> 6); the lane
@@ -925,8 +894,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
     const uint32_t nbk = 1u << tg.nbbits;
     BsgsTable Tb;
     Tb.fp = reinterpret_cast<uint16_t*>(lds_tab);
-    Tb.idx = Tb.fp + nbk * 8u;
-    Tb.cnt = reinterpret_cast<uint32_t*>(Tb.idx + nbk * 8u);
+    Tb.cnt = reinterpret_cast<uint32_t*>(Tb.fp + nbk * 8u);
     Tb.nbmask = nbk - 1u;
     const uint32_t n = min(pb.cnt->n_ent, pb.ent_cap);
     // entries blockIdx.x, blockIdx.x + gridDim.x, ...: no shared claim counter (one same-address
@@ -966,20 +934,26 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         }
         __syncthreads();
 
-        // baby steps t * B^i, i == tid (mod L)
+        // baby steps t * B^i, i == tid (mod L); a baby value equal to t for some 0 < i < m means
+        // the order of B is below m: the giant steps might not reach the smallest solution, and
+        // the host decides the prime
+        const uint64_t kt = f.key(be.t);
+        bool cyc = false;
         {
-            T step;
-            T v = f.mul(be.t, thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, step));
+            T bstep;
+            T v = f.mul(be.t, thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, bstep));
             for (uint32_t i = tid; i < be.m; i += L) {
+                const uint64_t x = f.key(v);
                 uint32_t fp, b;
-                hash_key(f.key(v), Tb.nbmask, fp, b);
-                tab_insert(Tb, b, fp, i);
-                v = f.mul(v, step);
+                hash_key(x, Tb.nbmask, fp, b);
+                tab_insert(Tb, b, fp);
+                cyc |= (x == kt) & (i != 0);
+                v = f.mul(v, bstep);
             }
         }
+        if (cyc) hits.n = kCycleFlag;  // every writer stores the same value
         __syncthreads();
-
-        if (has_cycle(be, Tb)) {  // the same verdict in every lane
+        if (hits.n >= kCycleFlag) {  // the same verdict in every lane
             if (tid == 0) put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, kHostDecides);
         } else {
             // giant steps G_j * (B^m)^g, g == tid (mod L), g < ng
@@ -993,7 +967,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                     hash_key(x, Tb.nbmask, fp, b);
                     const uint4 w = read_bucket(Tb, b);
                     v = f.mul(v, mstep);  // the next giant value while the LDS read is in flight
-                    if (bucket_needs_look(w, fp)) collect_hits(Tb, b, fp, x, g, j, hits);
+                    if (bucket_needs_look(w, fp)) collect_hit(Tb, b, fp, x, g, j, hits);
                 }
             }
             __syncthreads();
@@ -1007,22 +981,30 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                 const uint64_t stride = qmode ? 1 : ((be.meta >> 8) & 1u) + 1;
                 const uint64_t parity = qmode ? 0 : (be.meta >> 9) & 1u;
                 const uint64_t cmul = qmode ? be.qp : ((be.meta >> 11) & 1u) ? E[E_DJ] : item_cm;
-                for (uint32_t h = tid; h < nh; h += L) {
-                    const uint4 c = hits.h[h];
-                    const uint32_t i = c.w & 0xFFFFu, j = c.w >> 16;
-                    if (f.key(be.baby(i)) != ((uint64_t)c.x | (uint64_t)c.y << 32)) continue;
-                    uint64_t nj, Kj;
-                    if (qmode) {
-                        const uint32_t dj = (uint32_t)(E[E_DJ + (j >> 3)] >> (8 * (j & 7))) & 0xFFu;
-                        nj = be.nb + dj;
-                        Kj = (item_N1 - nj + be.qp - 1) / be.qp;
-                    } else {
-                        nj = be.nb;
-                        Kj = be.klim;
+                // the baby steps again: every baby value t * B^i equal to a hit's giant value gives
+                // the solution k = g*m - i
+                T bstep;
+                T v = f.mul(be.t, thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, bstep));
+                for (uint32_t i = tid; nh != 0 && i < be.m; i += L) {
+                    const uint64_t x = f.key(v);
+                    for (uint32_t h = 0; h < nh; ++h) {
+                        const uint4 c = hits.h[h];
+                        if (x != ((uint64_t)c.x | (uint64_t)c.y << 32)) continue;
+                        const uint32_t j = c.w;
+                        uint64_t nj, Kj;
+                        if (qmode) {
+                            const uint32_t dj = (uint32_t)(E[E_DJ + (j >> 3)] >> (8 * (j & 7))) & 0xFFu;
+                            nj = be.nb + dj;
+                            Kj = (item_N1 - nj + be.qp - 1) / be.qp;
+                        } else {
+                            nj = be.nb;
+                            Kj = be.klim;
+                        }
+                        const int64_t kk = (int64_t)c.z * be.m - (int64_t)i;
+                        if (kk < 0 || (uint64_t)kk >= Kj) continue;
+                        atomicMin(&hits.best, (unsigned long long)(nj + ((uint64_t)kk * stride + parity) * cmul));
                     }
-                    const int64_t kk = (int64_t)c.z * be.m - (int64_t)i;
-                    if (kk < 0 || (uint64_t)kk >= Kj) continue;
-                    atomicMin(&hits.best, (unsigned long long)(nj + ((uint64_t)kk * stride + parity) * cmul));
+                    v = f.mul(v, bstep);
                 }
                 __syncthreads();
                 // primality of the number is decided by k_prime_records, which runs next
@@ -1127,8 +1109,8 @@ TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load) {
     return tg;
 }
 
-// fingerprints (16 B) + indices (16 B) + a fill counter (4 B) per bucket
-size_t bsgs_lds_bytes(const TableGeom& tg) { return (size_t)36 << tg.nbbits; }
+// fingerprints (16 B) + a fill counter (4 B) per bucket
+size_t bsgs_lds_bytes(const TableGeom& tg) { return (size_t)20 << tg.nbbits; }
 
 // n_ent: entries, one workgroup each (0 = a persistent grid of `grid` workgroups that drains the
 // pass's entries).

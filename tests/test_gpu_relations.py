@@ -168,3 +168,31 @@ def test_pseudoprime_with_a_solution_is_dropped(gpu, mps, oracle, policy):
     want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 8)
     assert q not in [r[0] for r in rels]
     assert rels == want
+
+
+# primes of [1001, 1001 + 2^20] where 2 has order below 32 (divisors of 2^k - 1, k < 32)
+SMALL_ORDER = {1103: 29, 1801: 25, 2089: 29, 2731: 26, 8191: 13, 131071: 17, 178481: 23, 262657: 27, 524287: 19}
+
+
+@pytest.mark.parametrize("policy", [0, 1, 2])
+def test_small_order_primes_left_to_host(gpu, mps, oracle, policy):
+    """b = 2, c = -1: p | 2^n - 1 exactly when ord_p(2) divides n, so every prime has a relation.
+    Where the BSGS base has order below the baby-step count (the primes above, with at least 64
+    baby steps per entry) the baby values repeat and the smallest solution might be missed; the
+    device must leave those primes to the host, and the relations must be the oracle's."""
+    P0, P1 = 1001, 1001 + 2**20
+    item = (2, -1, P0, P1, 1, 100_000, 1, 0)
+    print(f"small-order primes, policy {policy}", flush=True)
+    gpu.set_class_policy(policy)
+    try:
+        recs = gpu.sieve_relations(P0, P1, mps.work_item(*item))
+        rels, _ = gpu.run_item(mps.work_item(*item), chunk_ints=1 << 21)
+    finally:
+        gpu.set_class_policy(1)
+    assert recs is not None
+    host = {p for p, M in recs if M == mps.HOST_DECIDES}
+    assert set(SMALL_ORDER) <= host
+    want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 8)
+    assert rels == want
+    got = {p: M for p, M, q in rels}
+    assert all(got[p] == o for p, o in SMALL_ORDER.items())
