@@ -188,7 +188,7 @@ CONFIGS = {  # the other configurations of the metric, as (b, c, P0, P1, N0, N1,
 }
 
 
-def run_configs(mps, g, log, reps=2):
+def run_configs(mps, g, log, reps=3):
     """Item latency and primes/s of c1, c3 and c4 on the plain worker and the batched engine."""
     out = {}
     for name, (b, c, P0, P1, N0, N1, mod, res, classes) in CONFIGS.items():
@@ -203,11 +203,12 @@ def run_configs(mps, g, log, reps=2):
                 t_plain.append(time.perf_counter() - t)
             eng = mps.Batch(g)
             t_batch, same = [], True
-            for _ in range(reps):
+            for _ in range(reps):  # the first builds the slots' graphs
                 t = time.perf_counter()
                 i = eng.submit(w)
-                done = drain(eng, 1, log, f"{name} batched")
+                eng.wait_all()
                 t_batch.append(time.perf_counter() - t)
+                done = eng.poll()
                 same = same and done[0]["status"] == 0 and eng.relations(i) == rels
             eng.close()
         finally:
@@ -345,14 +346,17 @@ def main():
         line["config"]["untuned_relations_identical"] = rels_untuned == rels
 
     # the same item through the batched engine (its passes spread over the engine's slots)
+    # (wait_all returns when the engine's last item finishes: no polling interval in the timing)
     eng = mps.Batch(g)
-    for _ in range(args.warmup):
+    for _ in range(max(1, args.warmup)):
         eng.submit(item)
-    drain(eng, args.warmup, log, "c2 batched warmup")
+    eng.wait_all()
+    eng.poll()
     t = time.perf_counter()
     bids = [eng.submit(item) for _ in range(args.steps)]
-    drain(eng, args.steps, log, "c2 batched")
+    eng.wait_all()
     ms_batched = 1000.0 * (time.perf_counter() - t) / args.steps
+    eng.poll()
     same = all(eng.relations(i) == rels for i in bids)
     eng.close()
     line["config"]["batched_ms_per_step"] = ms_batched

@@ -14,6 +14,7 @@ constexpr uint32_t kEntWords = 72;                // u64 words per BSGS entry (5
 constexpr uint32_t kMaxClasses = 24;              // classes per prime in class mode
 constexpr uint32_t kClassMaxM = 1024;             // default largest baby-step count in class mode
 constexpr size_t kMaxBsgsLds = 64 * 1024;          // LDS of one BSGS workgroup at most
+constexpr uint32_t kBsgsOversubscribe = 16;      // batch engine: BSGS grid / resident workgroups
 constexpr uint32_t kSetupTile = 1024;             // candidates a setup workgroup sorts at a time (class, order mode)
 // Setup modes (ItemParams.qmode).
 constexpr uint32_t kModePlain = 0, kModeClass = 1, kModeOrder = 2;

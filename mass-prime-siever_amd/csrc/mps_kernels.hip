@@ -1142,7 +1142,10 @@ uint32_t bsgs_persistent_grid(FieldKind fk, const TableGeom& tg, int cus) {
     const size_t lds = bsgs_lds_bytes(tg);
     int nb = fk == FIELD_F64 ? bsgs_blocks_per_cu<F64Field>(tg, lds) : bsgs_blocks_per_cu<MontField>(tg, lds);
     if (nb <= 0) nb = 1;
-    return (uint32_t)nb * (uint32_t)std::max(1, cus);
+    // kBsgsOversubscribe times the resident workgroups: each gets a few entries, and the hardware
+    // starts the next workgroup as one finishes, so a workgroup with costly entries does not hold
+    // up the pass (with exactly the resident count, the slowest static share set the pass time)
+    return kBsgsOversubscribe * (uint32_t)nb * (uint32_t)std::max(1, cus);
 }
 
 // The BSGS kernel of a variant and its workgroup size.
