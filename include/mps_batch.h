@@ -28,7 +28,7 @@ typedef struct mps_batch mps_batch;
 
 typedef struct mps_batch_opts {
     int slots;             /* passes in flight, 0 = 8, at most 16 */
-    uint64_t pass_odds;    /* odd numbers per pass (one chunk = 2 * pass_odds integers), 0 = 2^24 */
+    uint64_t pass_odds;    /* odd numbers per pass (one chunk = 2 * pass_odds integers), 0 = 2^25 */
     int use_graphs;        /* 1 = replay a captured hipGraph per pass, 0 = enqueue the kernels */
     int policy;            /* 0 = chunks in arrival order, 1 = round-robin over admitted items */
     int max_active;        /* admitted items at most, 0 = unlimited */

@@ -413,7 +413,7 @@ mps_batch* mps_batch_create(mps_gpu* g, const mps_batch_opts* opts) {
     if (opts) b->o = *opts;
     if (b->o.slots <= 0) b->o.slots = 8;
     if (b->o.slots > 16) b->o.slots = 16;
-    if (b->o.pass_odds == 0) b->o.pass_odds = 1ull << 24;
+    if (b->o.pass_odds == 0) b->o.pass_odds = 1ull << 25;
     if (b->o.pass_odds < 4096 || b->o.pass_odds > (1ull << 30)) {
         g->err = "batch pass_odds outside [4096, 2^30]";
         delete b;
