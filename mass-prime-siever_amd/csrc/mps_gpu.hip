@@ -178,6 +178,13 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
             while (cm % (cr * ip.opr[j]) == 0) cr *= ip.opr[j];
             ip.ocm[j] = (uint32_t)cr;
             ip.ocres[j] = (uint32_t)(ip.cres % cr);
+            uint64_t rinv = 1;
+            if (ip.opr[j] & 1) {
+                rinv = ip.opr[j];  // r * r == 1 (mod 8): 3 bits, doubled by each Newton step
+                for (int k = 0; k < 5; ++k) rinv *= 2 - ip.opr[j] * rinv;
+            }
+            ip.oinv[j] = rinv;
+            ip.olim[j] = ~0ull / ip.opr[j];
         }
         // an entry's class has at most kcount members: m <= sqrt(kcount) in whole rounds, <= mcap
         uint64_t m = 1;
@@ -383,18 +390,22 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     launch_setup(s, fk, ip.qmode, d, g->pb, setup_grid);
     HIPCHK(hipGetLastError(), "setup launch");
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
+    if (stop_stage == 2) {
+        if (!read_counts(g)) return false;
+        g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
+        g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
+        g->st.ms_setup += elapsed(g->ev[2], g->ev[3]);
+        return true;
+    }
+    // BSGS over the entries setup wrote, on a grid of several times the resident workgroups that
+    // reads the entry count on the device (no read-back before it)
+    launch_bsgs(s, fk, tg, d, g->pb, 0, bsgs_persistent_grid(fk, tg, g->num_cu));
+    HIPCHK(hipGetLastError(), "bsgs launch");
+    HIPCHK(hipEventRecord(g->ev[4], s), "event");
     if (!read_counts(g)) return false;
     g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
     g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
     g->st.ms_setup += elapsed(g->ev[2], g->ev[3]);
-    const uint32_t n_ent = std::min(g->h_cnt->n_ent, g->pb.ent_cap);
-    if (stop_stage == 2 || n_ent == 0) return true;
-
-    HIPCHK(hipEventRecord(g->ev[3], s), "event");
-    launch_bsgs(s, fk, tg, d, g->pb, n_ent, 0);
-    HIPCHK(hipGetLastError(), "bsgs launch");
-    HIPCHK(hipEventRecord(g->ev[4], s), "event");
-    if (!read_counts(g)) return false;
     g->st.ms_bsgs += elapsed(g->ev[3], g->ev[4]);
     g->st.n_bsgs_launches++;
     if (g->h_cnt->n_out != 0) {  // the record copy that follows on the stream waits for it

@@ -8,7 +8,7 @@ namespace mps {
 
 // Odd numbers per sieve segment: one workgroup marks one segment in an LDS bitmap.
 constexpr uint32_t kSegOdds = 1u << 17;           // 16 KiB bitmap
-constexpr uint32_t kSieveThreads = 256;
+constexpr uint32_t kSieveThreads = 1024;
 constexpr uint32_t kSmallPrimeBound = 1024;       // primes below this are shared by all lanes
 constexpr uint32_t kEntWords = 72;                // u64 words per BSGS entry (576 B)
 constexpr uint32_t kMaxClasses = 24;              // classes per prime in class mode
@@ -92,6 +92,9 @@ struct ItemParams {
     uint32_t Qo, nopr, n0modQo;
     uint32_t opr[10], oprpow[10];
     uint32_t ocrt[10], ocm[10], ocres[10];
+    // per odd prime r of Qo: r^-1 mod 2^64 and floor((2^64 - 1) / r); x is a multiple of r exactly
+    // when x * r^-1 mod 2^64 <= that bound, and the product is then x / r
+    uint64_t oinv[10], olim[10];
 };
 
 // BSGS table geometry of a pass: 2^nbbits buckets of 8 slots; a slot holds a fingerprint of

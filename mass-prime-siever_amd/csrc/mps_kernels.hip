@@ -402,7 +402,8 @@ __device__ __forceinline__ uint64_t exact_div_odd(uint64_t x, uint64_t d) {
 // the bits of g instead of those of p (b mod p != 0: the filter hands p | b to the host); a number
 // that fails it sets composite.
 // g = gcd(Qo, p - 1) from the valuations of p - 1 at the primes of Qo (prime powers in Qo are
-// below 2^16); returns g and, 4 bits per prime, the exponent of the j-th prime in g.
+// below 2^16); returns g and, 4 bits per prime, the exponent of the j-th prime in g.  Odd primes
+// divide by multiplying with their inverse modulo 2^64 (no division instruction).
 __device__ __forceinline__ uint32_t order_g(uint64_t p, const ItemParams& ip, uint64_t& gexp) {
     uint32_t g = 1;
     gexp = 0;
@@ -416,10 +417,12 @@ __device__ __forceinline__ uint32_t order_g(uint64_t p, const ItemParams& ip, ui
                 ++e;
             }
         } else {
+            uint64_t x = p - 1;
             while (ra < ip.oprpow[j]) {
-                const uint32_t dd = ra * r, t = 65536u % dd;  // 2^32 mod dd from (2^16 mod dd)^2
-                if (mod_small(p - 1, dd, t * t % dd) != 0) break;
-                ra = dd;
+                const uint64_t y = x * ip.oinv[j];
+                if (y > ip.olim[j]) break;  // r does not divide x
+                x = y;
+                ra *= r;
                 ++e;
             }
         }
@@ -427,6 +430,31 @@ __device__ __forceinline__ uint32_t order_g(uint64_t p, const ItemParams& ip, ui
         g *= ra;
     }
     return g;
+}
+
+// a mod m for a, m below 2^16 (m >= 1): the float quotient is within one of the true one.
+__device__ __forceinline__ uint32_t mod16(uint32_t a, uint32_t m) {
+    const uint32_t q = (uint32_t)((float)a * __frcp_rn((float)m));
+    int32_t r = (int32_t)(a - q * m);
+    if (r < 0) r += (int32_t)m;
+    if (r >= (int32_t)m) r -= (int32_t)m;
+    return (uint32_t)r;
+}
+
+// a mod m and a / m for a below 2^52 and 1 <= m < 2^32: the double quotient is within one of the
+// true one.
+__device__ __forceinline__ uint64_t divmod52(uint64_t a, uint32_t m, uint64_t& rem) {
+    uint64_t q = (uint64_t)((double)a / (double)m);
+    int64_t r = (int64_t)(a - q * m);
+    if (r < 0) {
+        r += m;
+        --q;
+    } else if (r >= (int64_t)m) {
+        r -= m;
+        ++q;
+    }
+    rem = (uint64_t)r;
+    return q;
 }
 
 template <class F>
@@ -456,9 +484,13 @@ __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::
         const uint32_t ej = (uint32_t)(gexp >> (4 * j)) & 15u;
         uint32_t dr = 1, kf = 0;
         if (ej != 0) {
-            uint32_t ra = 1;
-            for (uint32_t k = 0; k < ej; ++k) ra *= ip.opr[j];
-            const uint32_t h = g / ra;
+            // ra = r^ej and h = g / ra (exact: r^ej divides g), by multiplication only
+            const uint32_t r = ip.opr[j], rinv = (uint32_t)ip.oinv[j];
+            uint32_t ra = 1, h = g;
+            for (uint32_t k = 0; k < ej; ++k) {
+                ra *= r;
+                h = r == 2 ? h >> 1 : h * rinv;
+            }
             const T x = fpow(f, Be, h), y = fpow(f, Te, h);
             work += 2 * (32 - __builtin_clz(h));
             // the order dr of x is the first k >= 1 with x^k == 1; for a prime p, x^ra == b^(p-1) == 1,
@@ -478,26 +510,32 @@ __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::
                 xk = f.mul(xk, x);
             }
             work += dr;
-            if (dr == 0 || ra % dr != 0) return false;  // composite
-            if (kd == ~0u) return false;                // -c is not a power of b modulo p
+            if (dr == 0) return false;  // composite
+            // dr must be a power of r (it divides ra): divide ra by r down to dr
+            uint32_t q = ra;
+            while (q > dr) q = r == 2 ? q >> 1 : q * rinv;
+            if (q != dr) return false;  // composite
+            if (kd == ~0u) return false;  // -c is not a power of b modulo p
             kf = kd;
         }
         const uint32_t cmj = ip.ocm[j], crj = ip.ocres[j];
         uint32_t cj, lj;
         if (dr >= cmj) {
-            if (kf % cmj != crj) return false;
+            if (mod16(kf, cmj) != crj) return false;
             cj = kf;
             lj = dr;
         } else {
-            if (crj % dr != kf) return false;
+            if (mod16(crj, dr) != kf) return false;
             cj = crj;
             lj = cmj;
         }
         acc += (uint64_t)cj * ip.ocrt[j];
         Lw *= lj;
     }
-    const uint32_t nstar = (uint32_t)(acc % Lw);
-    const uint32_t off = (nstar + Lw - ip.n0modQo % Lw) % Lw;
+    uint64_t nstar, n0r;
+    divmod52(acc, Lw, nstar);
+    divmod52(ip.n0modQo, Lw, n0r);
+    const uint32_t off = (uint32_t)(nstar >= n0r ? nstar - n0r : nstar + Lw - n0r);
     L = Lw;
     nst = ip.N0 + off;
     if (nst >= ip.N1) {
@@ -505,7 +543,8 @@ __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::
         return false;
     }
     const uint64_t rest = ip.N1 - 1 - nst;
-    K = (rest < (1ull << 32) ? (uint64_t)((uint32_t)rest / Lw) : rest / Lw) + 1;
+    uint64_t rr;
+    K = (rest < (1ull << 52) ? divmod52(rest, Lw, rr) : rest / Lw) + 1;
     return true;
 }
 

@@ -109,7 +109,7 @@ def test_cli_sigterm_keeps_checkpoint_and_resumes(tmp_path):
     0; the next run resumes and its relations equal an uninterrupted run's."""
     import signal
     import time
-    item = "Factor=2,-25,1000000000,9589934592,3600000,5000000,2,1"  # 2^33 integers, 128 chunks
+    item = "Factor=2,-25,1000000000,35359738368,3600000,5000000,2,1"  # 2^35 integers, 512 chunks
     ini = "CPU_count=0\nMemoryAllowed=64\nUseGPU=1\nGPU_sieve=1\nGPU_classes=1\n"
     full_dir, cut_dir = tmp_path / "full", tmp_path / "cut"
     full_dir.mkdir()
@@ -121,10 +121,11 @@ def test_cli_sigterm_keeps_checkpoint_and_resumes(tmp_path):
     proc = subprocess.Popen([BIN], cwd=cut_dir, stdin=subprocess.DEVNULL, stdout=subprocess.PIPE,
                             stderr=subprocess.PIPE, text=True)
     log = cut_dir / "siever.log"
+    # the signal goes out once a chunk is checkpointed, well before the last of the 512
     t0 = time.time()
-    while time.time() - t0 < 120 and not (log.exists() and "Processing work item" in log.read_text()):
-        time.sleep(0.01)
-    time.sleep(0.3)
+    while time.time() - t0 < 120 and proc.poll() is None and \
+            not any(f.stat().st_size > 0 for f in cut_dir.glob("core_*.p")):
+        time.sleep(0.002)
     proc.send_signal(signal.SIGTERM)
     out, err = proc.communicate(timeout=120)
     assert proc.returncode == 0, err[-2000:]
