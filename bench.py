@@ -252,6 +252,8 @@ def main():
                     help="tuning table to load after the untuned legs ('none': keep the defaults)")
     ap.add_argument("--tune", action="store_true", help="autotune the workloads' keys live instead of loading")
     ap.add_argument("--no-configs", action="store_true", help="skip the c1/c3/c4 item legs")
+    ap.add_argument("--plain-only", action="store_true",
+                    help="only the tuned plain c2 leg (no untuned or batched legs): for kernel profiles")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB streaming copy (k_copy), the byte-count reference of HBM counters")
     args = ap.parse_args()
@@ -277,7 +279,7 @@ def main():
 
     # untuned legs first (the handle's default shapes)
     ms_untuned, rels_untuned, mix_untuned = None, None, None
-    if tuned:
+    if tuned and not args.plain_only:
         ms_untuned, _, rels_untuned, _ = time_plain(g, item, args.steps, args.warmup)
         log(f"c2 plain untuned {ms_untuned:.2f} ms/step")
         if args.mix:
@@ -345,6 +347,9 @@ def main():
         line["config"]["untuned_ms_per_step"] = ms_untuned
         line["config"]["untuned_relations_identical"] = rels_untuned == rels
 
+    if args.plain_only:
+        print(json.dumps(line), flush=True)
+        return
     # the same item through the batched engine (its passes spread over the engine's slots)
     # (wait_all returns when the engine's last item finishes: no polling interval in the timing)
     eng = mps.Batch(g)
