@@ -32,13 +32,13 @@ enum EntryWord : uint32_t {
     E_SSRC = 5,   // classes s (bits 0..7) | info/class mask (bits 8..31) | number index (bits 32..63)
     E_B64 = 6,    // B^64
     E_BM64 = 7,   // Bm^64
-    E_BLO = 8,    // B^k, k = 0..7
+    E_BLO = 8,    // t * B^k, k = 0..7
     E_BHI = 16,   // B^(8k), k = 0..7
-    E_MLO = 24,   // Bm^k, k = 0..7
+    E_MLO = 24,   // Bm^k, k = 0..7 (G * Bm^k with one class: E_META bit 12)
     E_MHI = 32,   // Bm^(8k), k = 0..7
     E_NB = 40,    // plain: n_start; class mode: N0
     E_META = 41,  // qp (bits 0..7) | stride 2 (bit 8) | odd k (bit 9) | class mode (bit 10) |
-                  // order mode (bit 11: n = n_start + k * L, L in E_DJ)
+                  // order mode (bit 11: n = n_start + k * L, L in E_DJ) | G folded into E_MLO (bit 12)
     E_KLIM = 42,  // plain: jcount (solutions k = g*m - i must lie in [0, jcount))
     E_DJ = 43,    // class mode: dj of the kept classes, 8 bits each, 3 words (n_j = N0 + dj);
                   // order mode: the entry's exponent modulus L

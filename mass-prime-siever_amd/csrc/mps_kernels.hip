@@ -354,14 +354,16 @@ __device__ __forceinline__ uint64_t isqrt_ceil64(uint64_t x) {
     return r * r < x ? r + 1 : r;
 }
 
-// Writes the powers tables of base x: tab[k] = x^k and tab[8 + k] = x^(8k), k < 8; returns x^64.
+// Writes the powers tables of base x: tab[k] = first * x^k and tab[8 + k] = x^(8k), k < 8; returns
+// x^64.  A lane's start first * x^tid then takes one product of two table words (two with x^64).
 template <class F>
-__device__ __forceinline__ typename F::T write_pow_tables(const F& f, typename F::T x, uint64_t* tab,
-                                                         unsigned long long& work) {
+__device__ __forceinline__ typename F::T write_pow_tables(const F& f, typename F::T x, typename F::T first,
+                                                         uint64_t* tab, unsigned long long& work) {
     typedef typename F::T T;
     T a = f.one();
+    tab[0] = F::to_bits(first);
     for (int k = 0; k < 8; ++k) {
-        tab[k] = F::to_bits(a);
+        if (k > 0) tab[k] = F::to_bits(f.mul(first, a));
         a = f.mul(a, x);
     }
     const T x8 = a;
@@ -370,7 +372,7 @@ __device__ __forceinline__ typename F::T write_pow_tables(const F& f, typename F
         tab[8 + k] = F::to_bits(c);
         c = f.mul(c, x8);
     }
-    work += 16;
+    work += 23;
     return c;  // x^64
 }
 
@@ -717,6 +719,7 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                 const T rr = f.r2();
                 const T bm = f.from(bmod, rr), tm = f.from(t, rr);
                 T B;
+                T G1 = f.one();  // the giant start when the entry has one class
                 E[E_P] = p;
                 E[E_AUX] = f.aux();
                 E[E_ONE] = F::to_bits(f.one());
@@ -751,7 +754,8 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                     E[E_DJ + 2] = djw[2];
                 } else if (MODE == kModeOrder) {
                     B = fpow(f, bm, oL);
-                    E[E_G] = F::to_bits(fpow(f, bm, onst));
+                    G1 = fpow(f, bm, onst);
+                    E[E_G] = F::to_bits(G1);
                     w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
                     E[E_NB] = onst;
                     E[E_META] = 1u << 11;
@@ -765,6 +769,7 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                         if (info & 2u) G0 = f.mul(G0, B);
                         B = f.mul(B, B);
                     }
+                    G1 = G0;
                     E[E_G] = F::to_bits(G0);
                     E[E_NB] = ip.n_start;
                     E[E_META] = ((info & 1u) ? (1u << 8) : 0u) | ((info & 2u) ? (1u << 9) : 0u);
@@ -772,8 +777,11 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                 }
                 const T Bm = fpow(f, B, m);
                 w_setup += 2 * bitlen64(m);
-                const T B64 = write_pow_tables(f, B, E + E_BLO, w_setup);
-                const T Bm64 = write_pow_tables(f, Bm, E + E_MLO, w_setup);
+                // baby starts t * B^tid; with one class (plain and order mode) the giant starts
+                // G * (B^m)^tid as well (E_META bit 12)
+                const T B64 = write_pow_tables(f, B, tm, E + E_BLO, w_setup);
+                const T Bm64 = write_pow_tables(f, Bm, G1, E + E_MLO, w_setup);
+                if (!QMODE) E[E_META] |= 1u << 12;
                 E[E_B64] = F::to_bits(B64);
                 E[E_BM64] = F::to_bits(Bm64);
                 E[E_BL] = F::to_bits(lane_stride_pow(f, B64, ip.lanes));
@@ -980,7 +988,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         bool cyc = false;
         {
             T bstep;
-            T v = f.mul(be.t, thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, bstep));
+            T v = thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, bstep);  // t * B^tid
             for (uint32_t i = tid; i < be.m; i += L) {
                 const uint64_t x = f.key(v);
                 uint32_t fp, b;
@@ -998,8 +1006,9 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             // giant steps G_j * (B^m)^g, g == tid (mod L), g < ng
             T mstep;
             const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(E[E_BM64]), E_BML, tid, mstep);
+            const bool gfold = (be.meta >> 12) & 1u;  // the giant tables hold G * (B^m)^k
             for (uint32_t j = 0; j < be.s; ++j) {
-                T v = f.mul(F::from_bits(E[E_G + j]), Mt);
+                T v = gfold ? Mt : f.mul(F::from_bits(E[E_G + j]), Mt);
                 for (uint32_t g = tid; g < be.ng; g += L) {
                     const uint64_t x = f.key(v);
                     uint32_t fp, b;
@@ -1023,7 +1032,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                 // the baby steps again: every baby value t * B^i equal to a hit's giant value gives
                 // the solution k = g*m - i
                 T bstep;
-                T v = f.mul(be.t, thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, bstep));
+                T v = thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, bstep);  // t * B^tid
                 for (uint32_t i = tid; nh != 0 && i < be.m; i += L) {
                     const uint64_t x = f.key(v);
                     for (uint32_t h = 0; h < nh; ++h) {
