@@ -818,13 +818,13 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
 
 // ------------------------------------------------------------------------------------- bsgs
 //
-// Table (LDS, one per entry): 2^nbbits buckets of 8 slots.  A slot is a 16-bit fingerprint in the
-// fingerprint array (one 16-byte bucket = one 128-bit LDS read) and a 16-bit baby index in a
-// parallel array read only on a fingerprint match; one u32 fill counter per bucket.  A baby step
-// takes the next slot of its home bucket from the counter (slots fill in order, so a bucket is full
-// exactly when slot 7 is written) and spills to the next bucket when full.  A lookup reads its
+// Table (LDS, one per entry): 2^nbbits buckets of 8 slots.  A slot is a 16-bit fingerprint (one
+// 16-byte bucket = one 128-bit LDS read); one u32 fill counter per bucket; no baby indices.  A baby
+// step takes the next slot of its home bucket from the counter (slots fill in order, so a bucket is
+// full exactly when slot 7 is written) and spills to the next bucket when full.  A lookup reads its
 // home bucket and continues only while buckets are full.  Every fingerprint match is verified
-// exactly (t * B^i against the giant value), so the device verdicts are exact.
+// exactly after the giant loop (the baby values t * B^i are run again and compared with the giant
+// value), so the device verdicts are exact.
 //
 // Hash of a canonical residue x (low word lo): fingerprint = lo & 0xFFFF (0xFFFF marks an empty
 // slot, so it maps to 0xFFFE), bucket = ((lo >> 16) ^ lo) & (2^nbbits - 1).
