@@ -182,6 +182,26 @@ __device__ __forceinline__ typename F::T fpow(const F& f, typename F::T base, ui
     return r;
 }
 
+// a^e and b^e in one loop (the exponent's bits are walked once for both); E is uint32_t or
+// uint64_t.
+template <class F, class E>
+__device__ __forceinline__ void fpow2(const F& f, typename F::T a, typename F::T b, E e, typename F::T& ra,
+                                      typename F::T& rb) {
+    ra = f.one();
+    rb = f.one();
+    while (e) {
+        if (e & 1) {
+            ra = f.mul(ra, a);
+            rb = f.mul(rb, b);
+        }
+        e >>= 1;
+        if (e) {
+            a = f.mul(a, a);
+            b = f.mul(b, b);
+        }
+    }
+}
+
 template <class F>
 __device__ __forceinline__ bool fequal(const F& f, typename F::T a, typename F::T b) {
     return f.key(a) == f.key(b);

@@ -467,7 +467,8 @@ __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::
     typedef typename F::T T;
     const uint32_t tz = __builtin_ctz(g);
     const uint64_t e = exact_div_odd((p - 1) >> tz, g >> tz);
-    const T Be = fpow(f, bm, e), Te = fpow(f, tm, e);
+    T Be, Te;
+    fpow2(f, bm, tm, e, Be, Te);
     work += 2 * (64 - __builtin_clzll(e));
     const uint64_t k1 = f.key(f.one());
     if (need_prime) {
@@ -493,7 +494,8 @@ __device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::
                 ra *= r;
                 h = r == 2 ? h >> 1 : h * rinv;
             }
-            const T x = fpow(f, Be, h), y = fpow(f, Te, h);
+            T x, y;
+            fpow2(f, Be, Te, h, x, y);
             work += 2 * (32 - __builtin_clz(h));
             // the order dr of x is the first k >= 1 with x^k == 1; for a prime p, x^ra == b^(p-1) == 1,
             // so dr divides ra.  A number where that fails is composite (a Fermat pseudoprime that
@@ -638,7 +640,8 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                         if (cs.g > 2) {
                             // class j needs t^e == (b^e)^(n_j mod g), e = (p-1)/g
                             const uint64_t e = (p - 1) / cs.g;
-                            const T u = fpow(f, tm, e), w = fpow(f, bm, e);
+                            T u, w;
+                            fpow2(f, tm, bm, e, u, w);
                             w_setup += 2 * bitlen64(e) + cs.g;
                             const uint64_t ku = f.key(u);
                             const uint32_t n0g = ip.n0modQ % cs.g;
@@ -684,7 +687,9 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                             // B = b^cm is a g-th power, so T = t*b^-n_start must be one: t^e == G0^e
                             const uint64_t e = (p - 1) / g;
                             const T G0 = fpow(f, bm, ip.n_start);
-                            ok = f.key(fpow(f, tm, e)) == f.key(fpow(f, G0, e));
+                            T te, ge;
+                            fpow2(f, tm, G0, e, te, ge);
+                            ok = f.key(te) == f.key(ge);
                             w_setup += 2 * bitlen64(e) + bitlen64(ip.n_start);
                         }
                         const uint32_t stride = (info & 1u) ? 2 : 1, parity = (info >> 1) & 1u;
