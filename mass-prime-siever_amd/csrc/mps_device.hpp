@@ -182,6 +182,21 @@ __device__ __forceinline__ typename F::T fpow(const F& f, typename F::T base, ui
     return r;
 }
 
+// x^e1 and x^e2 in one loop: the squarings of x are shared, the two products are independent.
+template <class F>
+__device__ __forceinline__ void fpow_2e(const F& f, typename F::T x, uint64_t e1, uint64_t e2, typename F::T& r1,
+                                        typename F::T& r2) {
+    r1 = f.one();
+    r2 = f.one();
+    while (e1 | e2) {
+        if (e1 & 1) r1 = f.mul(r1, x);
+        if (e2 & 1) r2 = f.mul(r2, x);
+        e1 >>= 1;
+        e2 >>= 1;
+        if (e1 | e2) x = f.mul(x, x);
+    }
+}
+
 // a^e and b^e in one loop (the exponent's bits are walked once for both); E is uint32_t or
 // uint64_t.
 template <class F, class E>

@@ -732,12 +732,12 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                 E[E_MG] = m | (ng << 32);
                 E[E_SSRC] = (uint64_t)s | ((uint64_t)((QMODE ? mask : info) & 0xFFFFFFu) << 8) | ((uint64_t)idx << 32);
                 if (QMODE) {
-                    B = fpow(f, bm, cs.qp);
+                    // G_j = b^(N0 + dj) = b^(N0 + dmin) * b^(dj - dmin); B and the base share squarings
+                    T base;
+                    fpow_2e(f, bm, cs.qp, ip.N0 + dmin, B, base);
                     E[E_NB] = ip.N0;
                     E[E_META] = cs.qp | (1u << 10);
                     E[E_KLIM] = 0;
-                    // G_j = b^(N0 + dj) = b^(N0 + dmin) * b^(dj - dmin)
-                    const T base = fpow(f, bm, ip.N0 + dmin);
                     w_setup += 2 * bitlen64(ip.N0 + dmin) + 12;
                     uint32_t o = 0;
                     uint64_t djw[3] = {0, 0, 0};
@@ -758,8 +758,7 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                     E[E_DJ + 1] = djw[1];
                     E[E_DJ + 2] = djw[2];
                 } else if (MODE == kModeOrder) {
-                    B = fpow(f, bm, oL);
-                    G1 = fpow(f, bm, onst);
+                    fpow_2e(f, bm, oL, onst, B, G1);
                     E[E_G] = F::to_bits(G1);
                     w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
                     E[E_NB] = onst;
@@ -767,8 +766,8 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                     E[E_KLIM] = jc;
                     E[E_DJ] = oL;
                 } else {
-                    B = fpow(f, bm, ip.cm);
-                    T G0 = fpow(f, bm, ip.n_start);
+                    T G0;
+                    fpow_2e(f, bm, ip.cm, ip.n_start, B, G0);
                     w_setup += 2 * bitlen64(ip.n_start) + 2 * bitlen64(ip.cm);
                     if (info & 1u) {  // stride 2: k = 2j + parity
                         if (info & 2u) G0 = f.mul(G0, B);
