@@ -130,6 +130,10 @@ struct MontField {
     }
     __device__ __forceinline__ T from(uint64_t x, T rr) const { return mul(x, rr); }  // x < p
     __device__ __forceinline__ uint64_t key(T a) const { return a; }
+    // A 32-bit word of key(a) for hash tables: Montgomery forms a * 2^64 mod p are already spread
+    // over all bits, so the low word serves.
+    __device__ __forceinline__ uint32_t hash_word(T a) const { return (uint32_t)a; }
+    __device__ __forceinline__ uint32_t hash_word_of_key(uint64_t k) const { return (uint32_t)k; }
     __device__ __forceinline__ uint64_t aux() const { return ninv; }
     __device__ __forceinline__ static uint64_t to_bits(T a) { return a; }
     __device__ __forceinline__ static T from_bits(uint64_t a) { return a; }
@@ -138,14 +142,17 @@ struct MontField {
 struct F64Field {
     typedef double T;
     double p, pinv;
+    uint32_t plo;  // low word of p
 
     __device__ __forceinline__ void init(uint64_t mod) {
         p = (double)mod;
         pinv = 1.0 / p;
+        plo = (uint32_t)mod;
     }
     __device__ __forceinline__ void set(uint64_t mod, uint64_t pinv_bits, uint64_t) {
         p = (double)mod;
         pinv = __longlong_as_double((long long)pinv_bits);
+        plo = (uint32_t)mod;
     }
     __device__ __forceinline__ T one() const { return 1.0; }
     __device__ __forceinline__ T mul(T a, T b) const {
@@ -166,6 +173,16 @@ struct F64Field {
         const double c = a < 0.0 ? a + p : a;
         return (uint64_t)__double_as_longlong(c + 4503599627370496.0) & 0x000FFFFFFFFFFFFFULL;
     }
+    // A 32-bit word of key(a) for hash tables: the low word of key(a) times an odd constant (a
+    // bijection of the low word, so equal words mean equal low words).  Plain residues have structure
+    // (t * B^i below p are small multiples of powers of b), which the product spreads over all bits.
+    // a + 1.5 * 2^52 (|a| < 2^51) holds a + 2^51 in its mantissa, whose low word is a mod 2^32; a
+    // negative a stands for a + p.
+    __device__ __forceinline__ uint32_t hash_word(T a) const {
+        const uint32_t lo = (uint32_t)__double_as_longlong(a + 6755399441055744.0);
+        return (a < 0.0 ? lo + plo : lo) * 0x9E3779B1u;
+    }
+    __device__ __forceinline__ uint32_t hash_word_of_key(uint64_t k) const { return (uint32_t)k * 0x9E3779B1u; }
     __device__ __forceinline__ uint64_t aux() const { return (uint64_t)__double_as_longlong(pinv); }
     __device__ __forceinline__ static uint64_t to_bits(T a) { return (uint64_t)__double_as_longlong(a); }
     __device__ __forceinline__ static T from_bits(uint64_t a) { return __longlong_as_double((long long)a); }

@@ -493,6 +493,11 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
         mps_gpu_destroy(g);
         return nullptr;
     }
+    if (!bsgs_lds_at_zero()) {
+        write_str(banner, banner_cap, "GPU disabled: the BSGS kernel's table is not at LDS address 0");
+        mps_gpu_destroy(g);
+        return nullptr;
+    }
     g->ok = true;
     char buf[512];
     std::snprintf(buf, sizeof(buf), "GPU enabled (HIP): %s (%s), %d CUs, global mem %zu MB", prop.name,

@@ -144,6 +144,9 @@ void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const PassDes
 // Workgroups of this BSGS variant resident on the device at once (cus compute units): the grid
 // of a launch that drains however many entries the pass produced.
 uint32_t bsgs_persistent_grid(FieldKind fk, const TableGeom& tg, int cus);
+// k_bsgs addresses its table by LDS byte offset from 0: true when no variant holds static LDS (then
+// the dynamic LDS, which holds the table, starts at address 0).
+bool bsgs_lds_at_zero();
 
 // One pass as a fixed chain: descriptor upload, reset, sieve, filter, setup, BSGS, counter and
 // record download.  Every grid is fixed (the kernels read their element counts on the device).

@@ -830,42 +830,51 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
 // exactly after the giant loop (the baby values t * B^i are run again and compared with the giant
 // value), so the device verdicts are exact.
 //
-// Hash of a canonical residue x (low word lo): fingerprint = lo & 0xFFFF (0xFFFF marks an empty
-// slot, so it maps to 0xFFFE), bucket = ((lo >> 16) ^ lo) & (2^nbbits - 1).
+// Hash of a residue from its field's 32-bit hash word w (F::hash_word): fingerprint = w & 0xFFFF
+// (0xFFFF marks an empty slot, so it maps to 0xFFFE), bucket = (w >> 20) & (2^nbbits - 1), carried as
+// the bucket's byte offset (w >> 16) & (mask << 4).  Fingerprint and bucket come from disjoint bits
+// of w.
 
 constexpr uint32_t kEmpty16 = 0xFFFFu;
 
 struct BsgsTable {
     uint16_t* fp;    // [nbk * 8]
     uint32_t* cnt;   // [nbk]
-    uint32_t nbmask;
+    uint32_t omask;  // (nbk - 1) << 4: bucket byte offsets
 };
 
-__device__ __forceinline__ void hash_key(uint64_t x, uint32_t nbmask, uint32_t& fp, uint32_t& bucket) {
-    const uint32_t lo = (uint32_t)x;
-    fp = min(lo & 0xFFFFu, 0xFFFEu);
-    bucket = ((lo >> 16) ^ lo) & nbmask;
+__device__ __forceinline__ void hash_word(uint32_t w, uint32_t omask, uint32_t& fp, uint32_t& boff) {
+    fp = min(w & 0xFFFFu, 0xFFFEu);
+    boff = (w >> 16) & omask;
 }
 
-__device__ __forceinline__ void tab_insert(const BsgsTable& T, uint32_t bucket, uint32_t fp) {
+__device__ __forceinline__ void tab_insert(const BsgsTable& T, uint32_t boff, uint32_t fp) {
     for (;;) {
-        const uint32_t old = atomicAdd(&T.cnt[bucket], 1u);
+        const uint32_t old = atomicAdd(&T.cnt[boff >> 4], 1u);
         if (old < 8) {
-            T.fp[bucket * 8 + old] = (uint16_t)fp;
+            T.fp[(boff >> 1) + old] = (uint16_t)fp;
             return;
         }
-        bucket = (bucket + 1) & T.nbmask;
+        boff = (boff + 16u) & T.omask;
     }
 }
 
-__device__ __forceinline__ uint4 read_bucket(const BsgsTable& T, uint32_t bucket) {
-    return reinterpret_cast<const uint4*>(T.fp)[bucket];
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) const u32x4 lds_u32x4;
+
+// The bucket at byte offset boff.  The table starts the kernel's dynamic LDS, which starts at LDS
+// address 0 because k_bsgs has no static LDS (bsgs_lds_at_zero checks that on the host), so boff is
+// the bucket's LDS address: the giant loop needs no add for the table base.
+__device__ __forceinline__ uint4 read_bucket(const BsgsTable&, uint32_t boff) {
+    const u32x4 w = *reinterpret_cast<lds_u32x4*>((size_t)boff);
+    return make_uint4(w.x, w.y, w.z, w.w);
 }
 
-// Some slot of the bucket holds fingerprint fp, or the bucket is full (the chain goes on).
+// Some slot of the bucket holds fingerprint fp, or the bucket is full (the chain goes on).  fp is
+// never 0xFFFF, so slot 7 holding fp implies slot 7 not empty: the full test covers slot 7.
 __device__ __forceinline__ bool bucket_needs_look(uint4 w, uint32_t fp) {
     const bool m = ((w.x & 0xFFFFu) == fp) | ((w.x >> 16) == fp) | ((w.y & 0xFFFFu) == fp) | ((w.y >> 16) == fp) |
-                   ((w.z & 0xFFFFu) == fp) | ((w.z >> 16) == fp) | ((w.w & 0xFFFFu) == fp) | ((w.w >> 16) == fp);
+                   ((w.z & 0xFFFFu) == fp) | ((w.z >> 16) == fp) | ((w.w & 0xFFFFu) == fp);
     return m | (w.w < 0xFFFF0000u);
 }
 
@@ -903,9 +912,9 @@ struct HitList {
 };
 
 // A giant value whose fingerprint is somewhere in the chain from bucket goes to the hit list once.
-__device__ __forceinline__ void collect_hit(const BsgsTable& T, uint32_t bucket, uint32_t fp, uint64_t xv, uint32_t g,
+__device__ __forceinline__ void collect_hit(const BsgsTable& T, uint32_t boff, uint32_t fp, uint64_t xv, uint32_t g,
                                             uint32_t j, HitList& hl) {
-    for (uint32_t b = bucket;; b = (b + 1) & T.nbmask) {
+    for (uint32_t b = boff;; b = (b + 16u) & T.omask) {
         const uint4 w = read_bucket(T, b);
         if (match_mask(w, fp) != 0) {
             const uint32_t slot = atomicAdd(&hl.n, 1u);
@@ -939,14 +948,15 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
     const TableGeom tg = d->tg;
     typedef typename F::T T;
     constexpr uint32_t L = 64 * WB;
+    // dynamic LDS: the table (20 bytes per bucket) at offset 0, so that a bucket's byte offset is its
+    // LDS address, then the hit list
     extern __shared__ uint32_t lds_tab[];
-    __shared__ HitList hits;
     const uint32_t tid = threadIdx.x;
     const uint32_t nbk = 1u << tg.nbbits;
+    HitList& hits = *reinterpret_cast<HitList*>(lds_tab + 5u * nbk);
     BsgsTable Tb;
     Tb.fp = reinterpret_cast<uint16_t*>(lds_tab);
     Tb.cnt = reinterpret_cast<uint32_t*>(Tb.fp + nbk * 8u);
-    Tb.nbmask = nbk - 1u;
     const uint32_t n = min(pb.cnt->n_ent, pb.ent_cap);
     // entries blockIdx.x, blockIdx.x + gridDim.x, ...: no shared claim counter (one same-address
     // atomic per entry costs more than a short entry's whole BSGS)
@@ -973,7 +983,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         uint32_t nbe = tg.nbbits;
         while (nbe > 4 && ((uint64_t)tg.load << (nbe - 1)) >= be.m) --nbe;
         const uint32_t nbk_e = 1u << nbe;
-        Tb.nbmask = nbk_e - 1u;
+        Tb.omask = (nbk_e - 1u) << 4;
         uint4* fp4 = reinterpret_cast<uint4*>(Tb.fp);
         for (uint32_t i = tid; i < nbk_e; i += L) {
             fp4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
@@ -993,14 +1003,20 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         {
             T bstep;
             T v = thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, bstep);  // t * B^tid
-            for (uint32_t i = tid; i < be.m; i += L) {
-                const uint64_t x = f.key(v);
-                uint32_t fp, b;
-                hash_key(x, Tb.nbmask, fp, b);
-                tab_insert(Tb, b, fp);
-                cyc |= (x == kt) & (i != 0);
+            // rounds in which every lane steps run on a uniform counter; lanes below the remainder
+            // take one step more
+            const uint32_t rounds = be.m / L, rem = be.m - rounds * L;
+            const uint32_t wt = f.hash_word_of_key(kt);
+            auto baby = [&](uint32_t i) {
+                const uint32_t w = f.hash_word(v);
+                uint32_t fp, bo;
+                hash_word(w, Tb.omask, fp, bo);
+                tab_insert(Tb, bo, fp);
+                if ((w == wt) & (i != 0)) cyc |= f.key(v) == kt;
                 v = f.mul(v, bstep);
-            }
+            };
+            for (uint32_t r = 0; r < rounds; ++r) baby(tid + r * L);
+            if (tid < rem) baby(tid + rounds * L);
         }
         if (cyc) hits.n = kCycleFlag;  // every writer stores the same value
         __syncthreads();
@@ -1011,16 +1027,19 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             T mstep;
             const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(E[E_BM64]), E_BML, tid, mstep);
             const bool gfold = (be.meta >> 12) & 1u;  // the giant tables hold G * (B^m)^k
+            const uint32_t rounds = be.ng / L, rem = be.ng - rounds * L;
             for (uint32_t j = 0; j < be.s; ++j) {
                 T v = gfold ? Mt : f.mul(F::from_bits(E[E_G + j]), Mt);
-                for (uint32_t g = tid; g < be.ng; g += L) {
-                    const uint64_t x = f.key(v);
-                    uint32_t fp, b;
-                    hash_key(x, Tb.nbmask, fp, b);
-                    const uint4 w = read_bucket(Tb, b);
+                auto giant = [&](uint32_t g) {
+                    uint32_t fp, bo;
+                    hash_word(f.hash_word(v), Tb.omask, fp, bo);
+                    const uint4 w = read_bucket(Tb, bo);
+                    const T vg = v;
                     v = f.mul(v, mstep);  // the next giant value while the LDS read is in flight
-                    if (bucket_needs_look(w, fp)) collect_hit(Tb, b, fp, x, g, j, hits);
-                }
+                    if (bucket_needs_look(w, fp)) collect_hit(Tb, bo, fp, f.key(vg), g, j, hits);
+                };
+                for (uint32_t r = 0; r < rounds; ++r) giant(tid + r * L);
+                if (tid < rem) giant(tid + rounds * L);
             }
             __syncthreads();
             // exact check of every match: the solution k = g*m - i, when in range, gives
@@ -1162,7 +1181,7 @@ TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load) {
 }
 
 // fingerprints (16 B) + a fill counter (4 B) per bucket
-size_t bsgs_lds_bytes(const TableGeom& tg) { return (size_t)20 << tg.nbbits; }
+size_t bsgs_lds_bytes(const TableGeom& tg) { return ((size_t)20 << tg.nbbits) + sizeof(HitList); }
 
 // n_ent: entries, one workgroup each (0 = a persistent grid of `grid` workgroups that drains the
 // pass's entries).
@@ -1198,6 +1217,17 @@ uint32_t bsgs_persistent_grid(FieldKind fk, const TableGeom& tg, int cus) {
     // starts the next workgroup as one finishes, so a workgroup with costly entries does not hold
     // up the pass (with exactly the resident count, the slowest static share set the pass time)
     return kBsgsOversubscribe * (uint32_t)nb * (uint32_t)std::max(1, cus);
+}
+
+bool bsgs_lds_at_zero() {
+    const void* k[6] = {reinterpret_cast<const void*>(&k_bsgs<F64Field, 1>), reinterpret_cast<const void*>(&k_bsgs<F64Field, 2>),
+                        reinterpret_cast<const void*>(&k_bsgs<F64Field, 4>), reinterpret_cast<const void*>(&k_bsgs<MontField, 1>),
+                        reinterpret_cast<const void*>(&k_bsgs<MontField, 2>), reinterpret_cast<const void*>(&k_bsgs<MontField, 4>)};
+    for (const void* f : k) {
+        hipFuncAttributes a;
+        if (hipFuncGetAttributes(&a, f) != hipSuccess || a.sharedSizeBytes != 0) return false;
+    }
+    return true;
 }
 
 // The BSGS kernel of a variant and its workgroup size.
