@@ -133,7 +133,7 @@ struct MontField {
     // A 32-bit word of key(a) for hash tables: Montgomery forms a * 2^64 mod p are already spread
     // over all bits, so the low word serves.
     __device__ __forceinline__ uint32_t hash_word(T a) const { return (uint32_t)a; }
-    __device__ __forceinline__ uint32_t hash_word_of_key(uint64_t k) const { return (uint32_t)k; }
+
     __device__ __forceinline__ uint64_t aux() const { return ninv; }
     __device__ __forceinline__ static uint64_t to_bits(T a) { return a; }
     __device__ __forceinline__ static T from_bits(uint64_t a) { return a; }
@@ -182,7 +182,6 @@ struct F64Field {
         const uint32_t lo = (uint32_t)__double_as_longlong(a + 6755399441055744.0);
         return (a < 0.0 ? lo + plo : lo) * 0x9E3779B1u;
     }
-    __device__ __forceinline__ uint32_t hash_word_of_key(uint64_t k) const { return (uint32_t)k * 0x9E3779B1u; }
     __device__ __forceinline__ uint64_t aux() const { return (uint64_t)__double_as_longlong(pinv); }
     __device__ __forceinline__ static uint64_t to_bits(T a) { return (uint64_t)__double_as_longlong(a); }
     __device__ __forceinline__ static T from_bits(uint64_t a) { return __longlong_as_double((long long)a); }

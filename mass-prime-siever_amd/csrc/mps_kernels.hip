@@ -979,15 +979,21 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         be.B64 = F::from_bits(E[E_B64]);
         const F& f = be.f;
 
-        // the entry uses as many buckets as its m needs (at most the launch's table)
-        uint32_t nbe = tg.nbbits;
-        while (nbe > 4 && ((uint64_t)tg.load << (nbe - 1)) >= be.m) --nbe;
+        // the entry uses as many buckets as its m needs, at least 16 and at most the launch's table:
+        // the largest nbe with load * 2^(nbe-1) < m, i.e. bitlen(m - 1) - log2(load) (load is 2 or 4)
+        const int nbe_m = (int)(32u - __clz(be.m - 1u)) - (int)(31u - __clz(tg.load));
+        const uint32_t nbe = (uint32_t)max(4, min((int)tg.nbbits, nbe_m));
         const uint32_t nbk_e = 1u << nbe;
         Tb.omask = (nbk_e - 1u) << 4;
         uint4* fp4 = reinterpret_cast<uint4*>(Tb.fp);
-        for (uint32_t i = tid; i < nbk_e; i += L) {
-            fp4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
-            Tb.cnt[i] = 0u;
+        if (nbk_e >= L) {  // both powers of two: every lane clears nbk_e / L buckets
+            for (uint32_t k = 0; k < nbk_e / L; ++k) {
+                fp4[tid + k * L] = make_uint4(~0u, ~0u, ~0u, ~0u);
+                Tb.cnt[tid + k * L] = 0u;
+            }
+        } else if (tid < nbk_e) {
+            fp4[tid] = make_uint4(~0u, ~0u, ~0u, ~0u);
+            Tb.cnt[tid] = 0u;
         }
         if (tid == 0) {
             hits.n = 0;
@@ -998,21 +1004,31 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         // baby steps t * B^i, i == tid (mod L); a baby value equal to t for some 0 < i < m means
         // the order of B is below m: the giant steps might not reach the smallest solution, and
         // the host decides the prime
-        const uint64_t kt = f.key(be.t);
         bool cyc = false;
+        // the workgroup's next entry: one load per cache line of its first 48 words (the header, the
+        // start tables and the strides) sets those lines on their way to the cache while this
+        // entry's steps run; consumed at the end of the iteration, when it has long arrived
+        uint32_t pf;
         {
             T bstep;
             T v = thread_pow<F, WB>(f, E, E_BLO, E_BHI, be.B64, E_BL, tid, bstep);  // t * B^tid
+            __builtin_amdgcn_sched_barrier(0);  // issued after this entry's table loads
+            {  // no branch around the load, so the wait for this entry's tables does not include it
+                const uint32_t enx = e + gridDim.x < n ? e + gridDim.x : e;
+                // one 32-bit word: a register the load writes stays live until the wait at the end
+                pf = reinterpret_cast<const uint32_t*>(pb.ent)[((uint64_t)enx * kEntWords + min((tid & 3u) * 16u, 47u)) * 2];
+            }
+            __builtin_amdgcn_sched_barrier(0);
             // rounds in which every lane steps run on a uniform counter; lanes below the remainder
             // take one step more
             const uint32_t rounds = be.m / L, rem = be.m - rounds * L;
-            const uint32_t wt = f.hash_word_of_key(kt);
+            const uint32_t wt = f.hash_word(be.t);
             auto baby = [&](uint32_t i) {
                 const uint32_t w = f.hash_word(v);
                 uint32_t fp, bo;
                 hash_word(w, Tb.omask, fp, bo);
                 tab_insert(Tb, bo, fp);
-                if ((w == wt) & (i != 0)) cyc |= f.key(v) == kt;
+                if ((w == wt) & (i != 0)) cyc |= f.key(v) == f.key(be.t);
                 v = f.mul(v, bstep);
             };
             for (uint32_t r = 0; r < rounds; ++r) baby(tid + r * L);
@@ -1083,6 +1099,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                     put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, hits.best);
             }
         }
+        asm volatile("" ::"v"(pf));
         __syncthreads();
     }
 }
