@@ -902,7 +902,7 @@ struct BsgsEntry {
 // indices, so the check runs the baby steps again and compares every baby value with every hit's
 // key exactly.  More than kMaxHits hits in one entry (a short cycle of the giant values, say)
 // leave the number to the host.
-constexpr uint32_t kMaxHits = 32;
+constexpr uint32_t kMaxHits = 16;
 constexpr uint32_t kCycleFlag = 1u << 30;  // hit count of an entry whose baby values repeat
 
 struct HitList {
@@ -949,34 +949,58 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
     typedef typename F::T T;
     constexpr uint32_t L = 64 * WB;
     // dynamic LDS: the table (20 bytes per bucket) at offset 0, so that a bucket's byte offset is its
-    // LDS address, then the hit list
+    // LDS address, then two hit lists used by alternate entries
     extern __shared__ uint32_t lds_tab[];
     const uint32_t tid = threadIdx.x;
     const uint32_t nbk = 1u << tg.nbbits;
-    HitList& hits = *reinterpret_cast<HitList*>(lds_tab + 5u * nbk);
+    HitList* hitl = reinterpret_cast<HitList*>(lds_tab + 5u * nbk);
     BsgsTable Tb;
     Tb.fp = reinterpret_cast<uint16_t*>(lds_tab);
     Tb.cnt = reinterpret_cast<uint32_t*>(Tb.fp + nbk * 8u);
+    uint4* fp4 = reinterpret_cast<uint4*>(Tb.fp);
     const uint32_t n = min(pb.cnt->n_ent, pb.ent_cap);
+    // the table starts empty; afterwards every entry empties the buckets it used before the barrier
+    // that ends it, so the next entry needs no barrier of its own between the clear and its inserts
+    for (uint32_t i = tid; i < nbk; i += L) {
+        fp4[i] = make_uint4(~0u, ~0u, ~0u, ~0u);
+        Tb.cnt[i] = 0u;
+    }
+    if (tid == 0) {
+        hitl[0].n = 0;
+        hitl[0].best = ~0ull;
+    }
+    __syncthreads();
+    uint32_t par = 0;
     // entries blockIdx.x, blockIdx.x + gridDim.x, ...: no shared claim counter (one same-address
     // atomic per entry costs more than a short entry's whole BSGS)
-    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
+    for (uint32_t e = blockIdx.x; e < n; e += gridDim.x, par ^= 1u) {
+        HitList& hits = hitl[par];
+        // the other list, last read before the previous entry's end barrier, is made ready for the
+        // next entry (which reads it after this entry's end barrier)
+        if (tid == 0) {
+            hitl[par ^ 1u].n = 0;
+            hitl[par ^ 1u].best = ~0ull;
+        }
         BsgsEntry<F> be;
         be.E = pb.ent + (uint64_t)e * kEntWords;
         const uint64_t* __restrict__ E = be.E;
-        const uint64_t p = E[E_P];
-        be.f.set(p, E[E_AUX], E[E_ONE]);
-        const uint64_t mg = E[E_MG];
+        // the entry's uniform words through the constant address space: scalar loads (k_setup wrote
+        // the entries in an earlier launch; nothing writes them during this one)
+        typedef __attribute__((address_space(4))) const uint64_t cu64;
+        const cu64* Ec = reinterpret_cast<const cu64*>(reinterpret_cast<uintptr_t>(E));
+        const uint64_t p = Ec[E_P];
+        be.f.set(p, Ec[E_AUX], Ec[E_ONE]);
+        const uint64_t mg = Ec[E_MG];
         be.m = (uint32_t)mg;
         be.ng = (uint32_t)(mg >> 32);
-        const uint64_t ssrc = E[E_SSRC];
+        const uint64_t ssrc = Ec[E_SSRC];
         be.s = (uint32_t)ssrc & 0xFFu;
-        be.meta = (uint32_t)E[E_META];
+        be.meta = (uint32_t)Ec[E_META];
         be.qp = be.meta & 0xFFu;
-        be.nb = E[E_NB];
-        be.klim = E[E_KLIM];
-        be.t = F::from_bits(E[E_T]);
-        be.B64 = F::from_bits(E[E_B64]);
+        be.nb = Ec[E_NB];
+        be.klim = Ec[E_KLIM];
+        be.t = F::from_bits(Ec[E_T]);
+        be.B64 = F::from_bits(Ec[E_B64]);
         const F& f = be.f;
 
         // the entry uses as many buckets as its m needs, at least 16 and at most the launch's table:
@@ -985,21 +1009,6 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         const uint32_t nbe = (uint32_t)max(4, min((int)tg.nbbits, nbe_m));
         const uint32_t nbk_e = 1u << nbe;
         Tb.omask = (nbk_e - 1u) << 4;
-        uint4* fp4 = reinterpret_cast<uint4*>(Tb.fp);
-        if (nbk_e >= L) {  // both powers of two: every lane clears nbk_e / L buckets
-            for (uint32_t k = 0; k < nbk_e / L; ++k) {
-                fp4[tid + k * L] = make_uint4(~0u, ~0u, ~0u, ~0u);
-                Tb.cnt[tid + k * L] = 0u;
-            }
-        } else if (tid < nbk_e) {
-            fp4[tid] = make_uint4(~0u, ~0u, ~0u, ~0u);
-            Tb.cnt[tid] = 0u;
-        }
-        if (tid == 0) {
-            hits.n = 0;
-            hits.best = ~0ull;
-        }
-        __syncthreads();
 
         // baby steps t * B^i, i == tid (mod L); a baby value equal to t for some 0 < i < m means
         // the order of B is below m: the giant steps might not reach the smallest solution, and
@@ -1041,11 +1050,11 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         } else {
             // giant steps G_j * (B^m)^g, g == tid (mod L), g < ng
             T mstep;
-            const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(E[E_BM64]), E_BML, tid, mstep);
+            const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(Ec[E_BM64]), E_BML, tid, mstep);
             const bool gfold = (be.meta >> 12) & 1u;  // the giant tables hold G * (B^m)^k
             const uint32_t rounds = be.ng / L, rem = be.ng - rounds * L;
             for (uint32_t j = 0; j < be.s; ++j) {
-                T v = gfold ? Mt : f.mul(F::from_bits(E[E_G + j]), Mt);
+                T v = gfold ? Mt : f.mul(F::from_bits(Ec[E_G + j]), Mt);
                 auto giant = [&](uint32_t g) {
                     uint32_t fp, bo;
                     hash_word(f.hash_word(v), Tb.omask, fp, bo);
@@ -1067,7 +1076,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                 const bool qmode = (be.meta >> 10) & 1u;
                 const uint64_t stride = qmode ? 1 : ((be.meta >> 8) & 1u) + 1;
                 const uint64_t parity = qmode ? 0 : (be.meta >> 9) & 1u;
-                const uint64_t cmul = qmode ? be.qp : ((be.meta >> 11) & 1u) ? E[E_DJ] : item_cm;
+                const uint64_t cmul = qmode ? be.qp : ((be.meta >> 11) & 1u) ? Ec[E_DJ] : item_cm;
                 // the baby steps again: every baby value t * B^i equal to a hit's giant value gives
                 // the solution k = g*m - i
                 T bstep;
@@ -1098,6 +1107,17 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                 if (tid == 0 && hits.best != ~0ull)
                     put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, hits.best);
             }
+        }
+        // the buckets this entry used are emptied (the giant steps, the last readers of the table,
+        // ended at a barrier)
+        if (nbk_e >= L) {  // both powers of two: every lane clears nbk_e / L buckets
+            for (uint32_t k = 0; k < nbk_e / L; ++k) {
+                fp4[tid + k * L] = make_uint4(~0u, ~0u, ~0u, ~0u);
+                Tb.cnt[tid + k * L] = 0u;
+            }
+        } else if (tid < nbk_e) {
+            fp4[tid] = make_uint4(~0u, ~0u, ~0u, ~0u);
+            Tb.cnt[tid] = 0u;
         }
         asm volatile("" ::"v"(pf));
         __syncthreads();
@@ -1198,7 +1218,7 @@ TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load) {
 }
 
 // fingerprints (16 B) + a fill counter (4 B) per bucket
-size_t bsgs_lds_bytes(const TableGeom& tg) { return ((size_t)20 << tg.nbbits) + sizeof(HitList); }
+size_t bsgs_lds_bytes(const TableGeom& tg) { return ((size_t)20 << tg.nbbits) + 2 * sizeof(HitList); }
 
 // n_ent: entries, one workgroup each (0 = a persistent grid of `grid` workgroups that drains the
 // pass's entries).
