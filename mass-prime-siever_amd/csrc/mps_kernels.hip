@@ -560,11 +560,14 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
     typedef typename F::T T;
     __shared__ uint32_t sh[5];
     // class and order mode: each tile of kSetupTile candidates is processed in the order of a cost
-    // key (the size of g), so the lanes of a wavefront run exponent loops of like length
+    // key, so the lanes of a wavefront run loops of like length: in class mode the size of g; in
+    // order mode the set of odd primes of Qo that divide g (a wavefront runs the Pohlig-Hellman step
+    // of every prime that one of its lanes has)
     constexpr bool SORT = MODE != kModePlain;
     constexpr uint32_t kTile = SORT ? kSetupTile : 256;
+    constexpr uint32_t kBins = MODE == kModeOrder ? 64 : 32;
     __shared__ uint16_t s_order[SORT ? kSetupTile : 1];
-    __shared__ uint32_t s_bin[32];
+    __shared__ uint32_t s_bin[kBins];
     __shared__ uint64_t s_gexp[MODE == kModeOrder ? kSetupTile : 1];  // order mode: exponents of g
     __shared__ uint32_t s_g[MODE == kModeOrder ? kSetupTile : 1];
     const uint32_t n = min(pb.cnt->n_cand, pb.cand_cap);
@@ -573,12 +576,12 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
     uint32_t fermat_fail = 0;
     for (uint32_t t0 = blockIdx.x * kTile; t0 < n; t0 += gridDim.x * kTile) {
         if (SORT) {
-            if (threadIdx.x < 32) s_bin[threadIdx.x] = 0;
+            if (threadIdx.x < kBins) s_bin[threadIdx.x] = 0;
             __syncthreads();
             uint32_t keys[kTile / 256];
             for (uint32_t q = 0; q < kTile / 256; ++q) {
                 const uint32_t i = t0 + q * 256 + threadIdx.x;
-                uint32_t key = 31;  // past the end: last
+                uint32_t key = kBins - 1;  // past the end: last
                 if (i < n) {
                     const uint32_t ix = (uint32_t)pb.cand[i];
                     const uint64_t pk = pb.vals ? pb.vals[ix] : lo + 2 * (uint64_t)ix;
@@ -587,7 +590,9 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                         const uint32_t gv = order_g(pk, ip, gx);
                         s_gexp[q * 256 + threadIdx.x] = gx;
                         s_g[q * 256 + threadIdx.x] = gv;
-                        key = min(32u - __builtin_clz(gv), 30u);  // bits of g
+                        // bit j - 1 for the primes j = 1..6 of Qo (3, 5, ..., 17) in g; 2 always is
+                        key = 0;
+                        for (uint32_t j = 1; j <= 6; ++j) key |= (((gx >> (4 * j)) & 15u) != 0 ? 1u : 0u) << (j - 1);
                     } else {
                         ClassSplit cs;
                         class_split(pk, ip, cs);
@@ -600,7 +605,7 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
             __syncthreads();
             if (threadIdx.x == 0) {
                 uint32_t tot = 0;
-                for (int b = 0; b < 32; ++b) {
+                for (uint32_t b = 0; b < kBins; ++b) {
                     const uint32_t c = s_bin[b];
                     s_bin[b] = tot;
                     tot += c;
