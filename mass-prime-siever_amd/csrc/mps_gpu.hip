@@ -385,8 +385,9 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
         g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
         return true;
     }
-    // class and order mode sort tiles of kSetupTile candidates, a workgroup a tile at a time
-    const uint32_t setup_grid = blocks_for(g, pc, ip.qmode == kModePlain ? 256 : kSetupTile);
+    // class mode sorts tiles of kSetupTile candidates, a workgroup a tile at a time; order mode runs
+    // rounds of 256
+    const uint32_t setup_grid = blocks_for(g, pc, ip.qmode == kModeClass ? kSetupTile : 256);
     launch_setup(s, fk, ip.qmode, d, g->pb, setup_grid);
     HIPCHK(hipGetLastError(), "setup launch");
     HIPCHK(hipEventRecord(g->ev[3], s), "event");

@@ -12,6 +12,7 @@
 //             G_j*(B^m)^g looked up; a hit appends p to the flagged list.
 // Numbers the device cannot decide (p < 1000, p | b) are appended to the flagged list directly.
 #include <algorithm>
+#include <type_traits>
 
 #include "mps_device.hpp"
 #include "mps_kernels.h"
@@ -396,13 +397,11 @@ __device__ __forceinline__ uint64_t exact_div_odd(uint64_t x, uint64_t d) {
 // Order mode: with g = gcd(Qo, p - 1) and e = (p - 1) / g, b^n == t forces (b^e)^n == t^e in the
 // group of g-th roots of unity.  For each prime power r^a of g (Pohlig-Hellman), x = (b^e)^(g/r^a)
 // and y = (t^e)^(g/r^a) give n == k_r (mod d_r), d_r = ord(x), or no solution when y is not a
-// power of x.  With n == cres (mod cm) the admissible n form one class n == n* (mod L); returns its
-// first member n_start >= N0 and count K below N1 (false: the prime has no solution).  All moduli
-// divide Qo < 2^32; only the exponent e needs 64-bit arithmetic.  g and its exponents (gexp, 4
-// bits per prime of Qo) come from order_g.
-// With need_prime, p is first put to a Fermat test to base b, b^(p-1) = (b^e)^g == 1, which costs
-// the bits of g instead of those of p (b mod p != 0: the filter hands p | b to the host); a number
-// that fails it sets composite.
+// power of x (order_prime).  With n == cres (mod cm) the admissible n form one class
+// n == n* (mod L), of which order_finish gives the first member n_start >= N0 and the count K below
+// N1.  All moduli divide Qo < 2^32; only the exponent e needs 64-bit arithmetic.  The Fermat test to
+// base b, b^(p-1) = (b^e)^g == 1, is part of the step for r = 2 (b mod p != 0: the filter hands
+// p | b to the host).
 // g = gcd(Qo, p - 1) from the valuations of p - 1 at the primes of Qo (prime powers in Qo are
 // below 2^16); returns g and, 4 bits per prime, the exponent of the j-th prime in g.  Odd primes
 // divide by multiplying with their inverse modulo 2^64 (no division instruction).
@@ -459,82 +458,111 @@ __device__ __forceinline__ uint64_t divmod52(uint64_t a, uint32_t m, uint64_t& r
     return q;
 }
 
+// Order mode, one round of 256 candidates (one per thread) as work lists: phase 1, a lane per
+// candidate, computes g, b^e and t^e; phase 2, a lane per (candidate, prime of g) pair with the
+// pairs binned by prime and exponent, runs that prime's Pohlig-Hellman step; order_finish, a lane
+// per candidate again, adds the primes up by the CRT.  A wavefront of pairs shares one prime and
+// one search bound, where a wavefront of candidates ran the step of every prime any of its lanes
+// had, for as long as its costliest lane needed.
+constexpr uint32_t kOrderRound = 256;
+constexpr uint32_t kOrderBins = 64;  // pair bins j * 6 + min(exponent, 5), j < 10
+
 template <class F>
-__device__ __forceinline__ bool order_class(const F& f, uint64_t p, typename F::T bm, typename F::T tm,
-                                            const ItemParams& ip, uint32_t g, uint64_t gexp, bool need_prime,
-                                            uint64_t& L, uint64_t& nst, uint64_t& K, unsigned long long& work,
-                                            unsigned long long& fermat_work, bool& composite) {
+struct OrderRound {
+    uint64_t p[kOrderRound], aux[kOrderRound];     // the candidates' fields
+    uint64_t one[F::kKeepsOne ? kOrderRound : 1];  // (the Montgomery field's 1)
+    uint64_t Be[kOrderRound], Te[kOrderRound];     // b^e and t^e, e = (p - 1) / g
+    uint64_t gexp[kOrderRound];                    // exponent of each prime of Qo in g, 4 bits each
+    unsigned long long acc[kOrderRound];            // CRT sum over the primes
+    unsigned long long lx[kOrderRound];             // exponent of each prime in L, 4 bits each
+    uint32_t g[kOrderRound], fail[kOrderRound];
+    uint16_t pair[kOrderRound * 10];                // candidate | prime index << 8
+    uint32_t bin[kOrderBins];
+    uint32_t npair;
+};
+
+// Phase 2 for the pair (candidate c, j-th prime r of Qo, r^a exactly dividing g): with h = g / r^a,
+// x = (b^e)^h and y = (t^e)^h give n == k (mod d), d = ord(x), k = log_x(y); with n == cres modulo
+// the r-part of cm the finer congruence holds, and its residue and the exponent of r in its modulus
+// go to the candidate's CRT sum and exponent word.  For a prime x^(r^a) == b^(p-1) == 1, so d
+// divides r^a; a number where that fails is composite and is dropped.  For r = 2, which divides
+// every g, this is also the Fermat test to base b: x^(r^a) == (b^e)^g.
+template <class F>
+__device__ __forceinline__ void order_prime(OrderRound<F>& R, uint32_t c, uint32_t j, const ItemParams& ip,
+                                            unsigned long long& work, uint32_t& composite) {
     typedef typename F::T T;
-    const uint32_t tz = __builtin_ctz(g);
-    const uint64_t e = exact_div_odd((p - 1) >> tz, g >> tz);
-    T Be, Te;
-    fpow2(f, bm, tm, e, Be, Te);
-    work += 2 * (64 - __builtin_clzll(e));
-    const uint64_t k1 = f.key(f.one());
-    if (need_prime) {
-        fermat_work += 2 * (32 - __builtin_clz(g));
-        if (f.key(fpow(f, Be, g)) != k1) {
-            composite = true;
-            return false;
+    F f;
+    f.set(R.p[c], R.aux[c], F::kKeepsOne ? R.one[c] : 0);
+    const uint32_t g = R.g[c], ej = (uint32_t)(R.gexp[c] >> (4 * j)) & 15u;
+    const uint32_t r = ip.opr[j], rinv = (uint32_t)ip.oinv[j];
+    // ra = r^ej and h = g / ra (exact), by multiplication only
+    uint32_t ra = 1, h = g;
+    for (uint32_t k = 0; k < ej; ++k) {
+        ra *= r;
+        h = r == 2 ? h >> 1 : h * rinv;
+    }
+    T x, y;
+    fpow2(f, F::from_bits(R.Be[c]), F::from_bits(R.Te[c]), h, x, y);
+    work += 2 * (32 - __builtin_clz(h));
+    const uint64_t k1 = f.key(f.one()), ky = f.key(y);
+    T xk = f.one();
+    uint32_t kd = ~0u, dr = 0;
+    for (uint32_t k = 0; k <= ra; ++k) {
+        const uint64_t kx = f.key(xk);
+        if (k > 0 && kx == k1) {
+            dr = k;
+            break;
+        }
+        if (kd == ~0u && kx == ky) kd = k;
+        xk = f.mul(xk, x);
+    }
+    work += dr;
+    // dr must be a power of r: divide ra by r down to dr, counting the exponent
+    uint32_t q = ra, kq = ej;
+    while (dr != 0 && q > dr) {
+        q = r == 2 ? q >> 1 : q * rinv;
+        --kq;
+    }
+    bool ok = dr != 0 && q == dr;
+    if (!ok && j == 0) ++composite;  // (b^e)^g != 1: fails the Fermat test to base b
+    ok = ok && kd != ~0u;             // else -c is not a power of b modulo p
+    uint32_t cj = 0, kl = 0;
+    if (ok) {
+        const uint32_t cmj = ip.ocm[j], crj = ip.ocres[j];
+        if (dr >= cmj) {
+            ok = mod16(kd, cmj) == crj;
+            cj = kd;
+            kl = kq;
+        } else {
+            ok = mod16(crj, dr) == kd;
+            cj = crj;
+            for (uint32_t v = cmj; v > 1; v = r == 2 ? v >> 1 : v * rinv) ++kl;
         }
     }
-    // per prime r of Qo: n == k_r (mod d_r) from the discrete log, and n == cres (mod cm_r); both
-    // moduli are powers of r, so they agree or there is no solution, and the finer one holds.
-    // The CRT basis of Qo then gives n* modulo L = lcm(prod d_r, cm) without an inverse.
-    uint64_t acc = 0;
+    if (!ok) {
+        R.fail[c] = 1u;  // every writer stores the same value
+    } else {
+        atomicAdd(&R.acc[c], (unsigned long long)cj * ip.ocrt[j]);
+        atomicOr(&R.lx[c], (unsigned long long)kl << (4 * j));
+    }
+}
+
+// The class of candidate c from its primes' results: n == n* (mod L); false when a prime failed or
+// no n lies in [N0, N1).
+template <class F>
+__device__ __forceinline__ bool order_finish(const OrderRound<F>& R, uint32_t c, const ItemParams& ip, uint64_t& L,
+                                             uint64_t& nst, uint64_t& K) {
+    if (R.fail[c]) return false;
+    uint64_t acc = R.acc[c];
+    const uint64_t gx = R.gexp[c], lx = R.lx[c];
     uint32_t Lw = 1;
     for (uint32_t j = 0; j < ip.nopr; ++j) {
-        const uint32_t ej = (uint32_t)(gexp >> (4 * j)) & 15u;
-        uint32_t dr = 1, kf = 0;
-        if (ej != 0) {
-            // ra = r^ej and h = g / ra (exact: r^ej divides g), by multiplication only
-            const uint32_t r = ip.opr[j], rinv = (uint32_t)ip.oinv[j];
-            uint32_t ra = 1, h = g;
-            for (uint32_t k = 0; k < ej; ++k) {
-                ra *= r;
-                h = r == 2 ? h >> 1 : h * rinv;
-            }
-            T x, y;
-            fpow2(f, Be, Te, h, x, y);
-            work += 2 * (32 - __builtin_clz(h));
-            // the order dr of x is the first k >= 1 with x^k == 1; for a prime p, x^ra == b^(p-1) == 1,
-            // so dr divides ra.  A number where that fails is composite (a Fermat pseudoprime that
-            // reached setup) and has no relation: it is dropped.
-            const uint64_t ky = f.key(y);
-            T xk = f.one();
-            uint32_t kd = ~0u;
-            dr = 0;
-            for (uint32_t k = 0; k <= ra; ++k) {
-                const uint64_t kx = f.key(xk);
-                if (k > 0 && kx == k1) {
-                    dr = k;
-                    break;
-                }
-                if (kd == ~0u && kx == ky) kd = k;
-                xk = f.mul(xk, x);
-            }
-            work += dr;
-            if (dr == 0) return false;  // composite
-            // dr must be a power of r (it divides ra): divide ra by r down to dr
-            uint32_t q = ra;
-            while (q > dr) q = r == 2 ? q >> 1 : q * rinv;
-            if (q != dr) return false;  // composite
-            if (kd == ~0u) return false;  // -c is not a power of b modulo p
-            kf = kd;
+        if ((gx >> (4 * j)) & 15u) {
+            for (uint32_t k = (uint32_t)(lx >> (4 * j)) & 15u; k > 0; --k) Lw *= ip.opr[j];
+        } else if (ip.ocm[j] > 1) {  // r does not divide g: n == cres modulo the r-part of cm alone
+            acc += (uint64_t)ip.ocres[j] * ip.ocrt[j];
+            Lw *= ip.ocm[j];
         }
-        const uint32_t cmj = ip.ocm[j], crj = ip.ocres[j];
-        uint32_t cj, lj;
-        if (dr >= cmj) {
-            if (mod16(kf, cmj) != crj) return false;
-            cj = kf;
-            lj = dr;
-        } else {
-            if (mod16(crj, dr) != kf) return false;
-            cj = crj;
-            lj = cmj;
-        }
-        acc += (uint64_t)cj * ip.ocrt[j];
-        Lw *= lj;
     }
     uint64_t nstar, n0r;
     divmod52(acc, Lw, nstar);
@@ -559,17 +587,16 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
     const ItemParams& ip = d->ip;  // read in place: uniform fields load as scalars, tables by index
     typedef typename F::T T;
     __shared__ uint32_t sh[5];
-    // class and order mode: each tile of kSetupTile candidates is processed in the order of a cost
-    // key, so the lanes of a wavefront run loops of like length: in class mode the size of g; in
-    // order mode the set of odd primes of Qo that divide g (a wavefront runs the Pohlig-Hellman step
-    // of every prime that one of its lanes has)
-    constexpr bool SORT = MODE != kModePlain;
+    // class mode: each tile of kSetupTile candidates is processed in the order of the size of g, so
+    // the lanes of a wavefront run loops of like length; order mode runs its rounds as work lists
+    constexpr bool SORT = MODE == kModeClass;
+    constexpr bool OMODE = MODE == kModeOrder;
     constexpr uint32_t kTile = SORT ? kSetupTile : 256;
-    constexpr uint32_t kBins = MODE == kModeOrder ? 64 : 32;
+    constexpr uint32_t kBins = 32;
     __shared__ uint16_t s_order[SORT ? kSetupTile : 1];
     __shared__ uint32_t s_bin[kBins];
-    __shared__ uint64_t s_gexp[MODE == kModeOrder ? kSetupTile : 1];  // order mode: exponents of g
-    __shared__ uint32_t s_g[MODE == kModeOrder ? kSetupTile : 1];
+    typedef typename std::conditional<OMODE, OrderRound<F>, uint32_t>::type RoundStore;
+    __shared__ RoundStore s_round;
     const uint32_t n = min(pb.cnt->n_cand, pb.cand_cap);
     const uint32_t lane = lane_id();
     unsigned long long w_fermat = 0, w_setup = 0, w_bsgs = 0;
@@ -585,19 +612,9 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                 if (i < n) {
                     const uint32_t ix = (uint32_t)pb.cand[i];
                     const uint64_t pk = pb.vals ? pb.vals[ix] : lo + 2 * (uint64_t)ix;
-                    if (MODE == kModeOrder) {
-                        uint64_t gx;
-                        const uint32_t gv = order_g(pk, ip, gx);
-                        s_gexp[q * 256 + threadIdx.x] = gx;
-                        s_g[q * 256 + threadIdx.x] = gv;
-                        // bit j - 1 for the primes j = 1..6 of Qo (3, 5, ..., 17) in g; 2 always is
-                        key = 0;
-                        for (uint32_t j = 1; j <= 6; ++j) key |= (((gx >> (4 * j)) & 15u) != 0 ? 1u : 0u) << (j - 1);
-                    } else {
-                        ClassSplit cs;
-                        class_split(pk, ip, cs);
-                        key = min(cs.g >> 1, 30u);
-                    }
+                    ClassSplit cs;
+                    class_split(pk, ip, cs);
+                    key = min(cs.g >> 1, 30u);
                 }
                 keys[q] = key;
                 atomicAdd(&s_bin[key], 1u);
@@ -615,6 +632,69 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
             for (uint32_t q = 0; q < kTile / 256; ++q) s_order[atomicAdd(&s_bin[keys[q]], 1u)] = (uint16_t)(q * 256 + threadIdx.x);
             __syncthreads();
         }
+        if constexpr (OMODE) {
+            OrderRound<F>& R = s_round;
+            const uint32_t c = threadIdx.x, i = t0 + c;
+            const bool live = i < n;
+            uint64_t gx = 0;
+            if (c < kOrderBins) R.bin[c] = 0;
+            // phase 1: g, b^e and t^e of each candidate
+            if (live) {
+                const uint32_t ix = (uint32_t)pb.cand[i];
+                const uint64_t pc = pb.vals ? pb.vals[ix] : lo + 2 * (uint64_t)ix;
+                F f;
+                f.init(pc);
+                const uint32_t g = order_g(pc, ip, gx);
+                const uint64_t bmod = mod_signed(ip.b, pc), tv = mod_signed(-ip.c, pc);
+                const T rr = f.r2();
+                const uint32_t tz = __builtin_ctz(g);
+                const uint64_t e = exact_div_odd((pc - 1) >> tz, g >> tz);
+                T Be, Te;
+                fpow2(f, f.from(bmod, rr), f.from(tv, rr), e, Be, Te);
+                w_setup += 2 * bitlen64(e);
+                R.p[c] = pc;
+                R.aux[c] = f.aux();
+                if (F::kKeepsOne) R.one[c] = F::to_bits(f.one());
+                R.Be[c] = F::to_bits(Be);
+                R.Te[c] = F::to_bits(Te);
+                R.g[c] = g;
+                R.gexp[c] = gx;
+            }
+            R.fail[c] = live ? 0u : 1u;
+            R.acc[c] = 0;
+            R.lx[c] = 0;
+            __syncthreads();
+            // the (candidate, prime) pairs, binned by prime and exponent (a counting sort)
+            if (live)
+                for (uint32_t j = 0; j < ip.nopr; ++j) {
+                    const uint32_t ej = (uint32_t)(gx >> (4 * j)) & 15u;
+                    if (ej != 0) atomicAdd(&R.bin[j * 6 + min(ej, 5u)], 1u);
+                }
+            __syncthreads();
+            if (c == 0) {
+                uint32_t tot = 0;
+                for (uint32_t b = 0; b < kOrderBins; ++b) {
+                    const uint32_t v = R.bin[b];
+                    R.bin[b] = tot;
+                    tot += v;
+                }
+                R.npair = tot;
+            }
+            __syncthreads();
+            if (live)
+                for (uint32_t j = 0; j < ip.nopr; ++j) {
+                    const uint32_t ej = (uint32_t)(gx >> (4 * j)) & 15u;
+                    if (ej != 0) R.pair[atomicAdd(&R.bin[j * 6 + min(ej, 5u)], 1u)] = (uint16_t)(c | (j << 8));
+                }
+            __syncthreads();
+            // phase 2: a lane per pair
+            const uint32_t np = R.npair;
+            for (uint32_t k = c; k < np; k += 256) {
+                const uint32_t pr = R.pair[k];
+                order_prime(R, pr & 255u, pr >> 8, ip, w_setup, fermat_fail);
+            }
+            __syncthreads();
+        }
         for (uint32_t q = 0; q < kTile / 256; ++q) {
             const uint32_t li = SORT ? (uint32_t)s_order[q * 256 + threadIdx.x] : q * 256 + threadIdx.x;
             const uint32_t i = t0 + li;
@@ -629,8 +709,11 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                 idx = (uint32_t)cd;
                 info = (uint32_t)(cd >> 32);
                 p = pb.vals ? pb.vals[idx] : lo + 2 * (uint64_t)idx;
-                f.init(p);
-                // order mode runs its Fermat test inside order_class, on the power b^e it needs anyway
+                if constexpr (OMODE)
+                    f.set(p, s_round.aux[li], F::kKeepsOne ? s_round.one[li] : 0);
+                else
+                    f.init(p);
+                // order mode runs its Fermat test inside order_prime, on the power b^e it needs anyway
                 const bool fermat_here = pb.need_prime && MODE != kModeOrder;
                 if (fermat_here) w_fermat += bitlen64(p - 1);
                 if (fermat_here && !fermat2(f, p)) {
@@ -639,7 +722,7 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                     const uint64_t bmod = mod_signed(ip.b, p), t = mod_signed(-ip.c, p);
                     const T rr = f.r2();
                     const T bm = f.from(bmod, rr), tm = f.from(t, rr);
-                    if (QMODE) {
+                    if constexpr (QMODE) {
                         class_split(p, ip, cs);
                         mask = info;
                         if (cs.g > 2) {
@@ -670,13 +753,9 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                             ng = sz.y;
                             keep = true;
                         }
-                    } else if (MODE == kModeOrder) {
+                    } else if constexpr (OMODE) {
                         uint64_t K = 0;
-                        bool composite = false;
-                        const bool found = order_class(f, p, bm, tm, ip, s_g[li], s_gexp[li], pb.need_prime, oL, onst,
-                                                       K, w_setup, w_fermat, composite);
-                        fermat_fail += composite ? 1u : 0u;
-                        if (found) {
+                        if (order_finish(s_round, li, ip, oL, onst, K)) {
                             jc = K < p ? K : p;  // exponents repeat with period dividing p - 1
                             m = isqrt_ceil64(jc);
                             m = (m + ip.lanes - 1) / ip.lanes * ip.lanes;  // whole rounds of the workgroup
@@ -798,7 +877,7 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
                 w_bsgs += m + (uint64_t)s * ng;
             }
         }
-        if (SORT) __syncthreads();  // s_order and s_bin are rewritten by the next tile
+        if (SORT || OMODE) __syncthreads();  // the tile's LDS is rewritten by the next tile
     }
     // one atomic per counter and workgroup
     __shared__ unsigned long long red[4][4];
