@@ -478,7 +478,7 @@ struct OrderRound {
     uint32_t g[kOrderRound], fail[kOrderRound];
     uint16_t pair[kOrderRound * 10];                // candidate | prime index << 8
     uint32_t bin[kOrderBins];
-    uint32_t npair;
+    uint32_t npair, nkeep, base;
 };
 
 // Phase 2 for the pair (candidate c, j-th prime r of Qo, r^a exactly dividing g): with h = g / r^a,
@@ -580,6 +580,23 @@ __device__ __forceinline__ bool order_finish(const OrderRound<F>& R, uint32_t c,
     return true;
 }
 
+// The entry's step tables from B (baby base), G1 (giant start with one class, else 1), t and m:
+// Bm = B^m, the start tables (baby starts t * B^k, giant starts G1 * Bm^k) and the lane strides.
+template <class F>
+__device__ __forceinline__ void write_step_tables(const F& f, uint64_t* E, typename F::T B, typename F::T G1,
+                                                  typename F::T tm, uint64_t m, uint32_t lanes,
+                                                  unsigned long long& work) {
+    typedef typename F::T T;
+    const T Bm = fpow(f, B, m);
+    work += 2 * bitlen64(m);
+    const T B64 = write_pow_tables(f, B, tm, E + E_BLO, work);
+    const T Bm64 = write_pow_tables(f, Bm, G1, E + E_MLO, work);
+    E[E_B64] = F::to_bits(B64);
+    E[E_BM64] = F::to_bits(Bm64);
+    E[E_BL] = F::to_bits(lane_stride_pow(f, B64, lanes));
+    E[E_BML] = F::to_bits(lane_stride_pow(f, Bm64, lanes));
+}
+
 template <class F, int MODE>
 __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassBuffers& pb) {
     constexpr bool QMODE = MODE == kModeClass;
@@ -638,6 +655,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
             const bool live = i < n;
             uint64_t gx = 0;
             if (c < kOrderBins) R.bin[c] = 0;
+            if (c == 0) R.nkeep = 0;
             // phase 1: g, b^e and t^e of each candidate
             if (live) {
                 const uint32_t ix = (uint32_t)pb.cand[i];
@@ -694,14 +712,73 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 order_prime(R, pr & 255u, pr >> 8, ip, w_setup, fermat_fail);
             }
             __syncthreads();
+            // phase 3: the class of each candidate; the kept ones are listed (in the pair array, free
+            // now) with their sizes in the round's arrays
+            {
+                uint64_t oL = 0, onst = 0, K = 0;
+                if (live && order_finish(R, c, ip, oL, onst, K)) {
+                    const uint64_t pc = R.p[c];
+                    const uint64_t cd = pb.cand[i];
+                    const uint64_t jc = K < pc ? K : pc;  // exponents repeat with period dividing p - 1
+                    uint64_t m = isqrt_ceil64(jc);
+                    m = (m + ip.lanes - 1) / ip.lanes * ip.lanes;  // whole rounds of the workgroup
+                    if (m > ip.mcap) m = ip.mcap;
+                    if (m == 0) m = 1;
+                    const uint64_t ng = (jc + m - 1) / m + 1;
+                    R.Be[c] = jc;
+                    R.Te[c] = m | (ng << 32);
+                    R.acc[c] = oL;
+                    R.lx[c] = onst;
+                    R.g[c] = (uint32_t)cd;           // index
+                    R.fail[c] = (uint32_t)(cd >> 32);  // info
+                    R.pair[atomicAdd(&R.nkeep, 1u)] = (uint16_t)c;
+                }
+            }
+            __syncthreads();
+            if (c == 0) R.base = R.nkeep ? atomicAdd(&pb.cnt->n_ent, R.nkeep) : 0u;  // one reservation a round
+            __syncthreads();
+            // phase 4: the entries, a lane per kept candidate (a quarter of the candidates or so: a
+            // lane per candidate left most lanes idle through the entry's hundred-odd products)
+            const uint32_t nk = R.nkeep, base = R.base;
+            for (uint32_t k = c; k < nk; k += 256) {
+                const uint32_t cc = R.pair[k], slot = base + k;
+                const uint64_t pc = R.p[cc];
+                const uint32_t idx = R.g[cc], info = R.fail[cc];
+                if (slot >= pb.ent_cap) {  // no room: the host decides this one
+                    put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? idx : pc, kHostDecides);
+                    continue;
+                }
+                F f;
+                f.set(pc, R.aux[cc], F::kKeepsOne ? R.one[cc] : 0);
+                const uint64_t jc = R.Be[cc], mg = R.Te[cc], oL = R.acc[cc], onst = R.lx[cc];
+                const uint64_t m = (uint32_t)mg;
+                const T rr = f.r2();
+                const T bm = f.from(mod_signed(ip.b, pc), rr), tm = f.from(mod_signed(-ip.c, pc), rr);
+                uint64_t* E = pb.ent + (uint64_t)slot * kEntWords;
+                E[E_P] = pc;
+                E[E_AUX] = f.aux();
+                E[E_ONE] = F::to_bits(f.one());
+                E[E_T] = F::to_bits(tm);
+                E[E_MG] = mg;
+                E[E_SSRC] = 1u | ((uint64_t)(info & 0xFFFFFFu) << 8) | ((uint64_t)idx << 32);
+                T B, G1;
+                fpow_2e(f, bm, oL, onst, B, G1);
+                w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
+                E[E_G] = F::to_bits(G1);
+                E[E_NB] = onst;
+                E[E_META] = (1u << 11) | (1u << 12);
+                E[E_KLIM] = jc;
+                E[E_DJ] = oL;
+                write_step_tables(f, E, B, G1, tm, m, ip.lanes, w_setup);
+                w_bsgs += m + (mg >> 32);
+            }
         }
-        for (uint32_t q = 0; q < kTile / 256; ++q) {
+        for (uint32_t q = 0; q < (OMODE ? 0 : kTile / 256); ++q) {
             const uint32_t li = SORT ? (uint32_t)s_order[q * 256 + threadIdx.x] : q * 256 + threadIdx.x;
             const uint32_t i = t0 + li;
             bool keep = false;
             uint64_t p = 0, m = 0, ng = 0, jc = 0;
             uint32_t idx = 0, info = 0, s = 1, mask = 0, dmin = 0;
-            uint64_t oL = 0, onst = 0;  // order mode: exponent modulus and first exponent
             ClassSplit cs;
             F f;
             if (i < n) {
@@ -709,10 +786,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 idx = (uint32_t)cd;
                 info = (uint32_t)(cd >> 32);
                 p = pb.vals ? pb.vals[idx] : lo + 2 * (uint64_t)idx;
-                if constexpr (OMODE)
-                    f.set(p, s_round.aux[li], F::kKeepsOne ? s_round.one[li] : 0);
-                else
-                    f.init(p);
+                f.init(p);
                 // order mode runs its Fermat test inside order_prime, on the power b^e it needs anyway
                 const bool fermat_here = pb.need_prime && MODE != kModeOrder;
                 if (fermat_here) w_fermat += bitlen64(p - 1);
@@ -751,17 +825,6 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                             const uint2 sz = ip.csz[cs.qp * 32 + s];
                             m = sz.x;
                             ng = sz.y;
-                            keep = true;
-                        }
-                    } else if constexpr (OMODE) {
-                        uint64_t K = 0;
-                        if (order_finish(s_round, li, ip, oL, onst, K)) {
-                            jc = K < p ? K : p;  // exponents repeat with period dividing p - 1
-                            m = isqrt_ceil64(jc);
-                            m = (m + ip.lanes - 1) / ip.lanes * ip.lanes;  // whole rounds of the workgroup
-                            if (m > ip.mcap) m = ip.mcap;
-                            if (m == 0) m = 1;
-                            ng = (jc + m - 1) / m + 1;
                             keep = true;
                         }
                     } else {
@@ -841,14 +904,6 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                     E[E_DJ] = djw[0];
                     E[E_DJ + 1] = djw[1];
                     E[E_DJ + 2] = djw[2];
-                } else if (MODE == kModeOrder) {
-                    fpow_2e(f, bm, oL, onst, B, G1);
-                    E[E_G] = F::to_bits(G1);
-                    w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
-                    E[E_NB] = onst;
-                    E[E_META] = 1u << 11;
-                    E[E_KLIM] = jc;
-                    E[E_DJ] = oL;
                 } else {
                     T G0;
                     fpow_2e(f, bm, ip.cm, ip.n_start, B, G0);
@@ -863,17 +918,10 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                     E[E_META] = ((info & 1u) ? (1u << 8) : 0u) | ((info & 2u) ? (1u << 9) : 0u);
                     E[E_KLIM] = jc;
                 }
-                const T Bm = fpow(f, B, m);
-                w_setup += 2 * bitlen64(m);
-                // baby starts t * B^tid; with one class (plain and order mode) the giant starts
-                // G * (B^m)^tid as well (E_META bit 12)
-                const T B64 = write_pow_tables(f, B, tm, E + E_BLO, w_setup);
-                const T Bm64 = write_pow_tables(f, Bm, G1, E + E_MLO, w_setup);
+                // baby starts t * B^tid; with one class (plain mode) the giant starts G * (B^m)^tid
+                // as well (E_META bit 12)
+                write_step_tables(f, E, B, G1, tm, m, ip.lanes, w_setup);
                 if (!QMODE) E[E_META] |= 1u << 12;
-                E[E_B64] = F::to_bits(B64);
-                E[E_BM64] = F::to_bits(Bm64);
-                E[E_BL] = F::to_bits(lane_stride_pow(f, B64, ip.lanes));
-                E[E_BML] = F::to_bits(lane_stride_pow(f, Bm64, ip.lanes));
                 w_bsgs += m + (uint64_t)s * ng;
             }
         }
