@@ -295,7 +295,9 @@ def main():
         t = time.perf_counter()
         table = tune.autotune_items(g, items, log=log)
         tuning_info = {"source": "autotuned live", "keys": len(table), "tune_s": time.perf_counter() - t}
-    elif args.tuning != "none" and os.path.exists(args.tuning):
+    elif args.tuning != "none":
+        if not os.path.exists(args.tuning):  # a missing table would leave the tuned legs untuned
+            raise SystemExit(f"tuning table {args.tuning} not found")
         tuning_info = {"source": os.path.relpath(args.tuning, ROOT), "keys": tune.load(g, args.tuning)}
     log(f"tuning: {tuning_info}")
 
