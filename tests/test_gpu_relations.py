@@ -108,6 +108,32 @@ def test_order_mode_relations_match_oracle(gpu, mps, oracle, it):
     assert st.declined == 0
 
 
+# Class moduli past the primes and prime powers of 367567200 = 2^5 3^3 5^2 7 11 13 17: 38 = 2 * 19
+# gives Qo an eighth prime, 81 = 3^4 a finer 3-part than Qo's own, and 46 = 2 * 23 takes Qo past
+# 2^32 (the item then runs in class mode).  Order mode runs one work-list pair per prime of g.
+ORDER_CM_ITEMS = [
+    (2, -25, 10**6, 10**6 + 2**21, 3_600_000, 5_000_000, 38, 5),
+    (2, -25, 10**6, 10**6 + 2**21, 3_600_000, 5_000_000, 81, 7),
+    (10, 111, 10**9, 10**9 + 2**21, 0, 10_000_000, 38, 4),
+    (3, -2, 10**9, 10**9 + 2**21, 0, 10_000_000, 81, 7),
+    (7, -3, 10**9, 10**9 + 2**21, 0, 10_000_000, 46, 5),
+]
+
+
+@pytest.mark.parametrize("it", ORDER_CM_ITEMS)
+def test_order_mode_wide_class_moduli(gpu, mps, oracle, it):
+    b, c, P0, P1, N0, N1, mod, res = it
+    print(f"order mode, class modulus {mod}: {it}", flush=True)
+    gpu.set_class_policy(2)
+    try:
+        rels, st = gpu.run_item(mps.work_item(b, c, P0, P1, N0, N1, mod, res), chunk_ints=1 << 20)
+    finally:
+        gpu.set_class_policy(1)
+    want, _ = oracle.process_range(oracle.make_item(b, c, P0, P1, N0, N1, mod, res), P0, P1, 1 << 20, 8)
+    assert want and rels == want
+    assert st.declined == 0
+
+
 # base-2 Fermat pseudoprimes near 1e12 (composites that pass the device's first primality test)
 PSEUDOPRIMES = [1000254486421, 1000001633401, 1000133402761]
 
