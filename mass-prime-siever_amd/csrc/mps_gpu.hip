@@ -98,8 +98,18 @@ bool class_mode(const mps_item* it, bool classes_on, int class_policy, uint64_t&
 // modulo Qo = lcm(cm, kOrderQ), when Qo fits 32 bits.
 bool order_mode(const mps_item* it, bool classes_on, int class_policy, uint64_t& Qo) {
     const uint64_t cm = it->cls_mod == 0 ? 1 : it->cls_mod;
-    Qo = cm / gcd_u64(cm, kOrderQ) * kOrderQ;
-    return classes_on && class_policy == 2 && it->N0 >= 0 && it->N1 > it->N0 && Qo < (1ull << 32);
+    auto lcm_cm = [cm](uint64_t b) { return (unsigned __int128)(cm / gcd_u64(cm, b)) * b; };
+    Qo = 0;
+    uint64_t B = kOrderQ;
+    if (lcm_cm(B) >= kOrderQLimit) return false;
+    Qo = (uint64_t)lcm_cm(B);
+    for (uint32_t r : kOrderMorePrimes) {  // the longest prefix that keeps Qo below 2^52
+        B *= r;
+        const unsigned __int128 q = lcm_cm(B);
+        if (q >= kOrderQLimit) break;
+        Qo = (uint64_t)q;
+    }
+    return classes_on && class_policy == 2 && it->N0 >= 0 && it->N1 > it->N0;
 }
 
 JacobiPre jacobi_pre(int64_t x) {
@@ -148,10 +158,11 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
     m_max = 1;
     csz.assign(49 * 32, make_uint2(1, L));
     if (omode) {
-        ip.Qo = (uint32_t)Qo;
-        ip.n0modQo = (uint32_t)(N0 % Qo);
+        ip.Qo = Qo;
+        ip.Qo_inv = 1.0 / (double)Qo;
+        ip.n0modQo = N0 % Qo;
         uint64_t q = Qo;
-        for (uint32_t r = 2; q > 1 && ip.nopr < 10; ++r) {
+        for (uint32_t r = 2; q > 1 && r < 65536 && ip.nopr < kOrderMaxPrimes; ++r) {
             if (q % r) continue;
             uint32_t ra = 1;
             while (q % r == 0) {
@@ -166,14 +177,14 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
             ip.oprpow[ip.nopr++] = ra;
         }
         if (q > 1) {
-            why = "class modulus with more than 10 prime factors";
+            why = "class modulus with a prime factor above 2^16 or too many prime factors";
             return false;
         }
         for (uint32_t j = 0; j < ip.nopr; ++j) {
             const uint64_t qr = ip.oprpow[j], M = Qo / qr;
             uint64_t inv = 1;  // M^-1 mod qr by search (qr < 2^16)
             while ((M % qr) * inv % qr != 1 % qr) ++inv;
-            ip.ocrt[j] = (uint32_t)(M * inv % Qo);
+            ip.ocrt[j] = (uint64_t)((unsigned __int128)M * inv % Qo);
             uint64_t cr = 1;
             while (cm % (cr * ip.opr[j]) == 0) cr *= ip.opr[j];
             ip.ocm[j] = (uint32_t)cr;

@@ -18,8 +18,13 @@ constexpr uint32_t kBsgsOversubscribe = 16;      // batch engine: BSGS grid / re
 constexpr uint32_t kSetupTile = 1024;             // candidates a setup workgroup sorts at a time (class, order mode)
 // Setup modes (ItemParams.qmode).
 constexpr uint32_t kModePlain = 0, kModeClass = 1, kModeOrder = 2;
-// Order mode: the power-residue modulus every p - 1 is split by, 2^5 3^3 5^2 7 11 13 17.
+// Order mode: every p - 1 is split by Qo = lcm(cm, B), B the longest of the prefix products
+// 367567200 = 2^5 3^3 5^2 7 11 13 17, then times 19, 23, 29, 31, 37, with Qo below 2^52 (the CRT
+// then stays exact in doubles).
 constexpr uint64_t kOrderQ = 367567200ull;
+constexpr uint32_t kOrderMorePrimes[5] = {19, 23, 29, 31, 37};
+constexpr uint64_t kOrderQLimit = 1ull << 52;
+constexpr uint32_t kOrderMaxPrimes = 16;          // distinct primes of Qo
 constexpr uint64_t kHostDecides = ~0ull;          // M of a record the host must decide exactly
 
 // Entry layout (u64 words).  B is the baby-step base, Bm = B^m the giant-step base.
@@ -86,15 +91,18 @@ struct ItemParams {
     uint64_t pjc[4];
     // class mode sizes: csz[qp * 32 + s] = (m, giant steps per class)
     const uint2* csz;
-    // order mode: Qo = lcm(cm, kOrderQ), its distinct prime factors and their powers in Qo,
-    // and N0 mod Qo; per prime: the CRT basis element of Qo (1 modulo the prime's power, 0 modulo
+    // order mode: Qo (above), 1 / Qo rounded, N0 mod Qo, the distinct prime factors of Qo and their
+    // powers in Qo; per prime: the CRT basis element of Qo (1 modulo the prime's power, 0 modulo
     // the others), the prime's part of cm and cres modulo that part
-    uint32_t Qo, nopr, n0modQo;
-    uint32_t opr[10], oprpow[10];
-    uint32_t ocrt[10], ocm[10], ocres[10];
+    uint64_t Qo, n0modQo;
+    double Qo_inv;
+    uint32_t nopr;
+    uint32_t opr[kOrderMaxPrimes], oprpow[kOrderMaxPrimes];
+    uint64_t ocrt[kOrderMaxPrimes];
+    uint32_t ocm[kOrderMaxPrimes], ocres[kOrderMaxPrimes];
     // per odd prime r of Qo: r^-1 mod 2^64 and floor((2^64 - 1) / r); x is a multiple of r exactly
     // when x * r^-1 mod 2^64 <= that bound, and the product is then x / r
-    uint64_t oinv[10], olim[10];
+    uint64_t oinv[kOrderMaxPrimes], olim[kOrderMaxPrimes];
 };
 
 // BSGS table geometry of a pass: 2^nbbits buckets of 8 slots; a slot holds a fingerprint of

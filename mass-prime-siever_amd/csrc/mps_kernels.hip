@@ -405,8 +405,8 @@ __device__ __forceinline__ uint64_t exact_div_odd(uint64_t x, uint64_t d) {
 // g = gcd(Qo, p - 1) from the valuations of p - 1 at the primes of Qo (prime powers in Qo are
 // below 2^16); returns g and, 4 bits per prime, the exponent of the j-th prime in g.  Odd primes
 // divide by multiplying with their inverse modulo 2^64 (no division instruction).
-__device__ __forceinline__ uint32_t order_g(uint64_t p, const ItemParams& ip, uint64_t& gexp) {
-    uint32_t g = 1;
+__device__ __forceinline__ uint64_t order_g(uint64_t p, const ItemParams& ip, uint64_t& gexp) {
+    uint64_t g = 1;
     gexp = 0;
     for (uint32_t j = 0; j < ip.nopr; ++j) {
         const uint32_t r = ip.opr[j];
@@ -442,9 +442,9 @@ __device__ __forceinline__ uint32_t mod16(uint32_t a, uint32_t m) {
     return (uint32_t)r;
 }
 
-// a mod m and a / m for a below 2^52 and 1 <= m < 2^32: the double quotient is within one of the
-// true one.
-__device__ __forceinline__ uint64_t divmod52(uint64_t a, uint32_t m, uint64_t& rem) {
+// a mod m and a / m for a and m below 2^52 (m >= 1): both are exact doubles, so the double quotient
+// is within one of the true one.
+__device__ __forceinline__ uint64_t divmod52(uint64_t a, uint64_t m, uint64_t& rem) {
     uint64_t q = (uint64_t)((double)a / (double)m);
     int64_t r = (int64_t)(a - q * m);
     if (r < 0) {
@@ -458,6 +458,16 @@ __device__ __forceinline__ uint64_t divmod52(uint64_t a, uint32_t m, uint64_t& r
     return q;
 }
 
+// a * b mod Qo for a < 2^16 and b < Qo < 2^52: the quotient from doubles (a, b exact; the product
+// and the scaling by 1/Qo round) is within one of the true one, and a*b - q*Qo is exact modulo 2^64.
+__device__ __forceinline__ uint64_t mulmod_qo(uint64_t a, uint64_t b, const ItemParams& ip) {
+    const uint64_t q = (uint64_t)((double)a * (double)b * ip.Qo_inv);
+    int64_t r = (int64_t)(a * b - q * ip.Qo);
+    if (r < 0) r += (int64_t)ip.Qo;
+    if (r >= (int64_t)ip.Qo) r -= (int64_t)ip.Qo;
+    return (uint64_t)r;
+}
+
 // Order mode, one round of 256 candidates (one per thread) as work lists: phase 1, a lane per
 // candidate, computes g, b^e and t^e; phase 2, a lane per (candidate, prime of g) pair with the
 // pairs binned by prime and exponent, runs that prime's Pohlig-Hellman step; order_finish, a lane
@@ -465,7 +475,14 @@ __device__ __forceinline__ uint64_t divmod52(uint64_t a, uint32_t m, uint64_t& r
 // one search bound, where a wavefront of candidates ran the step of every prime any of its lanes
 // had, for as long as its costliest lane needed.
 constexpr uint32_t kOrderRound = 256;
-constexpr uint32_t kOrderBins = 64;  // pair bins j * 6 + min(exponent, 5), j < 10
+// Pair bins j * 6 + min(exponent, 5): a wavefront of pairs shares the prime and the search bound.
+// (Splitting the bins further by the bit length of h, the exponent of the pair's powers, measured
+// slower: c2 setup 1.69 -> 1.83 ms.)
+constexpr uint32_t kOrderBins = 6 * kOrderMaxPrimes;
+// Pair slots of a round: 8 per candidate, where a candidate averages about 2.5 (one pair per prime
+// of Qo dividing p - 1); a round with more (never seen: over 70 standard deviations out) hands
+// all its candidates to the host.
+constexpr uint32_t kOrderPairs = 8 * kOrderRound;
 
 template <class F>
 struct OrderRound {
@@ -475,8 +492,9 @@ struct OrderRound {
     uint64_t gexp[kOrderRound];                    // exponent of each prime of Qo in g, 4 bits each
     unsigned long long acc[kOrderRound];            // CRT sum over the primes
     unsigned long long lx[kOrderRound];             // exponent of each prime in L, 4 bits each
-    uint32_t g[kOrderRound], fail[kOrderRound];
-    uint16_t pair[kOrderRound * 10];                // candidate | prime index << 8
+    uint64_t g[kOrderRound];                        // g; after phase 3 the candidate word
+    uint8_t fail[kOrderRound];
+    uint16_t pair[kOrderPairs];                     // candidate | prime index << 8
     uint32_t bin[kOrderBins];
     uint32_t npair, nkeep, base;
 };
@@ -493,17 +511,18 @@ __device__ __forceinline__ void order_prime(OrderRound<F>& R, uint32_t c, uint32
     typedef typename F::T T;
     F f;
     f.set(R.p[c], R.aux[c], F::kKeepsOne ? R.one[c] : 0);
-    const uint32_t g = R.g[c], ej = (uint32_t)(R.gexp[c] >> (4 * j)) & 15u;
+    const uint32_t ej = (uint32_t)(R.gexp[c] >> (4 * j)) & 15u;
     const uint32_t r = ip.opr[j], rinv = (uint32_t)ip.oinv[j];
     // ra = r^ej and h = g / ra (exact), by multiplication only
-    uint32_t ra = 1, h = g;
+    uint32_t ra = 1;
+    uint64_t h = R.g[c];
     for (uint32_t k = 0; k < ej; ++k) {
         ra *= r;
-        h = r == 2 ? h >> 1 : h * rinv;
+        h = r == 2 ? h >> 1 : h * ip.oinv[j];
     }
     T x, y;
     fpow2(f, F::from_bits(R.Be[c]), F::from_bits(R.Te[c]), h, x, y);
-    work += 2 * (32 - __builtin_clz(h));
+    work += 2 * bitlen64(h);
     const uint64_t k1 = f.key(f.one()), ky = f.key(y);
     T xk = f.one();
     uint32_t kd = ~0u, dr = 0;
@@ -542,7 +561,7 @@ __device__ __forceinline__ void order_prime(OrderRound<F>& R, uint32_t c, uint32
     if (!ok) {
         R.fail[c] = 1u;  // every writer stores the same value
     } else {
-        atomicAdd(&R.acc[c], (unsigned long long)cj * ip.ocrt[j]);
+        atomicAdd(&R.acc[c], (unsigned long long)mulmod_qo(cj, ip.ocrt[j], ip));  // below Qo
         atomicOr(&R.lx[c], (unsigned long long)kl << (4 * j));
     }
 }
@@ -553,21 +572,26 @@ template <class F>
 __device__ __forceinline__ bool order_finish(const OrderRound<F>& R, uint32_t c, const ItemParams& ip, uint64_t& L,
                                              uint64_t& nst, uint64_t& K) {
     if (R.fail[c]) return false;
-    uint64_t acc = R.acc[c];
+    uint64_t acc = R.acc[c];  // a sum of nopr terms below Qo: below 2^56
     const uint64_t gx = R.gexp[c], lx = R.lx[c];
-    uint32_t Lw = 1;
+    uint64_t Lw = 1;
     for (uint32_t j = 0; j < ip.nopr; ++j) {
         if ((gx >> (4 * j)) & 15u) {
             for (uint32_t k = (uint32_t)(lx >> (4 * j)) & 15u; k > 0; --k) Lw *= ip.opr[j];
         } else if (ip.ocm[j] > 1) {  // r does not divide g: n == cres modulo the r-part of cm alone
-            acc += (uint64_t)ip.ocres[j] * ip.ocrt[j];
+            acc += mulmod_qo(ip.ocres[j], ip.ocrt[j], ip);
             Lw *= ip.ocm[j];
         }
     }
+    // acc modulo Qo (the quotient is below 2^5, and within one of the double one), then modulo L,
+    // which divides Qo, from exact doubles
+    int64_t aq = (int64_t)(acc - (uint64_t)((double)acc * ip.Qo_inv) * ip.Qo);
+    while (aq < 0) aq += (int64_t)ip.Qo;
+    while (aq >= (int64_t)ip.Qo) aq -= (int64_t)ip.Qo;
     uint64_t nstar, n0r;
-    divmod52(acc, Lw, nstar);
+    divmod52((uint64_t)aq, Lw, nstar);
     divmod52(ip.n0modQo, Lw, n0r);
-    const uint32_t off = (uint32_t)(nstar >= n0r ? nstar - n0r : nstar + Lw - n0r);
+    const uint64_t off = nstar >= n0r ? nstar - n0r : nstar + Lw - n0r;
     L = Lw;
     nst = ip.N0 + off;
     if (nst >= ip.N1) {
@@ -654,7 +678,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
             const uint32_t c = threadIdx.x, i = t0 + c;
             const bool live = i < n;
             uint64_t gx = 0;
-            if (c < kOrderBins) R.bin[c] = 0;
+            for (uint32_t b = c; b < kOrderBins; b += 256) R.bin[b] = 0;  // more bins than threads
             if (c == 0) R.nkeep = 0;
             // phase 1: g, b^e and t^e of each candidate
             if (live) {
@@ -662,10 +686,10 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 const uint64_t pc = pb.vals ? pb.vals[ix] : lo + 2 * (uint64_t)ix;
                 F f;
                 f.init(pc);
-                const uint32_t g = order_g(pc, ip, gx);
+                const uint64_t g = order_g(pc, ip, gx);
                 const uint64_t bmod = mod_signed(ip.b, pc), tv = mod_signed(-ip.c, pc);
                 const T rr = f.r2();
-                const uint32_t tz = __builtin_ctz(g);
+                const uint32_t tz = __builtin_ctzll(g);
                 const uint64_t e = exact_div_odd((pc - 1) >> tz, g >> tz);
                 T Be, Te;
                 fpow2(f, f.from(bmod, rr), f.from(tv, rr), e, Be, Te);
@@ -678,7 +702,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 R.g[c] = g;
                 R.gexp[c] = gx;
             }
-            R.fail[c] = live ? 0u : 1u;
+            R.fail[c] = live ? 0 : 1;
             R.acc[c] = 0;
             R.lx[c] = 0;
             __syncthreads();
@@ -699,14 +723,19 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 R.npair = tot;
             }
             __syncthreads();
-            if (live)
+            const bool overflow = R.npair > kOrderPairs;  // the same in every lane
+            if (live && !overflow)
                 for (uint32_t j = 0; j < ip.nopr; ++j) {
                     const uint32_t ej = (uint32_t)(gx >> (4 * j)) & 15u;
                     if (ej != 0) R.pair[atomicAdd(&R.bin[j * 6 + min(ej, 5u)], 1u)] = (uint16_t)(c | (j << 8));
                 }
+            if (live && overflow) {  // no room for the round's pairs: the host decides its candidates
+                put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (uint32_t)pb.cand[i] : R.p[c], kHostDecides);
+                R.fail[c] = 1u;
+            }
             __syncthreads();
             // phase 2: a lane per pair
-            const uint32_t np = R.npair;
+            const uint32_t np = overflow ? 0u : R.npair;
             for (uint32_t k = c; k < np; k += 256) {
                 const uint32_t pr = R.pair[k];
                 order_prime(R, pr & 255u, pr >> 8, ip, w_setup, fermat_fail);
@@ -729,8 +758,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                     R.Te[c] = m | (ng << 32);
                     R.acc[c] = oL;
                     R.lx[c] = onst;
-                    R.g[c] = (uint32_t)cd;           // index
-                    R.fail[c] = (uint32_t)(cd >> 32);  // info
+                    R.g[c] = cd;  // index | info << 32
                     R.pair[atomicAdd(&R.nkeep, 1u)] = (uint16_t)c;
                 }
             }
@@ -743,7 +771,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
             for (uint32_t k = c; k < nk; k += 256) {
                 const uint32_t cc = R.pair[k], slot = base + k;
                 const uint64_t pc = R.p[cc];
-                const uint32_t idx = R.g[cc], info = R.fail[cc];
+                const uint32_t idx = (uint32_t)R.g[cc], info = (uint32_t)(R.g[cc] >> 32);
                 if (slot >= pb.ent_cap) {  // no room: the host decides this one
                     put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? idx : pc, kHostDecides);
                     continue;

@@ -103,17 +103,33 @@ def test_entries_match_oracle_decisions(gpu, mps, oracle, it, field, policy):
 
 
 ORDER_Q = 367567200  # 2^5 3^3 5^2 7 11 13 17
+ORDER_MORE = (19, 23, 29, 31, 37)
+
+
+def order_modulus(cm):
+    """Qo = lcm(cm, B) for the longest prefix product B of 367567200, then times 19, 23, 29, 31,
+    37, with Qo below 2^52."""
+    import math
+    lcm = lambda a, b: a // math.gcd(a, b) * b
+    B = ORDER_Q
+    Qo = lcm(cm, B)
+    for r in ORDER_MORE:
+        B *= r
+        if lcm(cm, B) >= 2**52:
+            break
+        Qo = lcm(cm, B)
+    return Qo
 
 
 def order_class(p, b, c, cm, cres, N0, N1):
     """The exponent class the device must find in order mode, restated from the mathematics: with
-    g = gcd(lcm(cm, ORDER_Q), p - 1), every solution n of b^n == -c (mod p) has (b^e)^n == (-c)^e,
+    g = gcd(Qo, p - 1) (order_modulus), every solution n of b^n == -c (mod p) has (b^e)^n == (-c)^e,
     e = (p-1)/g; per prime power of g this fixes n modulo the order of the projected b^e (or rules
     p out), and with n == cres (mod cm) the admissible n are one class mod L.  Returns
     (L, first n >= N0, count below N1) or None."""
     import math
     import sympy
-    Qo = cm // math.gcd(cm, ORDER_Q) * ORDER_Q
+    Qo = order_modulus(cm)
     g = math.gcd(Qo, p - 1)
     e = (p - 1) // g
     Be, Te = pow(b % p, e, p), pow((-c) % p, e, p)
@@ -127,14 +143,14 @@ def order_class(p, b, c, cm, cres, N0, N1):
             return None
         residues.append((ks[0], d))
     residues.append((cres % cm, cm))
-    # brute-force CRT over the small moduli (no shared code with the device's extended Euclid)
-    L = 1
-    for _, d in residues:
-        L = L * d // math.gcd(L, d)
-    sols = [n for n in range(L) if all(n % d == k % d for k, d in residues)]
-    if not sols:
+    # the CRT over moduli that share primes (sympy's general solver; no code shared with the
+    # device's CRT basis of Qo)
+    from sympy.ntheory.modular import solve_congruence
+    sol = solve_congruence(*residues)
+    if sol is None:
         return None
-    first = N0 + (sols[0] - N0) % L
+    n0, L = int(sol[0]), int(sol[1])
+    first = N0 + (n0 - N0) % L
     K = (N1 - 1 - first) // L + 1 if first < N1 else 0
     return (L, first, K) if K > 0 else None
 
