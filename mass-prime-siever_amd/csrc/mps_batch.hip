@@ -366,7 +366,9 @@ int run_chunk(mps_batch* b, Slot& s, const Task& t, std::vector<mps_relation>& r
     dev_s += t1 - t0;
     s.recs.resize(n_out);
     for (uint32_t i = 0; i < n_out; ++i) s.recs[i] = mps_dev_relation{s.h_out[2 * i], s.h_out[2 * i + 1]};
-    const int st = finish_records(&it->w, lo, t.hi, s.recs.data(), n_out, rels, &prime_flagged);
+    // a pass with many relations (small b and c over long exponent ranges: thousands, each with
+    // q = ord_p(b) from a factorisation of p - 1) is finished on a few threads
+    const int st = finish_records_parallel(&it->w, lo, t.hi, s.recs.data(), n_out, rels, &prime_flagged, 4, 512);
     host_s += now_s() - t1;
     return st;
 }

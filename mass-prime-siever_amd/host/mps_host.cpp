@@ -81,6 +81,40 @@ int finish_records(const mps_work_item* it, uint64_t lo, uint64_t hi, mps_dev_re
     return status;
 }
 
+int finish_records_parallel(const mps_work_item* it, uint64_t lo, uint64_t hi, mps_dev_relation* recs, size_t n,
+                            std::vector<mps_relation>& rels, uint64_t* prime_flagged, int threads,
+                            size_t min_records) {
+    const size_t parts = std::min<size_t>(threads > 1 ? (size_t)threads : 1, n / std::max<size_t>(1, min_records / 2));
+    if (parts <= 1 || n < min_records) return finish_records(it, lo, hi, recs, n, rels, prime_flagged);
+    std::sort(recs, recs + n,
+              [](const mps_dev_relation& a, const mps_dev_relation& b) { return a.p < b.p || (a.p == b.p && a.M < b.M); });
+    // range k is [cut[k], cut[k + 1]); a cut never separates records of one number
+    std::vector<size_t> cut(parts + 1, n);
+    cut[0] = 0;
+    for (size_t k = 1; k < parts; ++k) {
+        size_t c = std::max(cut[k - 1], n * k / parts);
+        while (c > 0 && c < n && recs[c].p == recs[c - 1].p) ++c;
+        cut[k] = c;
+    }
+    std::vector<std::vector<mps_relation>> out(parts);
+    std::vector<uint64_t> flagged(parts, 0);
+    std::vector<int> status(parts, MPS_OK);
+    auto run = [&](size_t k) {
+        status[k] = finish_records(it, lo, hi, recs + cut[k], cut[k + 1] - cut[k], out[k], &flagged[k]);
+    };
+    std::vector<std::thread> th;
+    for (size_t k = 1; k < parts; ++k) th.emplace_back(run, k);
+    run(0);
+    for (auto& t : th) t.join();
+    int st = MPS_OK;
+    for (size_t k = 0; k < parts; ++k) {
+        rels.insert(rels.end(), out[k].begin(), out[k].end());
+        if (prime_flagged) *prime_flagged += flagged[k];
+        if (status[k] != MPS_OK) st = status[k];
+    }
+    return st;
+}
+
 }  // namespace mps
 
 namespace {

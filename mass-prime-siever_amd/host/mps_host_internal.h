@@ -13,4 +13,12 @@ namespace mps {
 int finish_records(const mps_work_item* it, uint64_t lo, uint64_t hi, mps_dev_relation* recs, size_t n,
                    std::vector<mps_relation>& rels, uint64_t* prime_flagged);
 
+// finish_records on up to `threads` threads: the sorted records are cut into ranges at changes of
+// the number, each range is finished on a thread of its own, and the relations are joined in
+// range order, so the result is finish_records' result.  Below `min_records` it runs on the
+// calling thread.
+int finish_records_parallel(const mps_work_item* it, uint64_t lo, uint64_t hi, mps_dev_relation* recs, size_t n,
+                            std::vector<mps_relation>& rels, uint64_t* prime_flagged, int threads,
+                            size_t min_records);
+
 }  // namespace mps

@@ -266,3 +266,13 @@ def test_thread_sanitizer_clean(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     assert "ThreadSanitizer" not in r.stderr, r.stderr[-4000:]
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout
+
+
+def test_finish_records_parallel_matches_serial(tmp_path):
+    """The batch engine finishes passes with many records on several threads
+    (finish_records_parallel); the relations must be finish_records' own, for 1 to 6 threads."""
+    exe = str(tmp_path / "host_parallel_check")
+    compile_engine(exe, main=os.path.join(HERE, "host_parallel_check.cpp"))
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "ok" in r.stdout
