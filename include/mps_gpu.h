@@ -52,9 +52,10 @@ void mps_gpu_set_classes(mps_gpu* g, int on);
  * the reference's rule is gpu_classes, Math.hpp:324-337):
  *   0 = as the reference: an even class modulus cm with lcm(cm, 24) <= 48 and N1 - N0 >= 4096 lcm;
  *   1 = any cm with lcm(cm, 24) <= 48 and the same span (default);
- *   2 = order mode: every item (classes on, lcm(cm, 367567200) < 2^32) gets one exponent class per
- *       prime, n == n* (mod L), from the power residues of b and -c modulo gcd(lcm(cm, 367567200),
- *       p - 1) (Pohlig-Hellman on the device); other items as under 1.
+ *   2 = order mode: every item (classes on, lcm(cm, 367567200) < 2^52) gets one exponent class per
+ *       prime, n == n* (mod L), from the power residues of b and -c modulo gcd(Qo, p - 1), Qo =
+ *       lcm(cm, B) for the longest of B = 367567200 (2^5 3^3 5^2 7 11 13 17), then times 19, 23,
+ *       29, 31, 37, with Qo below 2^52 (Pohlig-Hellman on the device); other items as under 1.
  * Relations are identical under all three; only the device work differs. */
 int mps_gpu_set_class_policy(mps_gpu* g, int policy);
 
