@@ -15,7 +15,7 @@ constexpr uint32_t kMaxClasses = 24;              // classes per prime in class 
 constexpr uint32_t kClassMaxM = 1024;             // default largest baby-step count in class mode
 constexpr size_t kMaxBsgsLds = 64 * 1024;          // LDS of one BSGS workgroup at most
 constexpr uint32_t kBsgsOversubscribe = 16;      // batch engine: BSGS grid / resident workgroups
-constexpr uint32_t kSetupTile = 1024;             // candidates a setup workgroup sorts at a time (class, order mode)
+constexpr uint32_t kSetupTile = 1024;             // candidates a class-mode setup workgroup sorts at a time
 // Setup modes (ItemParams.qmode).
 constexpr uint32_t kModePlain = 0, kModeClass = 1, kModeOrder = 2;
 // Order mode: every p - 1 is split by Qo = lcm(cm, B), B the longest of the prefix products

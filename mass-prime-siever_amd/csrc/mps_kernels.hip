@@ -6,11 +6,15 @@
 //             never crossed off; survivors are compacted by a block scan of popcounts.
 //   k_filter  per survivor: Jacobi-symbol residue filter (plain mode) or per-prime class split with
 //             a quadratic-residue mask (class mode); wave ballot compaction of what is left.
-//   k_setup   per candidate: base-2 Fermat test, g-th power residue test, BSGS sizes and the entry
-//             (powers tables, giant starts), compacted by wave ballot.
-//   k_bsgs    one wavefront per entry: baby steps t*B^i into an LDS fingerprint table, giant steps
-//             G_j*(B^m)^g looked up; a hit appends p to the flagged list.
-// Numbers the device cannot decide (p < 1000, p | b) are appended to the flagged list directly.
+//   k_setup   per candidate: base-2 Fermat test, g-th power residue test (class mode), BSGS sizes
+//             and the entry (powers tables, giant starts), compacted by block ballot.
+//   k_setup_order  order mode: per candidate g, b^e, t^e; per (candidate, prime of g) pair a
+//             Pohlig-Hellman step; per candidate the CRT class; per kept candidate the entry.
+//   k_bsgs    one workgroup (1, 2 or 4 wavefronts) per entry: baby steps t*B^i into an LDS
+//             fingerprint table, giant steps G_j*(B^m)^g looked up, every match verified exactly;
+//             the smallest solution goes to the records.
+//   k_prime_records  12-base Miller-Rabin over the pass's records.
+// Numbers the device cannot decide (p < 1000, p | b, a short cycle) go to the records for the host.
 #include <algorithm>
 #include <type_traits>
 
