@@ -17,7 +17,6 @@
 
 namespace mps {
 
-__device__ __forceinline__ uint64_t umulhi64(uint64_t a, uint64_t b) { return __umul64hi(a, b); }
 
 // x mod d for d < 2^16, from 32-bit remainders only; r32 = 2^32 mod d.
 __device__ __forceinline__ uint32_t mod_small(uint64_t x, uint32_t d, uint32_t r32) {
@@ -112,12 +111,37 @@ struct MontField {
         one_ = one;
     }
     __device__ __forceinline__ T one() const { return one_; }
+    // a * b * 2^-64 mod p, as a Montgomery product over 32-bit words (CIOS: each word of b adds
+    // a * b_i, then m * p with m = t0 * (-p^-1) mod 2^32 clears the low word, which is shifted
+    // out).  With p < 2^62 the partial sums stay below 2^63 after each shift, so two words hold
+    // them and one final subtraction brings the result into [0, p): the same value as the 64-bit
+    // REDC (lo * ninv, high halves of a*b and m*p), in about half its instructions (each product
+    // of 32-bit words with a 64-bit addend is one v_mad_u64_u32).
     __device__ __forceinline__ T mul(T a, T b) const {
-        const uint64_t lo = a * b;
-        const uint64_t hi = umulhi64(a, b);
-        const uint64_t m = lo * ninv;
-        const uint64_t t = hi + umulhi64(m, p) + (lo != 0 ? 1 : 0);
-        return t >= p ? t - p : t;
+        const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+        const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+        const uint32_t p0 = (uint32_t)p, p1 = (uint32_t)(p >> 32), n0 = (uint32_t)ninv;
+        uint64_t s = (uint64_t)a0 * b0;
+        uint32_t t0 = (uint32_t)s;
+        s = (uint64_t)a1 * b0 + (s >> 32);
+        uint32_t t1 = (uint32_t)s, t2 = (uint32_t)(s >> 32);
+        uint32_t m = t0 * n0;
+        s = (uint64_t)m * p0 + t0;
+        s = (uint64_t)m * p1 + t1 + (s >> 32);
+        t0 = (uint32_t)s;
+        t1 = t2 + (uint32_t)(s >> 32);
+        s = (uint64_t)a0 * b1 + t0;
+        t0 = (uint32_t)s;
+        s = (uint64_t)a1 * b1 + t1 + (s >> 32);
+        t1 = (uint32_t)s;
+        t2 = (uint32_t)(s >> 32);
+        m = t0 * n0;
+        s = (uint64_t)m * p0 + t0;
+        s = (uint64_t)m * p1 + t1 + (s >> 32);
+        t0 = (uint32_t)s;
+        t1 = t2 + (uint32_t)(s >> 32);
+        const uint64_t r = ((uint64_t)t1 << 32) | t0;
+        return r >= p ? r - p : r;
     }
     __device__ __forceinline__ T dbl(T a) const {
         const uint64_t r = a + a;
