@@ -22,8 +22,9 @@ WORKLOAD = dict(b=10, c=111, P0=10**12, P1=10**12 + 2**28, N0=0, N1=10_000_000, 
 CHUNK = 1 << 26
 # fp64 operations per modular product of the FP64 field: mul, fma, fma + add (quotient), fma + add
 FLOP_PER_PRODUCT_F64 = 9
-# 32-bit multiplies per 64-bit Montgomery product (a*b: 4, lo*ninv: 3, high half of m*p: 4)
-MUL_PER_PRODUCT_MONT = 11
+# 32-bit multiplies per Montgomery product over two 32-bit words (per word of b: a * b_i: 2,
+# m = t0 * n0: 1, m * p: 2)
+MUL_PER_PRODUCT_MONT = 10
 PEAK_FP64_TFLOPS = 78.6          # MI355X published FP64 vector/matrix peak
 PEAK_INT_MUL_TOPS = 31.3         # measured full-rate v_mul_lo_u32 issue over the chip (mps_gpu_measure_valu)
 # HBM bytes per tuned k_bsgs launch on this workload: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scale factors
