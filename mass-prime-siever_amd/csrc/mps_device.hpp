@@ -96,6 +96,7 @@ __device__ __forceinline__ int jacobi_signed(int64_t x, uint64_t p) {
 struct MontField {
     typedef uint64_t T;
     static constexpr bool kKeepsOne = true;  // set() takes the Montgomery form of 1
+    static constexpr int kOrderSetupWaves = 6;  // order-mode setup: its LDS allows 6 workgroups per CU
     uint64_t p, ninv, one_;
 
     __device__ __forceinline__ void init(uint64_t mod) {
@@ -167,6 +168,7 @@ struct MontField {
 struct F64Field {
     typedef double T;
     static constexpr bool kKeepsOne = false;  // set() ignores its third argument
+    static constexpr int kOrderSetupWaves = 7;  // order-mode setup: 72 registers, 7 waves per SIMD
     double p, pinv;
     uint32_t plo;  // low word of p
 

@@ -989,11 +989,12 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
     setup_pass<F, MODE>(d, pb);
 }
 
-// Order mode at 7 waves per SIMD: the register cap (72) costs two spilled registers and measured
-// faster than 82 registers at 5 waves (c3 setup 1.23 -> 1.13 ms); the other modes keep theirs.
+// Order mode at F::kOrderSetupWaves waves per SIMD (FP64 field: 7, i.e. 72 registers, measured
+// faster than 82 registers at 5 waves, c3 setup 1.23 -> 1.13 ms; the Montgomery field's LDS allows
+// 6); the other modes keep their registers.
 template <class F>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void k_setup_order(const PassDesc* __restrict__ d,
-                                                                                             PassBuffers pb) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F::kOrderSetupWaves))) void k_setup_order(
+    const PassDesc* __restrict__ d, PassBuffers pb) {
     setup_pass<F, kModeOrder>(d, pb);
 }
 
