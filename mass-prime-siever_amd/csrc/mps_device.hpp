@@ -169,18 +169,22 @@ struct F64Field {
     typedef double T;
     static constexpr bool kKeepsOne = false;  // set() ignores its third argument
     static constexpr int kOrderSetupWaves = 7;  // order-mode setup: 72 registers, 7 waves per SIMD
+    static constexpr uint32_t kHashMul = 0x9E3779B1u;
     double p, pinv;
     uint32_t plo;  // low word of p
+    uint32_t plc;  // plo * kHashMul (mod 2^32)
 
     __device__ __forceinline__ void init(uint64_t mod) {
         p = (double)mod;
         pinv = 1.0 / p;
         plo = (uint32_t)mod;
+        plc = plo * kHashMul;
     }
     __device__ __forceinline__ void set(uint64_t mod, uint64_t pinv_bits, uint64_t) {
         p = (double)mod;
         pinv = __longlong_as_double((long long)pinv_bits);
         plo = (uint32_t)mod;
+        plc = plo * kHashMul;
     }
     __device__ __forceinline__ T one() const { return 1.0; }
     __device__ __forceinline__ T mul(T a, T b) const {
@@ -205,10 +209,16 @@ struct F64Field {
     // bijection of the low word, so equal words mean equal low words).  Plain residues have structure
     // (t * B^i below p are small multiples of powers of b), which the product spreads over all bits.
     // a + 1.5 * 2^52 (|a| < 2^51) holds a + 2^51 in its mantissa, whose low word is a mod 2^32; a
-    // negative a stands for a + p.
+    // negative a stands for a + p.  The sign comes from the same bits: mantissa bit 51 (bit 19 of
+    // the high word) is set exactly when a >= 0 (-0.0 included), and (lo + plo) * kHashMul is
+    // lo * kHashMul + plc, so the correction is a select of a per-modulus constant: v_bfe_i32
+    // spreads bit 51 over a word and v_bfi_b32 keeps plc where it is clear (two VALU; written
+    // out because the compiler turns the same expression into a compare, a move and a select).
     __device__ __forceinline__ uint32_t hash_word(T a) const {
-        const uint32_t lo = (uint32_t)__double_as_longlong(a + 6755399441055744.0);
-        return (a < 0.0 ? lo + plo : lo) * 0x9E3779B1u;
+        const uint64_t x = (uint64_t)__double_as_longlong(a + 6755399441055744.0);
+        uint32_t corr;  // plc when a < 0, else 0
+        asm("v_bfe_i32 %0, %1, 19, 1\n\tv_bfi_b32 %0, %0, 0, %2" : "=&v"(corr) : "v"((uint32_t)(x >> 32)), "v"(plc));
+        return (uint32_t)x * kHashMul + corr;
     }
     __device__ __forceinline__ uint64_t aux() const { return (uint64_t)__double_as_longlong(pinv); }
     __device__ __forceinline__ static uint64_t to_bits(T a) { return (uint64_t)__double_as_longlong(a); }
