@@ -214,10 +214,12 @@ struct F64Field {
     // lo * kHashMul + plc, so the correction is a select of a per-modulus constant: v_bfe_i32
     // spreads bit 51 over a word and v_bfi_b32 keeps plc where it is clear (two VALU; written
     // out because the compiler turns the same expression into a compare, a move and a select).
+    // The modulus must be the same in every lane of the wavefront (k_bsgs, one entry per
+    // workgroup, is the only caller): plc is an SGPR operand of the select.
     __device__ __forceinline__ uint32_t hash_word(T a) const {
         const uint64_t x = (uint64_t)__double_as_longlong(a + 6755399441055744.0);
         uint32_t corr;  // plc when a < 0, else 0
-        asm("v_bfe_i32 %0, %1, 19, 1\n\tv_bfi_b32 %0, %0, 0, %2" : "=&v"(corr) : "v"((uint32_t)(x >> 32)), "v"(plc));
+        asm("v_bfe_i32 %0, %1, 19, 1\n\tv_bfi_b32 %0, %0, 0, %2" : "=&v"(corr) : "v"((uint32_t)(x >> 32)), "s"(plc));
         return (uint32_t)x * kHashMul + corr;
     }
     __device__ __forceinline__ uint64_t aux() const { return (uint64_t)__double_as_longlong(pinv); }

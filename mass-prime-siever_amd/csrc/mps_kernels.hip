@@ -1026,11 +1026,15 @@ __device__ __forceinline__ void hash_word(uint32_t w, uint32_t omask, uint32_t& 
     boff = (w >> 16) & omask;
 }
 
+// boff is a multiple of 16 (a bucket's byte offset), so the bucket's counter is at byte offset
+// boff / 4 of the counters and slot k at byte offset boff + 2k of the fingerprints; byte offsets
+// keep the compiler from masking the low bits again (two address operations instead of five).
 __device__ __forceinline__ void tab_insert(const BsgsTable& T, uint32_t boff, uint32_t fp) {
     for (;;) {
-        const uint32_t old = atomicAdd(&T.cnt[boff >> 4], 1u);
+        uint32_t* c = reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(T.cnt) + (boff >> 2));
+        const uint32_t old = atomicAdd(c, 1u);
         if (old < 8) {
-            T.fp[(boff >> 1) + old] = (uint16_t)fp;
+            *reinterpret_cast<uint16_t*>(reinterpret_cast<char*>(T.fp) + boff + 2u * old) = (uint16_t)fp;
             return;
         }
         boff = (boff + 16u) & T.omask;
@@ -1190,8 +1194,9 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
 
         // baby steps t * B^i, i == tid (mod L); a baby value equal to t for some 0 < i < m means
         // the order of B is below m: the giant steps might not reach the smallest solution, and
-        // the host decides the prime
-        bool cyc = false;
+        // the host decides the prime.  Each lane counts its baby values equal to t; lane 0's first
+        // one (i = 0) is t itself, so the order is below m when a lane counts more than that.
+        uint32_t nt = 0;
         // the workgroup's next entry: one load per cache line of its first 48 words (the header, the
         // start tables and the strides) sets those lines on their way to the cache while this
         // entry's steps run; consumed at the end of the iteration, when it has long arrived
@@ -1210,18 +1215,18 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             // take one step more
             const uint32_t rounds = be.m / L, rem = be.m - rounds * L;
             const uint32_t wt = f.hash_word(be.t);
-            auto baby = [&](uint32_t i) {
+            auto baby = [&]() {  // baby step i = tid + r * L
                 const uint32_t w = f.hash_word(v);
                 uint32_t fp, bo;
                 hash_word(w, Tb.omask, fp, bo);
                 tab_insert(Tb, bo, fp);
-                if ((w == wt) & (i != 0)) cyc |= f.key(v) == f.key(be.t);
+                if (w == wt) nt += f.key(v) == f.key(be.t);  // rare: the full key decides
                 v = f.mul(v, bstep);
             };
-            for (uint32_t r = 0; r < rounds; ++r) baby(tid + r * L);
-            if (tid < rem) baby(tid + rounds * L);
+            for (uint32_t r = 0; r < rounds; ++r) baby();
+            if (tid < rem) baby();
         }
-        if (cyc) hits.n = kCycleFlag;  // every writer stores the same value
+        if (nt > (tid == 0 ? 1u : 0u)) hits.n = kCycleFlag;  // every writer stores the same value
         __syncthreads();
         if (hits.n >= kCycleFlag) {  // the same verdict in every lane
             if (tid == 0) put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (ssrc >> 32) : p, kHostDecides);
