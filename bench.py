@@ -29,7 +29,7 @@ PEAK_FP64_TFLOPS = 78.6          # MI355X published FP64 vector/matrix peak
 PEAK_INT_MUL_TOPS = 31.3         # measured full-rate v_mul_lo_u32 issue over the chip (mps_gpu_measure_valu)
 # HBM bytes per tuned k_bsgs launch on this workload: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scale factors
 # from a 1 GiB streaming copy), from profiles/k_bsgs_c2_hbm_traffic.txt
-K_BSGS_TRAFFIC_BYTES = 2.973e8
+K_BSGS_TRAFFIC_BYTES = 2.932e8
 TUNING_FILE = os.path.join(ROOT, "mass-prime-siever_amd", "tuning_mi355x.json")
 
 
