@@ -96,6 +96,7 @@ __device__ __forceinline__ int jacobi_signed(int64_t x, uint64_t p) {
 struct MontField {
     typedef uint64_t T;
     static constexpr bool kKeepsOne = true;  // set() takes the Montgomery form of 1
+    static constexpr bool kSmallMul = false;  // no mul_small (operands are in Montgomery form)
     static constexpr int kOrderSetupWaves = 6;  // order-mode setup: its LDS allows 6 workgroups per CU
     uint64_t p, ninv, one_;
 
@@ -168,6 +169,7 @@ struct MontField {
 struct F64Field {
     typedef double T;
     static constexpr bool kKeepsOne = false;  // set() ignores its third argument
+    static constexpr bool kSmallMul = true;   // mul_small: a times a small integer
     static constexpr int kOrderSetupWaves = 7;  // order-mode setup: 72 registers, 7 waves per SIMD
     static constexpr uint32_t kHashMul = 0x9E3779B1u;
     double p, pinv;
@@ -192,6 +194,13 @@ struct F64Field {
         const double l = __fma_rn(a, b, -h);
         const double q = __fma_rn(h, pinv, 6755399441055744.0) - 6755399441055744.0;
         return __fma_rn(-q, p, h) + l;
+    }
+    // a * s for an integer s with |a * s| < 2^52: the product is exact, so one rounding step
+    // (q = round(x / p), x - q * p, exact in the FMA) reduces it; 4 operations instead of 6.
+    __device__ __forceinline__ T mul_small(T a, double s) const {
+        const double x = a * s;
+        const double q = __fma_rn(x, pinv, 6755399441055744.0) - 6755399441055744.0;
+        return __fma_rn(-q, p, x);
     }
     __device__ __forceinline__ T dbl(T a) const {
         double r = a + a;
@@ -250,6 +259,27 @@ __device__ __forceinline__ void fpow_2e(const F& f, typename F::T x, uint64_t e1
         e1 >>= 1;
         e2 >>= 1;
         if (e1 | e2) x = f.mul(x, x);
+    }
+}
+
+// sa^e and sb^e for integers sa, sb with |sa| * p, |sb| * p < 2^52 (F64Field), left to right:
+// each bit squares both and, when set, multiplies them by sa and sb with mul_small (the base
+// stays a small integer, so that product is exact).  Any integer of the residue class will do
+// for sa and sb (a negative -c stands for p - c).
+template <class F>
+__device__ __forceinline__ void fpow2_small(const F& f, double sa, double sb, uint64_t e, double& ra, double& rb) {
+    ra = 1.0;
+    rb = 1.0;
+    if (e == 0) return;
+    ra = f.mul_small(ra, sa);  // the top bit
+    rb = f.mul_small(rb, sb);
+    for (int k = 62 - __builtin_clzll(e); k >= 0; --k) {
+        ra = f.mul(ra, ra);
+        rb = f.mul(rb, rb);
+        if ((e >> k) & 1) {
+            ra = f.mul_small(ra, sa);
+            rb = f.mul_small(rb, sb);
+        }
     }
 }
 

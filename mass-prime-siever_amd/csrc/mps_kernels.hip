@@ -696,7 +696,16 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 const uint32_t tz = __builtin_ctzll(g);
                 const uint64_t e = exact_div_odd((pc - 1) >> tz, g >> tz);
                 T Be, Te;
-                fpow2(f, f.from(bmod, rr), f.from(tv, rr), e, Be, Te);
+                // b and -c as small integers when their products with residues stay exact
+                const double sb = (double)bmod, sc = -(double)ip.c, pd = (double)pc;
+                if constexpr (F::kSmallMul) {
+                    if (sb * pd < 0x1p52 && fabs(sc) * pd < 0x1p52)
+                        fpow2_small(f, sb, sc, e, Be, Te);
+                    else
+                        fpow2(f, f.from(bmod, rr), f.from(tv, rr), e, Be, Te);
+                } else {
+                    fpow2(f, f.from(bmod, rr), f.from(tv, rr), e, Be, Te);
+                }
                 w_setup += 2 * bitlen64(e);
                 R.p[c] = pc;
                 R.aux[c] = f.aux();
