@@ -354,6 +354,31 @@ bool read_counts(mps_gpu* g) {
     return true;
 }
 
+// Records copied back with a full pass's counts, under the same wait (a pass has a few; more
+// are fetched by read_records).
+constexpr uint32_t kRecPrefix = 1024;
+
+// Counts and the first kRecPrefix records of the pass just enqueued, one wait.
+bool read_counts_and_records(mps_gpu* g) {
+    const size_t pre = std::min<size_t>(kRecPrefix, g->pb.out_cap);
+    HIPCHK(hipMemcpyAsync(g->h_cnt, g->pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, g->stream), "count read");
+    HIPCHK(hipMemcpyAsync(g->h_out, g->pb.out, 2 * pre * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream),
+           "record read");
+    HIPCHK(hipStreamSynchronize(g->stream), "stream sync");
+    return true;
+}
+
+// The pass's n_out records in h_out: those past the prefix read_counts_and_records brought.
+bool read_records(mps_gpu* g, uint32_t n_out) {
+    const size_t pre = std::min<size_t>(kRecPrefix, g->pb.out_cap);
+    if (n_out <= pre) return true;
+    HIPCHK(hipMemcpyAsync(g->h_out + 2 * pre, g->pb.out + 2 * pre, 2 * (n_out - pre) * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, g->stream),
+           "record read");
+    HIPCHK(hipStreamSynchronize(g->stream), "stream sync");
+    return true;
+}
+
 // Workgroups for n items of per_block each, capped at 8 per CU: the stage kernels loop over tiles,
 // so a capped grid keeps their per-workgroup atomics few.
 uint32_t blocks_for(const mps_gpu* g, uint64_t n, uint32_t per_block) {
@@ -414,16 +439,16 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     launch_bsgs(s, fk, tg, d, g->pb, 0, bsgs_persistent_grid(fk, tg, g->num_cu));
     HIPCHK(hipGetLastError(), "bsgs launch");
     HIPCHK(hipEventRecord(g->ev[4], s), "event");
-    if (!read_counts(g)) return false;
+    // the records' Miller-Rabin test reads the record count on the device, so it is enqueued now
+    // and the counts and records come back under one wait
+    launch_prime_records(s, g->pb);
+    HIPCHK(hipGetLastError(), "record test launch");
+    if (!read_counts_and_records(g)) return false;
     g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
     g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
     g->st.ms_setup += elapsed(g->ev[2], g->ev[3]);
     g->st.ms_bsgs += elapsed(g->ev[3], g->ev[4]);
     g->st.n_bsgs_launches++;
-    if (g->h_cnt->n_out != 0) {  // the record copy that follows on the stream waits for it
-        launch_prime_records(s, g->pb);
-        HIPCHK(hipGetLastError(), "record test launch");
-    }
     return true;
 }
 
@@ -586,12 +611,7 @@ int sieve_records(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, st
             return MPS_DECLINED;
         }
         if (n_out) {
-            if (hipMemcpyAsync(g->h_out, g->pb.out, 2 * n_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream) !=
-                    hipSuccess ||
-                hipStreamSynchronize(g->stream) != hipSuccess) {
-                hip_fail(g, hipErrorUnknown, "record read");
-                return MPS_ERROR;
-            }
+            if (!read_records(g, n_out)) return MPS_ERROR;
             for (uint32_t i = 0; i < n_out; ++i) {
                 if (g->h_out[2 * i] == 0) continue;  // composite (k_prime_records)
                 rec.push_back(g->h_out[2 * i]);
@@ -694,12 +714,7 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
             return MPS_DECLINED;
         }
         if (n_out) {
-            if (hipMemcpyAsync(g->h_out, g->pb.out, 2 * n_out * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream) !=
-                    hipSuccess ||
-                hipStreamSynchronize(g->stream) != hipSuccess) {
-                hip_fail(g, hipErrorUnknown, "record read");
-                return MPS_ERROR;
-            }
+            if (!read_records(g, n_out)) return MPS_ERROR;
             for (uint32_t i = 0; i < n_out; ++i) idx.push_back(off + g->h_out[2 * i]);
         }
     }
