@@ -1,7 +1,8 @@
-// Exact host checks of the numbers the GPU worker flags: deterministic Miller-Rabin, the exact
-// baby-step giant-step of run_bsgs and the multiplicative order (Math.hpp:209-310, :377-542).
-// The GPU verdicts are a superset (32-bit fingerprints, base-2 Fermat), so every relation written
-// comes from here.
+// Exact host checks: deterministic Miller-Rabin, the exact baby-step giant-step of run_bsgs and
+// the multiplicative order (Math.hpp:209-310, :377-542).  They finish the numbers
+// mps_gpu_sieve_filter returns (the reference's superset contract: base-2 Fermat pseudoprimes may
+// be among them) and the records the device leaves to the host (M = MPS_HOST_DECIDES), and give
+// q = ord_p(b) for every relation.
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
