@@ -321,10 +321,4 @@ __device__ __forceinline__ bool fermat2(const F& f, uint64_t p) {
     return f.key(x) == f.key(f.one());
 }
 
-// 32-bit table word of a canonical residue: the two halves folded; 0xFFFFFFFF marks an empty slot.
-__device__ __forceinline__ uint32_t table_word(uint64_t key) {
-    const uint32_t w = (uint32_t)key ^ (uint32_t)(key >> 32) * 0x9E3779B1u;
-    return w == 0xFFFFFFFFu ? 0xFFFFFFFEu : w;
-}
-
 }  // namespace mps
