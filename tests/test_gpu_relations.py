@@ -108,6 +108,31 @@ def test_order_mode_relations_match_oracle(gpu, mps, oracle, it):
     assert st.declined == 0
 
 
+# Order-mode setup raises b and t = -c to e = (p - 1) / g with exact small-integer multiplies
+# while b * p and |c| * p stay below 2^52, and with full products above; these ranges straddle
+# that bound (p = 2^52 / 1000003, about 4.5e9), with relations on both sides.
+SMALL_BASE_EDGE = (1 << 52) // 1000003
+SMALL_BASE_ITEMS = [
+    (3, -1000003, SMALL_BASE_EDGE - 2**20, SMALL_BASE_EDGE + 2**20, 0, 10_000_000, 6, 1),
+    (1000003, 5, SMALL_BASE_EDGE - 2**20, SMALL_BASE_EDGE + 2**20, 0, 10_000_000, 1, 0),
+]
+
+
+@pytest.mark.parametrize("it", SMALL_BASE_ITEMS)
+def test_order_mode_small_base_edge(gpu, mps, oracle, it):
+    b, c, P0, P1, N0, N1, mod, res = it
+    print(f"order mode small-base edge {it}", flush=True)
+    gpu.set_class_policy(2)
+    try:
+        rels, st = gpu.run_item(mps.work_item(b, c, P0, P1, N0, N1, mod, res), chunk_ints=1 << 20)
+    finally:
+        gpu.set_class_policy(1)
+    want, _ = oracle.process_range(oracle.make_item(b, c, P0, P1, N0, N1, mod, res), P0, P1, 1 << 20, 8)
+    assert rels == want
+    assert st.declined == 0
+    assert any(p < SMALL_BASE_EDGE for p, _, _ in want) and any(p > SMALL_BASE_EDGE for p, _, _ in want)
+
+
 # Class moduli past the primes and prime powers of 367567200 = 2^5 3^3 5^2 7 11 13 17: 38 = 2 * 19
 # gives Qo an eighth prime, 81 = 3^4 a finer 3-part than Qo's own, and 46 = 2 * 23 takes Qo past
 # 2^32 (the item then runs in class mode).  Order mode runs one work-list pair per prime of g.
