@@ -25,6 +25,22 @@ __device__ __forceinline__ uint32_t mod_small(uint64_t x, uint32_t d, uint32_t r
     return r % d;
 }
 
+// a mod d for d < 2^16 with dinv = 1.0 / d (rounded to nearest, from the host).  a and d are exact
+// doubles and a * dinv lies within 2^-20 of a / d, which is an integer or at least 1/d >= 2^-16
+// below one, so the truncated quotient is floor(a / d) or one less: r < 2d, and min corrects it.
+__device__ __forceinline__ uint32_t mod16_f(uint32_t a, uint32_t d, double dinv) {
+    const uint32_t q = (uint32_t)((double)a * dinv);
+    const uint32_t r = a - q * d;
+    return min(r, r - d);
+}
+
+// mod_small for a divisor fixed per item, with its reciprocal: three quotients from FP64 products
+// in place of three integer divisions ((d - 1) * r32 + d - 1 < 2^32 for d <= 2^16).
+__device__ __forceinline__ uint32_t mod_small_f(uint64_t x, uint32_t d, uint32_t r32, double dinv) {
+    const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
+    return mod16_f(mod16_f(hi, d, dinv) * r32 + mod16_f(lo, d, dinv), d, dinv);
+}
+
 __device__ __forceinline__ uint32_t gcd32(uint32_t a, uint32_t b) {
     while (b) {
         const uint32_t t = a % b;
@@ -34,10 +50,9 @@ __device__ __forceinline__ uint32_t gcd32(uint32_t a, uint32_t b) {
     return a;
 }
 
-// Jacobi symbol (a/n) for odd n, both below 2^32.
+// Jacobi symbol (a/n) for odd n < 2^32 and a < n (callers pass a reduced a).
 __device__ __forceinline__ int jacobi32(uint32_t a, uint32_t n) {
     int s = 1;
-    a %= n;
     while (a != 0) {
         const uint32_t e = __builtin_ctz(a);
         a >>= e;

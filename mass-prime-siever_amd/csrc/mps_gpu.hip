@@ -121,6 +121,7 @@ JacobiPre jacobi_pre(int64_t x) {
     if (ax >= 65536) return J;
     J.a = (uint32_t)ax;
     J.r32 = (uint32_t)((1ull << 32) % ax);
+    J.dinv = 1.0 / (double)ax;
     J.neg = x < 0 ? 1u : 0u;
     J.e_odd = e & 1u;
     return J;
@@ -144,6 +145,7 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
     ip.jb = jacobi_pre(it->b);
     ip.jt = jacobi_pre(it->c == INT64_MIN ? 0 : -it->c);
     ip.cm_r32 = cm < 65536 ? (uint32_t)((1ull << 32) % cm) : 0;
+    ip.cm_dinv = cm < 65536 ? 1.0 / (double)cm : 0.0;
     ip.mcap = mcap;
     ip.lanes = L;
     uint64_t Q = 0, Qo = 0;
@@ -153,6 +155,7 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
         ip.Qmax = Q;
         ip.n0modQ = (uint32_t)(N0 % Q);
         ip.Q_r32 = (uint32_t)((1ull << 32) % Q);
+        ip.Q_dinv = 1.0 / (double)Q;
     }
     ip.qmode = omode ? kModeOrder : qmode ? kModeClass : kModePlain;
     m_max = 1;

@@ -70,6 +70,7 @@ struct JacobiPre {
     uint32_t a, r32;
     uint32_t neg;    // x < 0: the factor (-1/p)
     uint32_t e_odd;  // e odd: the factor (2/p)
+    double dinv;     // 1.0 / a
 };
 
 struct ItemParams {
@@ -84,6 +85,8 @@ struct ItemParams {
     uint32_t qmode;         // setup mode: 0 plain, 1 per-prime classes, 2 per-prime order (kModeOrder)
     uint32_t cm_r32;        // 2^32 mod cm when cm < 2^16
     uint32_t Q_r32;         // 2^32 mod Qmax
+    double cm_dinv;         // 1.0 / cm when cm < 2^16
+    double Q_dinv;          // 1.0 / Qmax
     uint32_t mcap;          // plain mode: largest baby-step count (bounds the LDS table)
     uint32_t lanes;         // lanes of one BSGS workgroup
     // plain mode sizes by (stride - 1) * 2 + parity, valid when kcount <= p

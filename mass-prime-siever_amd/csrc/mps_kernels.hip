@@ -202,8 +202,8 @@ __device__ __forceinline__ uint64_t mod_signed(int64_t x, uint64_t p) {  // x mo
 }
 
 // (p - 1) mod d for the class modulus: 32-bit path when d < 2^16.
-__device__ __forceinline__ uint64_t pm1_mod(uint64_t p, uint64_t d, uint32_t r32) {
-    if (d < 65536u) return mod_small(p - 1, (uint32_t)d, r32);
+__device__ __forceinline__ uint64_t pm1_mod(uint64_t p, uint64_t d, uint32_t r32, double dinv) {
+    if (d < 65536u) return mod_small_f(p - 1, (uint32_t)d, r32, dinv);
     return (p - 1) % d;
 }
 
@@ -216,7 +216,7 @@ __device__ __forceinline__ int jacobi_item(const JacobiPre& J, int64_t x, uint64
     if (J.e_odd && (p8 == 3u || p8 == 5u)) s = -s;           // (2/p)
     if (J.a == 1) return s;
     if ((J.a & 3u) == 3u && (p8 & 3u) == 3u) s = -s;         // reciprocity
-    return s * jacobi32(mod_small(p, J.a, J.r32), J.a);
+    return s * jacobi32(mod_small_f(p, J.a, J.r32, J.dinv), J.a);
 }
 
 __device__ __forceinline__ uint64_t gcd64(uint64_t a, uint64_t b) {
@@ -242,7 +242,7 @@ __device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, b
     if (mod_signed(ip.b, p) == 0) { flag2 = true; return; }
     if (mod_signed(ip.c, p) == 0) return;  // t = -c == 0: b^n == 0 impossible
     if (ip.kcount == 0) return;
-    const uint64_t pm = pm1_mod(p, ip.cm, ip.cm_r32);
+    const uint64_t pm = pm1_mod(p, ip.cm, ip.cm_r32, ip.cm_dinv);
     const uint64_t g = ip.cm < (1ull << 32) ? gcd32((uint32_t)ip.cm, (uint32_t)pm) : gcd64(ip.cm, pm);
     const int jb = jacobi_item(ip.jb, ip.b, p);
     const int jt = jacobi_item(ip.jt, -ip.c, p);
@@ -271,7 +271,7 @@ struct ClassSplit {
 __device__ __forceinline__ void class_split(uint64_t p, const ItemParams& ip, ClassSplit& cs) {
     const uint32_t Q = (uint32_t)ip.Qmax;
     cs.cm = (uint32_t)ip.cm;
-    cs.g = gcd32(Q, mod_small(p - 1, Q, ip.Q_r32));
+    cs.g = gcd32(Q, mod_small_f(p - 1, Q, ip.Q_r32, ip.Q_dinv));
     cs.qp = cs.g / gcd32(cs.g, cs.cm) * cs.cm;
     cs.ncls = cs.qp / cs.cm;
     cs.n0q = ip.n0modQ % cs.qp;
@@ -869,7 +869,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                             keep = true;
                         }
                     } else {
-                        const uint64_t g = gcd64(ip.cm, pm1_mod(p, ip.cm, ip.cm_r32));
+                        const uint64_t g = gcd64(ip.cm, pm1_mod(p, ip.cm, ip.cm_r32, ip.cm_dinv));
                         bool ok = true;
                         if (g > 2) {
                             // B = b^cm is a g-th power, so T = t*b^-n_start must be one: t^e == G0^e
