@@ -138,7 +138,7 @@ bool batch_fail(mps_batch* b, hipError_t e, const std::string& what) {
 bool alloc_slot(mps_batch* b, Slot& s) {
     const uint64_t odds = b->o.pass_odds;
     const uint64_t ent_cap = std::max<uint64_t>(odds / 8, 4096);
-    const uint64_t out_cap = std::max<uint64_t>(odds / 16, 4096);
+    const uint64_t out_cap = record_cap(odds);
     BCHK(hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking), "slot stream");
     BCHK(hipMalloc(&s.pb.surv, odds * sizeof(uint32_t)), "surv alloc");
     BCHK(hipMalloc(&s.pb.cand, odds * sizeof(uint64_t)), "cand alloc");

@@ -10,16 +10,26 @@
 //               is exact because |h - q*p| < 2^53.  |q - x/p| < 0.76, so r lies in (-p, p): values
 //               are kept in that signed range and made canonical only where they are compared.
 // Both give the canonical residue through key(), which every hash and equality test uses.
+//
+// Every function here also compiles for the host (MPS_HD = __host__ __device__), where it runs on
+// the CPU's 64-bit integers and IEEE doubles (fma from the C library, which rounds once, as
+// v_fma_f64 does): the CPU tests check the same code against 128-bit and exact arithmetic.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#define MPS_HD __host__ __device__ __forceinline__
+
 namespace mps {
 
+MPS_HD uint64_t dbits(double x) { return __builtin_bit_cast(uint64_t, x); }
+MPS_HD double dfrom(uint64_t b) { return __builtin_bit_cast(double, b); }
+MPS_HD double fma_rn(double a, double b, double c) { return __builtin_fma(a, b, c); }
+MPS_HD uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
 
 // x mod d for d < 2^16, from 32-bit remainders only; r32 = 2^32 mod d.
-__device__ __forceinline__ uint32_t mod_small(uint64_t x, uint32_t d, uint32_t r32) {
+MPS_HD uint32_t mod_small(uint64_t x, uint32_t d, uint32_t r32) {
     const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
     const uint32_t r = (hi % d) * r32 + (lo % d);
     return r % d;
@@ -28,20 +38,20 @@ __device__ __forceinline__ uint32_t mod_small(uint64_t x, uint32_t d, uint32_t r
 // a mod d for d < 2^16 with dinv = 1.0 / d (rounded to nearest, from the host).  a and d are exact
 // doubles and a * dinv lies within 2^-20 of a / d, which is an integer or at least 1/d >= 2^-16
 // below one, so the truncated quotient is floor(a / d) or one less: r < 2d, and min corrects it.
-__device__ __forceinline__ uint32_t mod16_f(uint32_t a, uint32_t d, double dinv) {
+MPS_HD uint32_t mod16_f(uint32_t a, uint32_t d, double dinv) {
     const uint32_t q = (uint32_t)((double)a * dinv);
     const uint32_t r = a - q * d;
-    return min(r, r - d);
+    return umin(r, r - d);
 }
 
 // mod_small for a divisor fixed per item, with its reciprocal: three quotients from FP64 products
 // in place of three integer divisions ((d - 1) * r32 + d - 1 < 2^32 for d <= 2^16).
-__device__ __forceinline__ uint32_t mod_small_f(uint64_t x, uint32_t d, uint32_t r32, double dinv) {
+MPS_HD uint32_t mod_small_f(uint64_t x, uint32_t d, uint32_t r32, double dinv) {
     const uint32_t hi = (uint32_t)(x >> 32), lo = (uint32_t)x;
     return mod16_f(mod16_f(hi, d, dinv) * r32 + mod16_f(lo, d, dinv), d, dinv);
 }
 
-__device__ __forceinline__ uint32_t gcd32(uint32_t a, uint32_t b) {
+MPS_HD uint32_t gcd32(uint32_t a, uint32_t b) {
     while (b) {
         const uint32_t t = a % b;
         a = b;
@@ -51,7 +61,7 @@ __device__ __forceinline__ uint32_t gcd32(uint32_t a, uint32_t b) {
 }
 
 // Jacobi symbol (a/n) for odd n < 2^32 and a < n (callers pass a reduced a).
-__device__ __forceinline__ int jacobi32(uint32_t a, uint32_t n) {
+MPS_HD int jacobi32(uint32_t a, uint32_t n) {
     int s = 1;
     while (a != 0) {
         const uint32_t e = __builtin_ctz(a);
@@ -66,7 +76,7 @@ __device__ __forceinline__ int jacobi32(uint32_t a, uint32_t n) {
 }
 
 // Jacobi symbol (a/n) for odd n < 2^63, binary method (no 64-bit division).
-__device__ __forceinline__ int jacobi64(uint64_t a, uint64_t n) {
+MPS_HD int jacobi64(uint64_t a, uint64_t n) {
     int s = 1;
     while (a != 0) {
         const uint32_t e = (uint32_t)__builtin_ctzll(a);
@@ -85,7 +95,7 @@ __device__ __forceinline__ int jacobi64(uint64_t a, uint64_t n) {
 
 // (a/p) for a < 2^16, p odd < 2^63: strip the twos, then one reciprocity step brings the
 // problem into 32 bits.
-__device__ __forceinline__ int jacobi_small(uint32_t a, uint64_t p) {
+MPS_HD int jacobi_small(uint32_t a, uint64_t p) {
     if (a == 0) return 0;
     int s = 1;
     const uint32_t e = __builtin_ctz(a);
@@ -99,12 +109,72 @@ __device__ __forceinline__ int jacobi_small(uint32_t a, uint64_t p) {
 }
 
 // Jacobi symbol (x mod p / p) for x given as a signed 64-bit value (b or -c of an item).
-__device__ __forceinline__ int jacobi_signed(int64_t x, uint64_t p) {
+MPS_HD int jacobi_signed(int64_t x, uint64_t p) {
     const uint64_t ax = x < 0 ? (uint64_t)(-x) : (uint64_t)x;
     int s = 1;
     if (x < 0 && (p & 3) == 3) s = -1;  // (-1/p)
     if (ax < 65536u) return s * jacobi_small((uint32_t)ax, p);
     return s * jacobi64(ax % p, p);
+}
+
+
+// ---------------------------------------------------------------------------- integer helpers
+
+MPS_HD uint64_t isqrt_ceil64(uint64_t x) {
+    if (x == 0) return 0;
+    uint64_t r = (uint64_t)sqrt((double)x);
+    while (r > 0 && r * r > x) --r;
+    while ((r + 1) * (r + 1) <= x) ++r;
+    return r * r < x ? r + 1 : r;
+}
+
+MPS_HD uint32_t bitlen64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+// Quotient of an exact division x / d (d odd): x times the inverse of d modulo 2^64.
+MPS_HD uint64_t exact_div_odd(uint64_t x, uint64_t d) {
+    uint64_t inv = d;  // d * d == 1 (mod 8): correct to 3 bits, doubled by each Newton step
+    for (int k = 0; k < 5; ++k) inv *= 2 - d * inv;
+    return x * inv;
+}
+
+// a mod m for a, m below 2^16 (m >= 1): the float quotient is within one of the true one.
+MPS_HD uint32_t mod16(uint32_t a, uint32_t m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float rm = __frcp_rn((float)m);
+#else
+    const float rm = 1.0f / (float)m;  // the same correctly rounded reciprocal
+#endif
+    const uint32_t q = (uint32_t)((float)a * rm);
+    int32_t r = (int32_t)(a - q * m);
+    if (r < 0) r += (int32_t)m;
+    if (r >= (int32_t)m) r -= (int32_t)m;
+    return (uint32_t)r;
+}
+
+// a mod m and a / m for a and m below 2^52 (m >= 1): both are exact doubles, so the double quotient
+// is within one of the true one.
+MPS_HD uint64_t divmod52(uint64_t a, uint64_t m, uint64_t& rem) {
+    uint64_t q = (uint64_t)((double)a / (double)m);
+    int64_t r = (int64_t)(a - q * m);
+    if (r < 0) {
+        r += m;
+        --q;
+    } else if (r >= (int64_t)m) {
+        r -= m;
+        ++q;
+    }
+    rem = (uint64_t)r;
+    return q;
+}
+
+// a * b mod Qo for a < 2^16 and b < Qo < 2^52: the quotient from doubles (a, b exact; the product
+// and the scaling by 1/Qo round) is within one of the true one, and a*b - q*Qo is exact modulo 2^64.
+MPS_HD uint64_t mulmod_q52(uint64_t a, uint64_t b, uint64_t Qo, double Qo_inv) {
+    const uint64_t q = (uint64_t)((double)a * (double)b * Qo_inv);
+    int64_t r = (int64_t)(a * b - q * Qo);
+    if (r < 0) r += (int64_t)Qo;
+    if (r >= (int64_t)Qo) r -= (int64_t)Qo;
+    return (uint64_t)r;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -115,26 +185,26 @@ struct MontField {
     static constexpr int kOrderSetupWaves = 6;  // order-mode setup: its LDS allows 6 workgroups per CU
     uint64_t p, ninv, one_;
 
-    __device__ __forceinline__ void init(uint64_t mod) {
+    MPS_HD void init(uint64_t mod) {
         p = mod;
         uint64_t inv = (3 * mod) ^ 2;  // correct to 5 bits for odd mod
         for (int i = 0; i < 4; ++i) inv *= 2 - mod * inv;
         ninv = 0 - inv;
         one_ = (0 - mod) % mod;
     }
-    __device__ __forceinline__ void set(uint64_t mod, uint64_t ninv_, uint64_t one) {
+    MPS_HD void set(uint64_t mod, uint64_t ninv_, uint64_t one) {
         p = mod;
         ninv = ninv_;
         one_ = one;
     }
-    __device__ __forceinline__ T one() const { return one_; }
+    MPS_HD T one() const { return one_; }
     // a * b * 2^-64 mod p, as a Montgomery product over 32-bit words (CIOS: each word of b adds
     // a * b_i, then m * p with m = t0 * (-p^-1) mod 2^32 clears the low word, which is shifted
     // out).  With p < 2^62 the partial sums stay below 2^63 after each shift, so two words hold
     // them and one final subtraction brings the result into [0, p): the same value as the 64-bit
     // REDC (lo * ninv, high halves of a*b and m*p), in about half its instructions (each product
     // of 32-bit words with a 64-bit addend is one v_mad_u64_u32).
-    __device__ __forceinline__ T mul(T a, T b) const {
+    MPS_HD T mul(T a, T b) const {
         const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
         const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
         const uint32_t p0 = (uint32_t)p, p1 = (uint32_t)(p >> 32), n0 = (uint32_t)ninv;
@@ -160,25 +230,25 @@ struct MontField {
         const uint64_t r = ((uint64_t)t1 << 32) | t0;
         return r >= p ? r - p : r;
     }
-    __device__ __forceinline__ T dbl(T a) const {
+    MPS_HD T dbl(T a) const {
         const uint64_t r = a + a;
         return r >= p ? r - p : r;
     }
     // R^2 mod p as the Montgomery form of 2^64: square mont(2) six times.
-    __device__ __forceinline__ T r2() const {
+    MPS_HD T r2() const {
         T x = dbl(one_);
         for (int i = 0; i < 6; ++i) x = mul(x, x);
         return x;
     }
-    __device__ __forceinline__ T from(uint64_t x, T rr) const { return mul(x, rr); }  // x < p
-    __device__ __forceinline__ uint64_t key(T a) const { return a; }
+    MPS_HD T from(uint64_t x, T rr) const { return mul(x, rr); }  // x < p
+    MPS_HD uint64_t key(T a) const { return a; }
     // A 32-bit word of key(a) for hash tables: Montgomery forms a * 2^64 mod p are already spread
     // over all bits, so the low word serves.
-    __device__ __forceinline__ uint32_t hash_word(T a) const { return (uint32_t)a; }
+    MPS_HD uint32_t hash_word(T a) const { return (uint32_t)a; }
 
-    __device__ __forceinline__ uint64_t aux() const { return ninv; }
-    __device__ __forceinline__ static uint64_t to_bits(T a) { return a; }
-    __device__ __forceinline__ static T from_bits(uint64_t a) { return a; }
+    MPS_HD uint64_t aux() const { return ninv; }
+    MPS_HD static uint64_t to_bits(T a) { return a; }
+    MPS_HD static T from_bits(uint64_t a) { return a; }
 };
 
 struct F64Field {
@@ -191,43 +261,43 @@ struct F64Field {
     uint32_t plo;  // low word of p
     uint32_t plc;  // plo * kHashMul (mod 2^32)
 
-    __device__ __forceinline__ void init(uint64_t mod) {
+    MPS_HD void init(uint64_t mod) {
         p = (double)mod;
         pinv = 1.0 / p;
         plo = (uint32_t)mod;
         plc = plo * kHashMul;
     }
-    __device__ __forceinline__ void set(uint64_t mod, uint64_t pinv_bits, uint64_t) {
+    MPS_HD void set(uint64_t mod, uint64_t pinv_bits, uint64_t) {
         p = (double)mod;
-        pinv = __longlong_as_double((long long)pinv_bits);
+        pinv = dfrom(pinv_bits);
         plo = (uint32_t)mod;
         plc = plo * kHashMul;
     }
-    __device__ __forceinline__ T one() const { return 1.0; }
-    __device__ __forceinline__ T mul(T a, T b) const {
+    MPS_HD T one() const { return 1.0; }
+    MPS_HD T mul(T a, T b) const {
         const double h = a * b;
-        const double l = __fma_rn(a, b, -h);
-        const double q = __fma_rn(h, pinv, 6755399441055744.0) - 6755399441055744.0;
-        return __fma_rn(-q, p, h) + l;
+        const double l = fma_rn(a, b, -h);
+        const double q = fma_rn(h, pinv, 6755399441055744.0) - 6755399441055744.0;
+        return fma_rn(-q, p, h) + l;
     }
     // a * s for an integer s with |a * s| < 2^52: the product is exact, so one rounding step
     // (q = round(x / p), x - q * p, exact in the FMA) reduces it; 4 operations instead of 6.
-    __device__ __forceinline__ T mul_small(T a, double s) const {
+    MPS_HD T mul_small(T a, double s) const {
         const double x = a * s;
-        const double q = __fma_rn(x, pinv, 6755399441055744.0) - 6755399441055744.0;
-        return __fma_rn(-q, p, x);
+        const double q = fma_rn(x, pinv, 6755399441055744.0) - 6755399441055744.0;
+        return fma_rn(-q, p, x);
     }
-    __device__ __forceinline__ T dbl(T a) const {
+    MPS_HD T dbl(T a) const {
         double r = a + a;
         r = r >= p ? r - p : r;
         return r <= -p ? r + p : r;
     }
-    __device__ __forceinline__ T r2() const { return 1.0; }
-    __device__ __forceinline__ T from(uint64_t x, T) const { return (double)x; }
+    MPS_HD T r2() const { return 1.0; }
+    MPS_HD T from(uint64_t x, T) const { return (double)x; }
     // canonical residue as an integer: adding 2^52 puts the integer in the mantissa bits
-    __device__ __forceinline__ uint64_t key(T a) const {
+    MPS_HD uint64_t key(T a) const {
         const double c = a < 0.0 ? a + p : a;
-        return (uint64_t)__double_as_longlong(c + 4503599627370496.0) & 0x000FFFFFFFFFFFFFULL;
+        return dbits(c + 4503599627370496.0) & 0x000FFFFFFFFFFFFFULL;
     }
     // A 32-bit word of key(a) for hash tables: the low word of key(a) times an odd constant (a
     // bijection of the low word, so equal words mean equal low words).  Plain residues have structure
@@ -240,19 +310,23 @@ struct F64Field {
     // out because the compiler turns the same expression into a compare, a move and a select).
     // The modulus must be the same in every lane of the wavefront (k_bsgs, one entry per
     // workgroup, is the only caller): plc is an SGPR operand of the select.
-    __device__ __forceinline__ uint32_t hash_word(T a) const {
-        const uint64_t x = (uint64_t)__double_as_longlong(a + 6755399441055744.0);
+    MPS_HD uint32_t hash_word(T a) const {
+        const uint64_t x = dbits(a + 6755399441055744.0);
         uint32_t corr;  // plc when a < 0, else 0
+#if defined(__HIP_DEVICE_COMPILE__)
         asm("v_bfe_i32 %0, %1, 19, 1\n\tv_bfi_b32 %0, %0, 0, %2" : "=&v"(corr) : "v"((uint32_t)(x >> 32)), "s"(plc));
+#else
+        corr = ((x >> 51) & 1) ? 0u : plc;  // the two instructions' result, for the CPU tests
+#endif
         return (uint32_t)x * kHashMul + corr;
     }
-    __device__ __forceinline__ uint64_t aux() const { return (uint64_t)__double_as_longlong(pinv); }
-    __device__ __forceinline__ static uint64_t to_bits(T a) { return (uint64_t)__double_as_longlong(a); }
-    __device__ __forceinline__ static T from_bits(uint64_t a) { return __longlong_as_double((long long)a); }
+    MPS_HD uint64_t aux() const { return dbits(pinv); }
+    MPS_HD static uint64_t to_bits(T a) { return dbits(a); }
+    MPS_HD static T from_bits(uint64_t a) { return dfrom(a); }
 };
 
 template <class F>
-__device__ __forceinline__ typename F::T fpow(const F& f, typename F::T base, uint64_t e) {
+MPS_HD typename F::T fpow(const F& f, typename F::T base, uint64_t e) {
     typename F::T r = f.one();
     while (e) {
         if (e & 1) r = f.mul(r, base);
@@ -264,7 +338,7 @@ __device__ __forceinline__ typename F::T fpow(const F& f, typename F::T base, ui
 
 // x^e1 and x^e2 in one loop: the squarings of x are shared, the two products are independent.
 template <class F>
-__device__ __forceinline__ void fpow_2e(const F& f, typename F::T x, uint64_t e1, uint64_t e2, typename F::T& r1,
+MPS_HD void fpow_2e(const F& f, typename F::T x, uint64_t e1, uint64_t e2, typename F::T& r1,
                                         typename F::T& r2) {
     r1 = f.one();
     r2 = f.one();
@@ -282,7 +356,7 @@ __device__ __forceinline__ void fpow_2e(const F& f, typename F::T x, uint64_t e1
 // stays a small integer, so that product is exact).  Any integer of the residue class will do
 // for sa and sb (a negative -c stands for p - c).
 template <class F>
-__device__ __forceinline__ void fpow2_small(const F& f, double sa, double sb, uint64_t e, double& ra, double& rb) {
+MPS_HD void fpow2_small(const F& f, double sa, double sb, uint64_t e, double& ra, double& rb) {
     ra = 1.0;
     rb = 1.0;
     if (e == 0) return;
@@ -301,7 +375,7 @@ __device__ __forceinline__ void fpow2_small(const F& f, double sa, double sb, ui
 // a^e and b^e in one loop (the exponent's bits are walked once for both); E is uint32_t or
 // uint64_t.
 template <class F, class E>
-__device__ __forceinline__ void fpow2(const F& f, typename F::T a, typename F::T b, E e, typename F::T& ra,
+MPS_HD void fpow2(const F& f, typename F::T a, typename F::T b, E e, typename F::T& ra,
                                       typename F::T& rb) {
     ra = f.one();
     rb = f.one();
@@ -319,13 +393,13 @@ __device__ __forceinline__ void fpow2(const F& f, typename F::T a, typename F::T
 }
 
 template <class F>
-__device__ __forceinline__ bool fequal(const F& f, typename F::T a, typename F::T b) {
+MPS_HD bool fequal(const F& f, typename F::T a, typename F::T b) {
     return f.key(a) == f.key(b);
 }
 
 // 2^(p-1) == 1 (mod p) by left-to-right square-and-double.
 template <class F>
-__device__ __forceinline__ bool fermat2(const F& f, uint64_t p) {
+MPS_HD bool fermat2(const F& f, uint64_t p) {
     const uint64_t e = p - 1;
     int k = 63 - __builtin_clzll(e);
     typename F::T x = f.one();

@@ -94,21 +94,46 @@ bool class_mode(const mps_item* it, bool classes_on, int class_policy, uint64_t&
     return (class_policy == 1 || (cm & 1) == 0) && Q <= 48 && span / Q >= 4096;
 }
 
+// Distinct primes of Qo and their powers in Qo, smallest first.  False when the kernels cannot
+// take Qo: a prime or a prime power of 2^16 or more, or more than kOrderMaxPrimes primes.
+bool order_factors(uint64_t Qo, uint32_t* opr, uint32_t* oprpow, uint32_t& nopr) {
+    nopr = 0;
+    uint64_t q = Qo;
+    for (uint32_t r = 2; q > 1 && r < 65536; ++r) {
+        if (q % r) continue;
+        uint32_t ra = 1;
+        while (q % r == 0) {
+            q /= r;
+            if ((uint64_t)ra * r >= 65536) return false;
+            ra *= r;
+        }
+        if (nopr == kOrderMaxPrimes) return false;
+        opr[nopr] = r;
+        oprpow[nopr++] = ra;
+    }
+    return q == 1;
+}
+
 // Order mode (class policy 2, classes on): one class per prime from the power residues of b and t
-// modulo Qo = lcm(cm, kOrderQ), when Qo fits 32 bits.
+// modulo Qo = lcm(cm, B), B the longest prefix product of kOrderQ and kOrderMorePrimes that keeps
+// Qo below 2^52.  A class modulus whose Qo the kernels cannot take (order_factors) leaves the item
+// to class or plain mode.
 bool order_mode(const mps_item* it, bool classes_on, int class_policy, uint64_t& Qo) {
     const uint64_t cm = it->cls_mod == 0 ? 1 : it->cls_mod;
     auto lcm_cm = [cm](uint64_t b) { return (unsigned __int128)(cm / gcd_u64(cm, b)) * b; };
     Qo = 0;
     uint64_t B = kOrderQ;
     if (lcm_cm(B) >= kOrderQLimit) return false;
-    Qo = (uint64_t)lcm_cm(B);
+    uint64_t q = (uint64_t)lcm_cm(B);
     for (uint32_t r : kOrderMorePrimes) {  // the longest prefix that keeps Qo below 2^52
         B *= r;
-        const unsigned __int128 q = lcm_cm(B);
-        if (q >= kOrderQLimit) break;
-        Qo = (uint64_t)q;
+        const unsigned __int128 qn = lcm_cm(B);
+        if (qn >= kOrderQLimit) break;
+        q = (uint64_t)qn;
     }
+    uint32_t opr[kOrderMaxPrimes], oprpow[kOrderMaxPrimes], nopr = 0;
+    if (!order_factors(q, opr, oprpow, nopr)) return false;
+    Qo = q;
     return classes_on && class_policy == 2 && it->N0 >= 0 && it->N1 > it->N0;
 }
 
@@ -164,23 +189,8 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
         ip.Qo = Qo;
         ip.Qo_inv = 1.0 / (double)Qo;
         ip.n0modQo = N0 % Qo;
-        uint64_t q = Qo;
-        for (uint32_t r = 2; q > 1 && r < 65536 && ip.nopr < kOrderMaxPrimes; ++r) {
-            if (q % r) continue;
-            uint32_t ra = 1;
-            while (q % r == 0) {
-                q /= r;
-                ra *= r;
-            }
-            if (ra >= 65536) {
-                why = "class modulus with a prime power above 2^16";
-                return false;
-            }
-            ip.opr[ip.nopr] = r;
-            ip.oprpow[ip.nopr++] = ra;
-        }
-        if (q > 1) {
-            why = "class modulus with a prime factor above 2^16 or too many prime factors";
+        if (!order_factors(Qo, ip.opr, ip.oprpow, ip.nopr)) {  // order_mode checked them
+            why = "order mode: class modulus the kernels cannot take";
             return false;
         }
         for (uint32_t j = 0; j < ip.nopr; ++j) {
@@ -331,7 +341,7 @@ bool ensure_buffers(mps_gpu* g, uint64_t odds) {
     if (g->buf_odds >= odds) return true;
     free_buffers(g);
     const uint64_t ent_cap = std::max<uint64_t>(odds / 8, 4096);
-    const uint64_t out_cap = std::max<uint64_t>(odds / 16, 4096);  // records of 2 words
+    const uint64_t out_cap = record_cap(odds);  // records of 2 words
     HIPCHK(hipMalloc(&g->pb.surv, odds * sizeof(uint32_t)), "surv alloc");
     HIPCHK(hipMalloc(&g->pb.cand, odds * sizeof(uint64_t)), "cand alloc");
     HIPCHK(hipMalloc(&g->pb.ent, ent_cap * kEntWords * sizeof(uint64_t)), "entry alloc");

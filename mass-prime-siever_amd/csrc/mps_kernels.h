@@ -26,6 +26,11 @@ constexpr uint32_t kOrderMorePrimes[5] = {19, 23, 29, 31, 37};
 constexpr uint64_t kOrderQLimit = 1ull << 52;
 constexpr uint32_t kOrderMaxPrimes = 16;          // distinct primes of Qo
 constexpr uint64_t kHostDecides = ~0ull;          // M of a record the host must decide exactly
+// Relation records a pass of `odds` odd numbers can produce: every prime of the pass in the worst
+// case (an item whose b^n + c vanishes for some in-class n has a relation at every prime), and
+// primes are at most 0.29 of the odd numbers above 1000 (2 / ln 1000); 4096 more cover the primes
+// below 1000, which the host decides.
+inline uint64_t record_cap(uint64_t odds) { return odds / 3 + 4096; }
 
 // Entry layout (u64 words).  B is the baby-step base, Bm = B^m the giant-step base.
 enum EntryWord : uint32_t {
@@ -148,7 +153,7 @@ size_t bsgs_lds_bytes(const TableGeom& tg);
 
 // Miller-Rabin over the pass's relation records (after launch_bsgs, same stream): composite
 // numbers are replaced by 0.
-constexpr uint32_t kPrimeRecordBlocks = 8;
+constexpr uint32_t kPrimeRecordBlocks = 256;  // one workgroup per CU: a pass may have a record per prime
 void launch_prime_records(hipStream_t s, const PassBuffers& pb);
 void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const PassDesc* d, const PassBuffers& pb,
                  uint32_t n_ent, uint32_t grid);

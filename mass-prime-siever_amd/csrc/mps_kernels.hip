@@ -351,13 +351,6 @@ __global__ __launch_bounds__(256) void k_filter(const PassDesc* __restrict__ d, 
 
 // ------------------------------------------------------------------------------------- setup
 
-__device__ __forceinline__ uint64_t isqrt_ceil64(uint64_t x) {
-    if (x == 0) return 0;
-    uint64_t r = (uint64_t)sqrt((double)x);
-    while (r > 0 && r * r > x) --r;
-    while ((r + 1) * (r + 1) <= x) ++r;
-    return r * r < x ? r + 1 : r;
-}
 
 // Writes the powers tables of base x: tab[k] = first * x^k and tab[8 + k] = x^(8k), k < 8; returns
 // x^64.  A lane's start first * x^tid then takes one product of two table words (two with x^64).
@@ -381,7 +374,6 @@ __device__ __forceinline__ typename F::T write_pow_tables(const F& f, typename F
     return c;  // x^64
 }
 
-__device__ __forceinline__ uint32_t bitlen64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
 
 // x^L from x^64 for L = 64, 128 or 256 lanes.
 template <class F>
@@ -391,12 +383,6 @@ __device__ __forceinline__ typename F::T lane_stride_pow(const F& f, typename F:
     return r;
 }
 
-// Quotient of an exact division x / d (d odd): x times the inverse of d modulo 2^64.
-__device__ __forceinline__ uint64_t exact_div_odd(uint64_t x, uint64_t d) {
-    uint64_t inv = d;  // d * d == 1 (mod 8): correct to 3 bits, doubled by each Newton step
-    for (int k = 0; k < 5; ++k) inv *= 2 - d * inv;
-    return x * inv;
-}
 
 // Order mode: with g = gcd(Qo, p - 1) and e = (p - 1) / g, b^n == t forces (b^e)^n == t^e in the
 // group of g-th roots of unity.  For each prime power r^a of g (Pohlig-Hellman), x = (b^e)^(g/r^a)
@@ -437,39 +423,11 @@ __device__ __forceinline__ uint64_t order_g(uint64_t p, const ItemParams& ip, ui
     return g;
 }
 
-// a mod m for a, m below 2^16 (m >= 1): the float quotient is within one of the true one.
-__device__ __forceinline__ uint32_t mod16(uint32_t a, uint32_t m) {
-    const uint32_t q = (uint32_t)((float)a * __frcp_rn((float)m));
-    int32_t r = (int32_t)(a - q * m);
-    if (r < 0) r += (int32_t)m;
-    if (r >= (int32_t)m) r -= (int32_t)m;
-    return (uint32_t)r;
-}
 
-// a mod m and a / m for a and m below 2^52 (m >= 1): both are exact doubles, so the double quotient
-// is within one of the true one.
-__device__ __forceinline__ uint64_t divmod52(uint64_t a, uint64_t m, uint64_t& rem) {
-    uint64_t q = (uint64_t)((double)a / (double)m);
-    int64_t r = (int64_t)(a - q * m);
-    if (r < 0) {
-        r += m;
-        --q;
-    } else if (r >= (int64_t)m) {
-        r -= m;
-        ++q;
-    }
-    rem = (uint64_t)r;
-    return q;
-}
 
-// a * b mod Qo for a < 2^16 and b < Qo < 2^52: the quotient from doubles (a, b exact; the product
-// and the scaling by 1/Qo round) is within one of the true one, and a*b - q*Qo is exact modulo 2^64.
+
 __device__ __forceinline__ uint64_t mulmod_qo(uint64_t a, uint64_t b, const ItemParams& ip) {
-    const uint64_t q = (uint64_t)((double)a * (double)b * ip.Qo_inv);
-    int64_t r = (int64_t)(a * b - q * ip.Qo);
-    if (r < 0) r += (int64_t)ip.Qo;
-    if (r >= (int64_t)ip.Qo) r -= (int64_t)ip.Qo;
-    return (uint64_t)r;
+    return mulmod_q52(a, b, ip.Qo, ip.Qo_inv);
 }
 
 // Order mode, one round of 256 candidates (one per thread) as work lists: phase 1, a lane per

@@ -447,7 +447,9 @@ static size_t g_nbase = 0;
 static uint64_t g_base_limit = 0;
 static pthread_mutex_t g_base_mu = PTHREAD_MUTEX_INITIALIZER;
 
-/* Odd primes up to limit (inclusive), by an odd-only sieve of Eratosthenes; cached. */
+/* Odd primes up to limit (inclusive), cached: sieve_small_primes (Sieve.hpp:13-39) gives the same
+ * list.  The odd numbers are sieved in segments of 2^18 by the odd primes up to sqrt(limit), which
+ * come from a plain odd-only sieve (limit < 2^32, so they are below 2^16). */
 static void ensure_base_primes(uint64_t limit) {
     pthread_mutex_lock(&g_base_mu);
     if (g_base != NULL && g_base_limit >= limit) {
@@ -455,20 +457,45 @@ static void ensure_base_primes(uint64_t limit) {
         return;
     }
     if (limit < 65536) limit = 65536;
-    const uint64_t nodd = limit / 2 + 1; /* index i <-> 2i+1 */
-    uint8_t* comp = (uint8_t*)calloc(nodd, 1);
-    for (uint64_t i = 1; (2 * i + 1) * (2 * i + 1) <= limit; ++i) {
-        if (comp[i]) continue;
+    uint64_t r = 1;
+    while ((r + 1) * (r + 1) <= limit) ++r;
+    const uint64_t rodd = r / 2 + 1;
+    uint8_t* rc = (uint8_t*)calloc(rodd, 1);
+    uint32_t* rp = (uint32_t*)malloc(rodd * sizeof(uint32_t));
+    size_t nrp = 0;
+    for (uint64_t i = 1; i < rodd; ++i) {
+        if (rc[i]) continue;
         const uint64_t q = 2 * i + 1;
-        for (uint64_t j = (q * q) / 2; j < nodd; j += q) comp[j] = 1;
+        rp[nrp++] = (uint32_t)q;
+        for (uint64_t j = (q * q) / 2; j < rodd; j += q) rc[j] = 1;
     }
-    size_t cnt = 0;
-    for (uint64_t i = 1; i < nodd && 2 * i + 1 <= limit; ++i) cnt += !comp[i];
-    uint32_t* pr = (uint32_t*)malloc(cnt * sizeof(uint32_t));
-    size_t k = 0;
-    for (uint64_t i = 1; i < nodd && 2 * i + 1 <= limit; ++i)
-        if (!comp[i]) pr[k++] = (uint32_t)(2 * i + 1);
+    free(rc);
+    size_t cap = 1 << 16, cnt = 0;
+    uint32_t* pr = (uint32_t*)malloc(cap * sizeof(uint32_t));
+    const uint64_t nodd = limit / 2 + 1; /* index i <-> 2i+1, i = 1 .. */
+    const uint64_t seg = 1u << 18;
+    uint8_t* comp = (uint8_t*)malloc(seg);
+    for (uint64_t s0 = 1; s0 < nodd; s0 += seg) {
+        const uint64_t len = nodd - s0 < seg ? nodd - s0 : seg;
+        memset(comp, 0, len);
+        for (size_t k = 0; k < nrp; ++k) {
+            const uint64_t q = rp[k];
+            uint64_t j = (q * q) / 2; /* index of q^2 */
+            if (j >= s0 + len) break;
+            if (j < s0) j += ((s0 - j + q - 1) / q) * q;
+            for (; j < s0 + len; j += q) comp[j - s0] = 1;
+        }
+        for (uint64_t i = 0; i < len; ++i) {
+            if (comp[i] || 2 * (s0 + i) + 1 > limit) continue;
+            if (cnt == cap) {
+                cap *= 2;
+                pr = (uint32_t*)realloc(pr, cap * sizeof(uint32_t));
+            }
+            pr[cnt++] = (uint32_t)(2 * (s0 + i) + 1);
+        }
+    }
     free(comp);
+    free(rp);
     free(g_base);
     g_base = pr;
     g_nbase = cnt;

@@ -134,8 +134,9 @@ def test_order_mode_small_base_edge(gpu, mps, oracle, it):
 
 
 # Class moduli past the primes and prime powers of 367567200 = 2^5 3^3 5^2 7 11 13 17: 38 = 2 * 19
-# gives Qo an eighth prime, 81 = 3^4 a finer 3-part than Qo's own, and 46 = 2 * 23 takes Qo past
-# 2^32 (the item then runs in class mode).  Order mode runs one work-list pair per prime of g.
+# and 46 = 2 * 23 bring in primes that the prefix products of Qo (x 19, 23, 29, 31 up to 2^52)
+# hold as well, and 81 = 3^4 a finer 3-part than Qo's own.  Order mode runs one work-list pair per
+# prime of g.
 ORDER_CM_ITEMS = [
     (2, -25, 10**6, 10**6 + 2**21, 3_600_000, 5_000_000, 38, 5),
     (2, -25, 10**6, 10**6 + 2**21, 3_600_000, 5_000_000, 81, 7),
@@ -247,3 +248,24 @@ def test_small_order_primes_left_to_host(gpu, mps, oracle, policy):
     assert rels == want
     got = {p: M for p, M, q in rels}
     assert all(got[p] == o for p, o in SMALL_ORDER.items())
+
+
+# Class moduli whose Qo order mode cannot take: a prime factor of 2^16 or more (2 * 65537) or a
+# prime power of 2^16 or more (2^16, 3^11).  Under class policy 2 such an item runs in class or
+# plain mode like any other; it must not be declined.
+UNSUPPORTED_CM = [(2 * 65537, 3), (65536, 5), (3**11, 7)]
+
+
+@pytest.mark.parametrize("mod,res", UNSUPPORTED_CM)
+def test_order_mode_unsupported_modulus_falls_back(gpu, mps, oracle, mod, res):
+    P0, P1 = 10**6, 10**6 + 2**21
+    item = (2, -25, P0, P1, 0, 10_000_000, mod, res)
+    print(f"policy 2, class modulus {mod}", flush=True)
+    gpu.set_class_policy(2)
+    try:
+        rels, st = gpu.run_item(mps.work_item(*item), chunk_ints=1 << 20)
+    finally:
+        gpu.set_class_policy(1)
+    want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 16)
+    assert st.declined == 0
+    assert want and rels == want

@@ -100,3 +100,21 @@ def test_cli_without_gpu_fails_loudly(tmp_path, mps):
     r = subprocess.run([binary], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode == 1
     assert "GPU disabled" in (tmp_path / "siever.log").read_text()
+
+
+def test_dropin_adapter_compiles_and_declines_without_gpu(tmp_path):
+    """include/mps_gpu_siever.hpp, the adapter a reference host includes in place of Gpu.hpp,
+    compiles with -Wall -Werror against a WorkItem with 128-bit fields (Types.hpp:14-26) and links
+    against libmps.so; without a usable device it reports 'GPU disabled' and stops (GPU_strict=1)."""
+    import subprocess
+    exe = str(tmp_path / "dropin_check")
+    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                    os.path.join(HERE, "dropin_check.cpp"), "-o", exe,
+                    "-L" + os.path.join(ROOT, "mass-prime-siever_amd"), "-lmps", "-L" + os.path.join(ROOT, "oracle"),
+                    "-loracle", "-Wl,-rpath," + os.path.join(ROOT, "mass-prime-siever_amd"),
+                    "-Wl,-rpath," + os.path.join(ROOT, "oracle"), "-lpthread"], check=True)
+    r = subprocess.run([exe, "sieve", "2", "-25", "1000000000", "1001048576", "3600000", "5000000", "2", "1"],
+                       capture_output=True, text=True, timeout=120)
+    if "GPU enabled" in r.stderr:
+        pytest.skip("a HIP device is present")
+    assert r.returncode == 1 and "GPU disabled" in r.stderr
