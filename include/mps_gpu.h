@@ -87,6 +87,19 @@ typedef struct mps_dev_relation {
 int mps_gpu_sieve_relations(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, mps_dev_relation* out,
                             size_t out_cap, size_t* out_n);
 
+/* mps_gpu_sieve_relations without allocation.  No counterpart in the reference: this is for
+ * callers that cannot allocate device or pinned memory during a call (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid it).  The range [lo|1, hi) holds at most
+ * MPS_NOALLOC_MAX_INTS integers (larger ranges: MPS_ERROR, nothing runs) and runs as one device
+ * pass on a scratch array of about 46 MB that the library keeps per GPU for the life of the
+ * process (set up at mps_gpu_create); the pass's stage counters, survivors, candidates, BSGS
+ * entries and records live there.  Results are those of mps_gpu_sieve_relations on the same
+ * range.  Calls on one handle are serialised; calls through different handles on the same GPU
+ * share its scratch array and must not overlap. */
+#define MPS_NOALLOC_MAX_INTS (1ull << 20)
+int mps_gpu_sieve_relations_noalloc(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item,
+                                    mps_dev_relation* out, size_t out_cap, size_t* out_n);
+
 /* GpuSiever::filter (Gpu.hpp:1058-1061, OpenClSiever::filter Gpu.hpp:726-798): the primes are
  * given by the caller.  Writes the ascending indices i with primes[i] possibly dividing an
  * in-class b^n + c (or left to the host) to todo[]. */

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -40,7 +41,7 @@ using namespace mps;
 
 namespace {
 
-constexpr size_t kCszBytes = 49 * 32 * sizeof(uint2);  // class-size table of one item
+constexpr size_t kCszBytes = kCszEntries * sizeof(uint2);  // class-size table of one item
 constexpr uint32_t kFastRecords = 8192;                 // records copied back inside the chain
 constexpr uint32_t kDefaultTables = 1024;               // table arena without a memory budget
 
@@ -55,7 +56,7 @@ struct BItem {
     ItemParams ip{};
     uint64_t m_max = 0;
     TableGeom tg{};
-    std::vector<uint2> csz;
+    std::array<uint2, kCszEntries> csz;
     int table = -1;  // arena slot of the class-size table while admitted
     std::vector<std::pair<uint64_t, uint64_t>> chunks;
     size_t next = 0, running = 0;  // chunks handed out, chunks on a slot now
@@ -247,7 +248,7 @@ void admit(mps_batch* b) {
                 it->ip = pl.ip;
                 it->m_max = pl.m_max;
                 it->tg = pl.tg;
-                it->csz = std::move(pl.csz);
+                it->csz = pl.csz;
             }
         }
         if (!it->chunks.empty()) {

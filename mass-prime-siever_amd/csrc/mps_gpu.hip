@@ -79,6 +79,69 @@ void class_size(uint64_t K, uint32_t s, uint32_t L, uint32_t maxm, uint32_t& m, 
     }
 }
 
+// ------------------------------------------------------------------ allocation-free scratch
+//
+// mps_gpu_sieve_relations_noalloc runs one pass of at most kScratchOdds odd numbers on buffers in
+// `scratch`, a file-scope device array (one copy per GPU, held until the process ends): stage
+// counters, survivors, candidates, BSGS entries and records, laid out at 256-byte boundaries with
+// the capacities the allocating path gives a pass of the same size.  Host code never uses the
+// array's name as an address: each device's address comes from hipGetSymbolAddress at
+// mps_gpu_create and is kept per device.
+constexpr uint64_t kScratchOdds = MPS_NOALLOC_MAX_INTS / 2;
+constexpr uint64_t kScratchEnt = kScratchOdds / 8;
+constexpr uint64_t kScratchRec = mps::record_cap(kScratchOdds);
+constexpr size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t kScrCnt = 0;
+constexpr size_t kScrSurv = align256(kScrCnt + sizeof(mps::Counters));
+constexpr size_t kScrCand = align256(kScrSurv + kScratchOdds * sizeof(uint32_t));
+constexpr size_t kScrEnt = align256(kScrCand + kScratchOdds * sizeof(uint64_t));
+constexpr size_t kScrRec = align256(kScrEnt + kScratchEnt * mps::kEntWords * sizeof(uint64_t));
+constexpr size_t kScratchBytes = align256(kScrRec + kScratchRec * 2 * sizeof(uint64_t));
+static_assert(kScratchBytes < (64ull << 20), "the scratch array stays modest");
+constexpr size_t kStageRecords = 4096;  // pinned staging for the records, allocated at create
+static_assert(kStageRecords <= kScratchRec, "the first slice is read with the counts");
+constexpr int kMaxScratchDevices = 64;
+
+}  // namespace
+
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+void* g_scratch_dev[kMaxScratchDevices] = {};
+
+// This device's address of `scratch` (the current device must be `device`), looked up once per
+// device; nullptr if the lookup fails.
+void* scratch_address(int device) {
+    if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+    std::lock_guard<std::mutex> l(g_scratch_mu);
+    if (g_scratch_dev[device] == nullptr) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_dev[device] = p;
+    }
+    return g_scratch_dev[device];
+}
+
+// The pass buffers of the scratch array at device address base.
+mps::PassBuffers scratch_buffers(void* base) {
+    uint8_t* b = static_cast<uint8_t*>(base);
+    mps::PassBuffers pb{};
+    pb.vals = nullptr;
+    pb.cnt = reinterpret_cast<mps::Counters*>(b + kScrCnt);
+    pb.surv = reinterpret_cast<uint32_t*>(b + kScrSurv);
+    pb.cand = reinterpret_cast<uint64_t*>(b + kScrCand);
+    pb.ent = reinterpret_cast<uint64_t*>(b + kScrEnt);
+    pb.out = reinterpret_cast<uint64_t*>(b + kScrRec);
+    pb.surv_cap = (uint32_t)kScratchOdds;
+    pb.cand_cap = (uint32_t)kScratchOdds;
+    pb.ent_cap = (uint32_t)kScratchEnt;
+    pb.out_cap = (uint32_t)kScratchRec;
+    pb.need_prime = 1;
+    pb.out_index = 0;
+    return pb;
+}
+
 }  // namespace
 
 namespace mps {
@@ -153,7 +216,7 @@ JacobiPre jacobi_pre(int64_t x) {
 }
 
 bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t cmax,
-                 uint32_t waves, ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why) {
+                 uint32_t waves, ItemParams& ip, uint64_t& m_max, uint2* csz, std::string& why) {
     std::memset(&ip, 0, sizeof(ip));
     ip.b = it->b;
     ip.c = it->c;
@@ -184,7 +247,7 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
     }
     ip.qmode = omode ? kModeOrder : qmode ? kModeClass : kModePlain;
     m_max = 1;
-    csz.assign(49 * 32, make_uint2(1, L));
+    for (uint32_t i = 0; i < kCszEntries; ++i) csz[i] = make_uint2(1, L);
     if (omode) {
         ip.Qo = Qo;
         ip.Qo_inv = 1.0 / (double)Qo;
@@ -289,11 +352,11 @@ bool plan_item(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes
     if (waves == 0) {
         // measured on MI355X: two wavefronts per entry for the class tables (m <= 1024) and short
         // plain ranges; four when plain-mode entries run up to 2048 baby steps
-        if (!item_params(item, classes_on, policy, t.mcap, t.cmax, 2, pl.ip, pl.m_max, pl.csz, why)) return false;
+        if (!item_params(item, classes_on, policy, t.mcap, t.cmax, 2, pl.ip, pl.m_max, pl.csz.data(), why)) return false;
         // order-mode entries are mostly short (their classes are narrow), whatever the item's cap
         waves = pl.m_max > 1024 && pl.ip.qmode != kModeOrder ? 4 : 2;
     }
-    if (!item_params(item, classes_on, policy, t.mcap, t.cmax, waves, pl.ip, pl.m_max, pl.csz, why)) return false;
+    if (!item_params(item, classes_on, policy, t.mcap, t.cmax, waves, pl.ip, pl.m_max, pl.csz.data(), why)) return false;
     pl.tg = table_geom(pl.m_max, waves, t.load);
     if (bsgs_lds_bytes(pl.tg) > kMaxBsgsLds) {
         why = "BSGS table would exceed the LDS budget";
@@ -361,8 +424,8 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-bool read_counts(mps_gpu* g) {
-    HIPCHK(hipMemcpyAsync(g->h_cnt, g->pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, g->stream), "count read");
+bool read_counts(mps_gpu* g, const PassBuffers& pb) {
+    HIPCHK(hipMemcpyAsync(g->h_cnt, pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, g->stream), "count read");
     HIPCHK(hipStreamSynchronize(g->stream), "stream sync");
     return true;
 }
@@ -371,12 +434,12 @@ bool read_counts(mps_gpu* g) {
 // are fetched by read_records).
 constexpr uint32_t kRecPrefix = 1024;
 
-// Counts and the first kRecPrefix records of the pass just enqueued, one wait.
-bool read_counts_and_records(mps_gpu* g) {
-    const size_t pre = std::min<size_t>(kRecPrefix, g->pb.out_cap);
-    HIPCHK(hipMemcpyAsync(g->h_cnt, g->pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, g->stream), "count read");
-    HIPCHK(hipMemcpyAsync(g->h_out, g->pb.out, 2 * pre * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream),
-           "record read");
+// Counts and the first `pre` records of the pass just enqueued into h_rec, one wait.
+bool read_counts_and_records(mps_gpu* g, const PassBuffers& pb, uint64_t* h_rec, size_t pre) {
+    HIPCHK(hipMemcpyAsync(g->h_cnt, pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, g->stream), "count read");
+    if (pre)
+        HIPCHK(hipMemcpyAsync(h_rec, pb.out, 2 * pre * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream),
+               "record read");
     HIPCHK(hipStreamSynchronize(g->stream), "stream sync");
     return true;
 }
@@ -400,36 +463,37 @@ uint32_t blocks_for(const mps_gpu* g, uint64_t n, uint32_t per_block) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
 }
 
-// One pass through the stage chain.  stop_stage: 0 after the sieve, 1 after the filter,
-// 2 after setup, 3 = full.  Leaves the final counts in g->h_cnt.
-bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, FieldKind fk, const TableGeom& tg,
-              int stop_stage) {
+// One pass through the stage chain on the buffers pb.  stop_stage: 0 after the sieve, 1 after the
+// filter, 2 after setup, 3 = full.  Leaves the final counts in g->h_cnt and, after a full pass,
+// the first rec_prefix records in h_rec.
+bool run_pass(mps_gpu* g, const PassBuffers& pb, uint64_t* h_rec, size_t rec_prefix, uint64_t plo, uint64_t pc,
+              const ItemParams& ip, FieldKind fk, const TableGeom& tg, int stop_stage) {
     hipStream_t s = g->stream;
     g->h_desc->lo = plo;
     g->h_desc->nodd = pc;
     g->h_desc->ip = ip;
     g->h_desc->tg = tg;
-    HIPCHK(hipMemcpyAsync(g->d_csz, g->h_csz, 49 * 32 * sizeof(uint2), hipMemcpyHostToDevice, s), "size table");
+    HIPCHK(hipMemcpyAsync(g->d_csz, g->h_csz, kCszEntries * sizeof(uint2), hipMemcpyHostToDevice, s), "size table");
     HIPCHK(hipMemcpyAsync(g->d_desc, g->h_desc, sizeof(PassDesc), hipMemcpyHostToDevice, s), "pass descriptor");
     const PassDesc* d = g->d_desc;
-    launch_reset(s, g->pb.cnt, g->pb.vals == nullptr ? 0u : (uint32_t)pc);
+    launch_reset(s, pb.cnt, pb.vals == nullptr ? 0u : (uint32_t)pc);
     HIPCHK(hipEventRecord(g->ev[0], s), "event");
-    if (g->pb.vals == nullptr) launch_sieve(s, d, pc, g->d_sp, g->d_sp_r32, g->nsp, g->nsmall, g->pb, 0);
+    if (pb.vals == nullptr) launch_sieve(s, d, pc, g->d_sp, g->d_sp_r32, g->nsp, g->nsmall, pb, 0);
     HIPCHK(hipGetLastError(), "sieve launch");
     HIPCHK(hipEventRecord(g->ev[1], s), "event");
     if (stop_stage == 0) {
-        if (!read_counts(g)) return false;
+        if (!read_counts(g, pb)) return false;
         g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
         return true;
     }
     // The filter and setup kernels read their element counts on the device, so their grids are
     // sized for the largest count and nothing is read back until the BSGS grid needs the entry
     // count.
-    launch_filter(s, d, g->pb, blocks_for(g, pc, 256));
+    launch_filter(s, d, pb, blocks_for(g, pc, 256));
     HIPCHK(hipGetLastError(), "filter launch");
     HIPCHK(hipEventRecord(g->ev[2], s), "event");
     if (stop_stage == 1) {
-        if (!read_counts(g)) return false;
+        if (!read_counts(g, pb)) return false;
         g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
         g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
         return true;
@@ -437,11 +501,11 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     // class mode sorts tiles of kSetupTile candidates, a workgroup a tile at a time; order mode runs
     // rounds of 256
     const uint32_t setup_grid = blocks_for(g, pc, ip.qmode == kModeClass ? kSetupTile : 256);
-    launch_setup(s, fk, ip.qmode, d, g->pb, setup_grid);
+    launch_setup(s, fk, ip.qmode, d, pb, setup_grid);
     HIPCHK(hipGetLastError(), "setup launch");
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
     if (stop_stage == 2) {
-        if (!read_counts(g)) return false;
+        if (!read_counts(g, pb)) return false;
         g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
         g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
         g->st.ms_setup += elapsed(g->ev[2], g->ev[3]);
@@ -449,14 +513,14 @@ bool run_pass(mps_gpu* g, uint64_t plo, uint64_t pc, const ItemParams& ip, Field
     }
     // BSGS over the entries setup wrote, on a grid of several times the resident workgroups that
     // reads the entry count on the device (no read-back before it)
-    launch_bsgs(s, fk, tg, d, g->pb, 0, bsgs_persistent_grid(fk, tg, g->num_cu));
+    launch_bsgs(s, fk, tg, d, pb, 0, bsgs_persistent_grid(fk, tg, g->num_cu));
     HIPCHK(hipGetLastError(), "bsgs launch");
     HIPCHK(hipEventRecord(g->ev[4], s), "event");
     // the records' Miller-Rabin test reads the record count on the device, so it is enqueued now
     // and the counts and records come back under one wait
-    launch_prime_records(s, g->pb);
+    launch_prime_records(s, pb);
     HIPCHK(hipGetLastError(), "record test launch");
-    if (!read_counts_and_records(g)) return false;
+    if (!read_counts_and_records(g, pb, h_rec, rec_prefix)) return false;
     g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
     g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
     g->st.ms_setup += elapsed(g->ev[2], g->ev[3]);
@@ -533,11 +597,13 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
          hipMemcpy(g->d_sp, sp.data(), sp.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(g->d_sp_r32, r32.data(), r32.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
          hipMalloc(&g->pb.cnt, sizeof(Counters)) == hipSuccess &&
-         hipMalloc(&g->d_csz, 49 * 32 * sizeof(uint2)) == hipSuccess &&
+         hipMalloc(&g->d_csz, kCszEntries * sizeof(uint2)) == hipSuccess &&
          hipMalloc(&g->d_desc, sizeof(PassDesc)) == hipSuccess &&
          hipHostMalloc((void**)&g->h_desc, sizeof(PassDesc), hipHostMallocDefault) == hipSuccess &&
-         hipHostMalloc((void**)&g->h_csz, 49 * 32 * sizeof(uint2), hipHostMallocDefault) == hipSuccess &&
-         hipHostMalloc((void**)&g->h_cnt, sizeof(Counters), hipHostMallocDefault) == hipSuccess;
+         hipHostMalloc((void**)&g->h_csz, kCszEntries * sizeof(uint2), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc((void**)&g->h_cnt, sizeof(Counters), hipHostMallocDefault) == hipSuccess &&
+         hipHostMalloc((void**)&g->h_stage, kStageRecords * 2 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess;
+    if (ok) g->scratch = scratch_address(device_index);  // with the device current (hipSetDevice above)
     if (!ok) {
         write_str(banner, banner_cap, "GPU disabled: HIP setup failed");
         mps_gpu_destroy(g);
@@ -616,7 +682,8 @@ int sieve_records(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, st
     for (uint64_t pb0 = 0; pb0 < count; pb0 += g->pass_odds) {
         const uint64_t pc = std::min(g->pass_odds, count - pb0);
         const uint64_t plo = lo + 2 * pb0;
-        if (!run_pass(g, plo, pc, ip, fk, tg, 3)) return MPS_ERROR;
+        if (!run_pass(g, g->pb, g->h_out, std::min<size_t>(kRecPrefix, g->pb.out_cap), plo, pc, ip, fk, tg, 3))
+            return MPS_ERROR;
         accumulate(g, pc);
         const uint32_t n_out = g->h_cnt->n_out;
         if (n_out > g->pb.out_cap) {
@@ -672,6 +739,68 @@ int mps_gpu_sieve_relations(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item
     return n > out_cap ? MPS_CAPACITY : MPS_OK;
 }
 
+int mps_gpu_sieve_relations_noalloc(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item,
+                                    mps_dev_relation* out, size_t out_cap, size_t* out_n) {
+    if (out_n) *out_n = 0;
+    if (g == nullptr || item == nullptr || out_n == nullptr) return MPS_ERROR;
+    std::lock_guard<std::mutex> lock(g->mu);
+    if (!g->ok || g->broken) return MPS_ERROR;
+    if (g->scratch == nullptr) {
+        g->err = "no-allocation path: the scratch array's device address is unknown";
+        return MPS_ERROR;
+    }
+    if (hi > lo && hi - lo > MPS_NOALLOC_MAX_INTS) {
+        g->err = "no-allocation path: a range of more than 2^20 integers (MPS_NOALLOC_MAX_INTS)";
+        return MPS_ERROR;
+    }
+    if (hi >= (1ull << 62) || lo <= 3) {
+        g->err = "range outside (3, 2^62)";
+        return MPS_DECLINED;
+    }
+    lo |= 1;
+    if (lo >= hi) return MPS_OK;
+    Plan pl;
+    if (!prepare_item(g, item, lo, g->classes_on, pl)) return MPS_DECLINED;
+    const FieldKind fk = pass_field(g, hi);
+    const uint64_t count = (hi - lo + 1) / 2;  // at most kScratchOdds
+    const PassBuffers pb = scratch_buffers(g->scratch);
+    if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
+    if (!run_pass(g, pb, g->h_stage, kStageRecords, lo, count, pl.ip, fk, pl.tg, 3)) return MPS_ERROR;
+    accumulate(g, count);
+    const uint32_t n_rec = g->h_cnt->n_out;
+    if (n_rec > pb.out_cap) {
+        g->err = "device records exceed the scratch record area";
+        return MPS_DECLINED;
+    }
+    // the records through the pinned staging area, kStageRecords at a time; composites (number 0,
+    // k_prime_records) are dropped
+    size_t n = 0;
+    for (uint32_t off = 0; off < n_rec; off += (uint32_t)kStageRecords) {
+        const uint32_t k = std::min<uint32_t>((uint32_t)kStageRecords, n_rec - off);
+        if (off > 0) {  // the first slice came back with the counts
+            hipError_t e = hipMemcpyAsync(g->h_stage, pb.out + 2 * (uint64_t)off, 2 * k * sizeof(uint64_t),
+                                          hipMemcpyDeviceToHost, g->stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+            if (e != hipSuccess) {
+                hip_fail(g, e, "record read");
+                return MPS_ERROR;
+            }
+        }
+        for (uint32_t i = 0; i < k; ++i) {
+            if (g->h_stage[2 * i] == 0) continue;
+            if (n < out_cap && out) {
+                out[n].p = g->h_stage[2 * i];
+                out[n].M = g->h_stage[2 * i + 1];
+            }
+            ++n;
+        }
+    }
+    g->st.field = (int)fk;
+    g->st.tbits = (int)pl.tg.nbbits;
+    *out_n = n;
+    return n > out_cap ? MPS_CAPACITY : MPS_OK;
+}
+
 int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item* item, uint64_t* todo,
                    size_t todo_cap, size_t* n_todo) {
     if (n_todo) *n_todo = 0;
@@ -715,7 +844,7 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
         g->pb.vals = g->d_vals;
         g->pb.need_prime = 0;
         g->pb.out_index = 1;
-        const bool ok = run_pass(g, 0, cnt, ip, fk, tg, 3);
+        const bool ok = run_pass(g, g->pb, g->h_out, std::min<size_t>(kRecPrefix, g->pb.out_cap), 0, cnt, ip, fk, tg, 3);
         g->pb.vals = nullptr;
         g->pb.need_prime = 1;
         g->pb.out_index = 0;
@@ -760,7 +889,8 @@ int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const m
     g->pb.vals = nullptr;
     g->pb.need_prime = 1;
     g->pb.out_index = 0;
-    if (!run_pass(g, lo, count, ip, fk, tg, stage)) return MPS_ERROR;
+    if (!run_pass(g, g->pb, g->h_out, std::min<size_t>(kRecPrefix, g->pb.out_cap), lo, count, ip, fk, tg, stage))
+        return MPS_ERROR;
     const Counters c = *g->h_cnt;
     size_t words = 0, recs = 0;
     if (stage == 0) {
@@ -825,6 +955,7 @@ void mps_gpu_destroy(mps_gpu* g) {
     hipFree(g->d_sp_r32);
     hipFree(g->pb.cnt);
     hipHostFree(g->h_cnt);
+    hipHostFree(g->h_stage);
     hipFree(g->d_csz);
     hipHostFree(g->h_csz);
     hipFree(g->d_desc);

@@ -15,6 +15,7 @@ constexpr uint32_t kMaxClasses = 24;              // classes per prime in class 
 constexpr uint32_t kClassMaxM = 1024;             // default largest baby-step count in class mode
 constexpr size_t kMaxBsgsLds = 64 * 1024;          // LDS of one BSGS workgroup at most
 constexpr uint32_t kBsgsOversubscribe = 16;      // batch engine: BSGS grid / resident workgroups
+constexpr uint32_t kCszEntries = 49 * 32;        // class-size table: (m, giant steps) by qp * 32 + s
 constexpr uint32_t kSetupTile = 1024;             // candidates a class-mode setup workgroup sorts at a time
 // Setup modes (ItemParams.qmode).
 constexpr uint32_t kModePlain = 0, kModeClass = 1, kModeOrder = 2;
@@ -30,7 +31,7 @@ constexpr uint64_t kHostDecides = ~0ull;          // M of a record the host must
 // case (an item whose b^n + c vanishes for some in-class n has a relation at every prime), and
 // primes are at most 0.29 of the odd numbers above 1000 (2 / ln 1000); 4096 more cover the primes
 // below 1000, which the host decides.
-inline uint64_t record_cap(uint64_t odds) { return odds / 3 + 4096; }
+constexpr uint64_t record_cap(uint64_t odds) { return odds / 3 + 4096; }
 
 // Entry layout (u64 words).  B is the baby-step base, Bm = B^m the giant-step base.
 enum EntryWord : uint32_t {

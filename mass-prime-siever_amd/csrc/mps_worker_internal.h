@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <array>
 #include <map>
 #include <mutex>
 #include <string>
@@ -41,6 +42,10 @@ struct mps_gpu {
     size_t vals_cap = 0;
     mps::Counters* h_cnt = nullptr;
     uint64_t* h_out = nullptr;
+    // allocation-free entry point: this device's scratch array (device address from
+    // hipGetSymbolAddress at create) and a pinned record staging area of kStageRecords records
+    void* scratch = nullptr;
+    uint64_t* h_stage = nullptr;
     hipEvent_t ev[6] = {};
     mps_gpu_stats st{};
     std::string err;
@@ -55,14 +60,14 @@ namespace mps {
 // class-mode size table csz[qp * 32 + s] = (m, giant steps per class).
 // Returns false, with the reason, when the BSGS table would not fit.
 bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t cmax,
-                 uint32_t waves, ItemParams& ip, uint64_t& m_max, std::vector<uint2>& csz, std::string& why);
+                 uint32_t waves, ItemParams& ip, uint64_t& m_max, uint2* csz, std::string& why);
 
 // Everything a pass of one item needs besides the range: constants, class-size table, geometry.
 struct Plan {
     ItemParams ip{};
     uint64_t m_max = 0;
     TableGeom tg{};
-    std::vector<uint2> csz;
+    std::array<uint2, kCszEntries> csz;
 };
 
 // Key of the tuning table: class mode and modulus kind, p magnitude (6-bit steps of log2 P0) and

@@ -40,6 +40,7 @@ class DevRelation(ctypes.Structure):
 
 
 HOST_DECIDES = 0xFFFFFFFFFFFFFFFF
+NOALLOC_MAX_INTS = 1 << 20  # MPS_NOALLOC_MAX_INTS (include/mps_gpu.h)
 
 
 class RunOpts(ctypes.Structure):
@@ -124,7 +125,7 @@ def build():
 
 
 EXPORTS = ["mps_gpu_create", "mps_gpu_ok", "mps_gpu_set_classes", "mps_gpu_set_class_policy", "mps_gpu_sieve_filter", "mps_gpu_filter",
-           "mps_gpu_sieve_relations",
+           "mps_gpu_sieve_relations", "mps_gpu_sieve_relations_noalloc",
            "mps_gpu_last_error", "mps_gpu_destroy", "mps_gpu_get_stats", "mps_gpu_reset_stats",
            "mps_gpu_configure", "mps_gpu_debug_stage", "mps_run_item", "mps_results_json", "mps_is_prime64",
            "mps_multiplicative_order", "mps_run_bsgs", "mps_gpu_measure_valu", "mps_gpu_copy_probe", "mps_gpu_autotune", "mps_gpu_get_tuning",
@@ -157,6 +158,8 @@ def lib():
     L.mps_gpu_sieve_filter.argtypes = [vp, u64, u64, P(Item), vp, sz, P(sz)]
     L.mps_gpu_sieve_relations.restype = i32
     L.mps_gpu_sieve_relations.argtypes = [vp, u64, u64, P(Item), vp, sz, P(sz)]
+    L.mps_gpu_sieve_relations_noalloc.restype = i32
+    L.mps_gpu_sieve_relations_noalloc.argtypes = [vp, u64, u64, P(Item), vp, sz, P(sz)]
     L.mps_gpu_filter.restype = i32
     L.mps_gpu_filter.argtypes = [vp, vp, sz, P(Item), vp, sz, P(sz)]
     L.mps_gpu_last_error.restype = ctypes.c_char_p
@@ -287,6 +290,15 @@ class GpuSiever:
             if st != MPS_OK:
                 return None
             return [(out[i].p, out[i].M) for i in range(n.value)]
+
+    def sieve_relations_noalloc(self, lo, hi, item, cap=1 << 14):
+        """mps_gpu_sieve_relations_noalloc: (status, records) for a range of at most
+        NOALLOC_MAX_INTS integers, computed on the library's per-GPU scratch array."""
+        di = device_item(item) if isinstance(item, WorkItem) else item
+        out = (DevRelation * cap)()
+        n = ctypes.c_size_t()
+        st = lib().mps_gpu_sieve_relations_noalloc(self.h, lo, hi, ctypes.byref(di), out, cap, ctypes.byref(n))
+        return st, [(out[i].p, out[i].M) for i in range(min(n.value, cap))], n.value
 
     def filter(self, primes, item, todo):
         """Appends the ascending indices of the primes that may divide an in-class candidate."""

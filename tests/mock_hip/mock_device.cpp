@@ -181,7 +181,7 @@ bool plan_item(const mps_gpu*, const mps_item* item, uint64_t, bool, Plan& pl, s
     pl.ip.qmode = item->cls_mod > 1 ? 1 : 0;
     pl.m_max = 64;
     pl.tg = TableGeom{4, 16, 16, item->cls_mod > 1 ? 2u : 4u};
-    pl.csz.assign(49 * 32, uint2{0, 0});
+    pl.csz.fill(uint2{0, 0});
     pl.csz[0] = uint2{(uint32_t)item->b, (uint32_t)item->c};
     return true;
 }
