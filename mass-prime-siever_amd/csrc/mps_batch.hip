@@ -182,10 +182,7 @@ PassChain slot_chain(const mps_batch* b, const Slot& s, FieldKind fk, uint32_t m
     c.h_cnt = s.h_cnt;
     c.h_out = s.h_out;
     c.out_words = 2 * (size_t)std::min(kFastRecords, s.pb.out_cap);
-    c.sp = b->g->d_sp;
-    c.sp_r32 = b->g->d_sp_r32;
-    c.nsp = b->g->nsp;
-    c.nsmall = b->g->nsmall;
+    c.tab = b->g->tab;
     c.pb = s.pb;
     c.grid_sieve = b->grid_sieve;
     c.grid_stage = b->grid_stage;

@@ -241,7 +241,7 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
     const bool qmode = !omode && class_mode(it, classes_on, class_policy == 2 ? 1 : class_policy, Q);
     if (qmode) {
         ip.Qmax = Q;
-        ip.n0modQ = (uint32_t)(N0 % Q);
+        ip.nsmodQ = (uint32_t)(ip.n_start % Q);
         ip.Q_r32 = (uint32_t)((1ull << 32) % Q);
         ip.Q_dinv = 1.0 / (double)Q;
     }
@@ -478,7 +478,7 @@ bool run_pass(mps_gpu* g, const PassBuffers& pb, uint64_t* h_rec, size_t rec_pre
     const PassDesc* d = g->d_desc;
     launch_reset(s, pb.cnt, pb.vals == nullptr ? 0u : (uint32_t)pc);
     HIPCHK(hipEventRecord(g->ev[0], s), "event");
-    if (pb.vals == nullptr) launch_sieve(s, d, pc, g->d_sp, g->d_sp_r32, g->nsp, g->nsmall, pb, 0);
+    if (pb.vals == nullptr) launch_sieve(s, d, pc, g->tab, pb, 0);
     HIPCHK(hipGetLastError(), "sieve launch");
     HIPCHK(hipEventRecord(g->ev[1], s), "event");
     if (stop_stage == 0) {
@@ -580,22 +580,26 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
     g->num_cu = prop.multiProcessorCount;
     bool ok = hipSetDevice(device_index) == hipSuccess && hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess;
     for (int i = 0; ok && i < 6; ++i) ok = hipEventCreate(&g->ev[i]) == hipSuccess;
-    // odd sieving primes 3 .. 65521 and 2^32 mod q
-    std::vector<uint32_t> sp, r32;
+    // odd sieving primes 3 .. 65521 with 2^32 mod q and 1/q
+    std::vector<SievePrime> sp;
     {
         std::vector<uint8_t> comp(65536, 0);
-        for (uint32_t i = 3; i < 65536; i += 2) {
-            if (comp[i]) continue;
-            sp.push_back(i);
-            r32.push_back((uint32_t)((1ull << 32) % i));
-            for (uint32_t j = i * i; j < 65536; j += 2 * i) comp[j] = 1;
+        for (uint32_t q = 3; q < 65536; q += 2) {
+            if (comp[q]) continue;
+            sp.push_back(SievePrime{q, (uint32_t)((1ull << 32) % q), 1.0 / (double)q});
+            for (uint32_t m = q * q; m < 65536; m += 2 * q) comp[m] = 1;
         }
     }
-    g->nsp = (uint32_t)sp.size();
-    g->nsmall = (uint32_t)(std::lower_bound(sp.begin(), sp.end(), kSmallPrimeBound) - sp.begin());
-    ok = ok && hipMalloc(&g->d_sp, sp.size() * 4) == hipSuccess && hipMalloc(&g->d_sp_r32, sp.size() * 4) == hipSuccess &&
-         hipMemcpy(g->d_sp, sp.data(), sp.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(g->d_sp_r32, r32.data(), r32.size() * 4, hipMemcpyHostToDevice) == hipSuccess &&
+    auto count_below = [&sp](uint32_t bound) {
+        return (uint32_t)(std::find_if(sp.begin(), sp.end(), [bound](const SievePrime& x) { return x.q >= bound; }) -
+                          sp.begin());
+    };
+    g->tab.n = (uint32_t)sp.size();
+    g->tab.n_comb = count_below(kCombPrimeBound);
+    g->tab.n_wave = count_below(kWavePrimeBound);
+    ok = ok && g->tab.n_comb == kCombPrimes &&
+         hipMalloc(&g->d_sp, sp.size() * sizeof(SievePrime)) == hipSuccess &&
+         hipMemcpy(g->d_sp, sp.data(), sp.size() * sizeof(SievePrime), hipMemcpyHostToDevice) == hipSuccess &&
          hipMalloc(&g->pb.cnt, sizeof(Counters)) == hipSuccess &&
          hipMalloc(&g->d_csz, kCszEntries * sizeof(uint2)) == hipSuccess &&
          hipMalloc(&g->d_desc, sizeof(PassDesc)) == hipSuccess &&
@@ -604,6 +608,7 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
          hipHostMalloc((void**)&g->h_cnt, sizeof(Counters), hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc((void**)&g->h_stage, kStageRecords * 2 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess;
     if (ok) g->scratch = scratch_address(device_index);  // with the device current (hipSetDevice above)
+    g->tab.sp = g->d_sp;
     if (!ok) {
         write_str(banner, banner_cap, "GPU disabled: HIP setup failed");
         mps_gpu_destroy(g);
@@ -952,7 +957,6 @@ void mps_gpu_destroy(mps_gpu* g) {
     free_buffers(g);
     hipFree(g->d_vals);
     hipFree(g->d_sp);
-    hipFree(g->d_sp_r32);
     hipFree(g->pb.cnt);
     hipHostFree(g->h_cnt);
     hipHostFree(g->h_stage);

@@ -9,7 +9,9 @@ namespace mps {
 // Odd numbers per sieve segment: one workgroup marks one segment in an LDS bitmap.
 constexpr uint32_t kSegOdds = 1u << 17;           // 16 KiB bitmap
 constexpr uint32_t kSieveThreads = 1024;
-constexpr uint32_t kSmallPrimeBound = 1024;       // primes below this are shared by all lanes
+constexpr uint32_t kCombPrimeBound = 64;          // sieving primes below this: register combs
+constexpr uint32_t kCombPrimes = 17;              // odd primes below kCombPrimeBound (3 .. 61)
+constexpr uint32_t kWavePrimeBound = 4096;        // below this (and from kCombPrimeBound): a wavefront per prime
 constexpr uint32_t kEntWords = 72;                // u64 words per BSGS entry (576 B)
 constexpr uint32_t kMaxClasses = 24;              // classes per prime in class mode
 constexpr uint32_t kClassMaxM = 1024;             // default largest baby-step count in class mode
@@ -87,7 +89,7 @@ struct ItemParams {
     uint64_t n_start;       // first in-class n >= N0 (plain mode)
     uint64_t kcount;        // in-class n in [n_start, N1) (plain mode)
     uint64_t Qmax;          // class mode: lcm(cm, 24)
-    uint32_t n0modQ;        // N0 mod Qmax
+    uint32_t nsmodQ;        // class mode: n_start mod Qmax
     uint32_t qmode;         // setup mode: 0 plain, 1 per-prime classes, 2 per-prime order (kModeOrder)
     uint32_t cm_r32;        // 2^32 mod cm when cm < 2^16
     uint32_t Q_r32;         // 2^32 mod Qmax
@@ -143,9 +145,21 @@ struct PassBuffers {
 
 enum FieldKind : int { FIELD_MONT = 0, FIELD_F64 = 1 };
 
+// One odd sieving prime with the constants of remainders by it: q, 2^32 mod q and 1/q (rounded).
+struct SievePrime {
+    uint32_t q, r32;
+    double inv;
+};
+// The odd primes 3 .. 65521 in ascending order: n in all, the first n_comb below kCombPrimeBound, the
+// first n_wave below kWavePrimeBound.
+struct SieveTable {
+    const SievePrime* sp;
+    uint32_t n, n_comb, n_wave;
+};
+
 void launch_reset(hipStream_t s, Counters* cnt, uint32_t n_surv0);
-void launch_sieve(hipStream_t s, const PassDesc* d, uint64_t nodd, const uint32_t* sp, const uint32_t* sp_r32,
-                  uint32_t nsp, uint32_t nsmall, const PassBuffers& pb, uint32_t grid_override);
+void launch_sieve(hipStream_t s, const PassDesc* d, uint64_t nodd, const SieveTable& tab, const PassBuffers& pb,
+                  uint32_t grid_override);
 void launch_filter(hipStream_t s, const PassDesc* d, const PassBuffers& pb, uint32_t grid);
 void launch_setup(hipStream_t s, FieldKind fk, uint32_t mode, const PassDesc* d, const PassBuffers& pb, uint32_t grid);
 TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load);
@@ -173,9 +187,7 @@ struct PassChain {
     Counters* h_cnt;          // pinned
     uint64_t* h_out;          // pinned, out_words words
     size_t out_words;
-    const uint32_t* sp;
-    const uint32_t* sp_r32;
-    uint32_t nsp, nsmall;
+    SieveTable tab;
     PassBuffers pb;
     uint32_t grid_sieve, grid_stage, grid_bsgs;
     FieldKind fk;

@@ -106,86 +106,117 @@ __global__ void k_reset(Counters* c, uint32_t n_surv0) {
 }
 
 // ------------------------------------------------------------------------------------- sieve
+//
+// One workgroup of kSieveThreads threads sieves one segment of kSegOdds odd numbers v0, v0 + 2, ...
+// in an LDS bitmap (bit i set: v0 + 2i has a sieving prime factor).  Thread t owns bits
+// [128t, 128t + 128): four words, one 128-bit LDS access.
+//   comb primes (q < kCombPrimeBound): the marks of a word are the prime's comb (bits 0, q, 2q, ...
+//     of a 64-bit word) shifted to the word's first multiple; a thread builds its four words in
+//     registers over all comb primes and stores them once, without atomics;
+//   wave primes (q < kWavePrimeBound): a wavefront per prime, its lanes on consecutive multiples;
+//   lane primes (the rest, up to 65521): a lane per prime.
+// Every prime crosses off all of its odd multiples in the segment, itself included.  A multiple
+// k*q with 1 < k < q has a prime factor below q, which crosses it off as well, so the survivors are
+// those of crossing off from q^2 once the sieving primes themselves are restored (only segments
+// that start below 65536 hold any).  Survivors, the odd numbers with no sieving prime factor other
+// than themselves, are written as indices from the pass start, one block reservation per segment.
 
-// Index j of the first odd multiple v0 + 2j of q that is >= max(v0, q^2); v0 odd.
-__device__ __forceinline__ uint32_t seg_start(uint64_t v0, uint32_t q, uint32_t r32) {
-    const uint32_t r = mod_small(v0, q, r32);
-    uint32_t j = ((q - r) % q) * ((q + 1) >> 1) % q;  // 2j == -v0 (mod q)
-    const uint64_t qq = (uint64_t)q * q;
-    if (v0 + 2 * (uint64_t)j < qq) {
-        const uint64_t jj = (qq - v0) >> 1;
-        j = jj > 0x7FFFFFFFull ? 0x7FFFFFFFu : (uint32_t)jj;
-    }
-    return j;
+// Index j of the first odd multiple v0 + 2j of the odd prime q, from r = v0 mod q: 2j == -r (mod q),
+// so j = t / 2 for t = (q - r) mod q when t is even, else (t + q) / 2.
+__device__ __forceinline__ uint32_t first_odd_multiple(uint32_t r, uint32_t q) {
+    const uint32_t t = r == 0 ? 0u : q - r;
+    return (t & 1u) ? (t + q) >> 1 : t >> 1;
 }
 
-__global__ __launch_bounds__(kSieveThreads) void k_sieve(const PassDesc* __restrict__ d, const uint32_t* __restrict__ sp,
-                                                         const uint32_t* __restrict__ sp_r32, uint32_t nsp,
-                                                         uint32_t nsmall, PassBuffers pb) {
-    const uint64_t lo = d->lo, nodd = d->nodd;
-    __shared__ uint32_t bits[kSegOdds / 32];
-    __shared__ uint32_t st[kSmallPrimeBound / 2];
-    __shared__ uint32_t wtot[kSieveThreads / 64];
-    __shared__ uint32_t gbase;
-    const uint32_t tid = threadIdx.x;
+__device__ __forceinline__ uint32_t seg_first(uint64_t v0, const SievePrime& sp) {
+    return first_odd_multiple(mod_small_f(v0, sp.q, sp.r32, sp.inv), sp.q);
+}
+
+__global__ __launch_bounds__(kSieveThreads) void k_sieve(const PassDesc* __restrict__ d, SieveTable tab, PassBuffers pb) {
+    constexpr uint32_t kWords = kSegOdds / 32, kOwn = 4;  // words per thread
+    static_assert(kWords == kOwn * kSieveThreads, "a thread owns four words of the segment");
+    __shared__ uint4 bits4[kWords / 4];
+    __shared__ SievePrime c_sp[kCombPrimes];
+    __shared__ uint32_t c_first[kCombPrimes], c_step[kCombPrimes];
+    __shared__ unsigned long long c_comb[kCombPrimes];
+    __shared__ uint32_t sh[kSieveThreads / 64 + 1];
+    uint32_t* bits = reinterpret_cast<uint32_t*>(bits4);
+    const uint64_t nodd = d->nodd;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
     const uint64_t seg0 = (uint64_t)blockIdx.x * kSegOdds;
-    if (seg0 >= nodd) return;
-    const uint64_t rest = nodd - seg0;
-    const uint32_t len = rest < kSegOdds ? (uint32_t)rest : kSegOdds;
-    const uint64_t v0 = lo + 2 * seg0;
+    if (seg0 >= nodd) return;  // the whole workgroup (the grid is sized for the largest pass)
+    const uint32_t len = nodd - seg0 < kSegOdds ? (uint32_t)(nodd - seg0) : kSegOdds;
+    const uint64_t v0 = d->lo + 2 * seg0;
 
-    uint4* b4 = reinterpret_cast<uint4*>(bits);
-    for (uint32_t i = tid; i < kSegOdds / 128; i += kSieveThreads) b4[i] = make_uint4(0u, 0u, 0u, 0u);
-    for (uint32_t k = tid; k < nsmall; k += kSieveThreads) st[k] = seg_start(v0, sp[k], sp_r32[k]);
-    __syncthreads();
-
-    // primes below kSmallPrimeBound: the lanes of the block share each prime
-    for (uint32_t k = 0; k < nsmall; ++k) {
-        const uint32_t q = sp[k];
-        for (uint32_t i = st[k] + tid * q; i < len; i += kSieveThreads * q) atomicOr(&bits[i >> 5], 1u << (i & 31));
-    }
-    // larger primes: one lane per prime
-    for (uint32_t k = nsmall + tid; k < nsp; k += kSieveThreads) {
-        const uint32_t q = sp[k];
-        for (uint32_t i = seg_start(v0, q, sp_r32[k]); i < len; i += q) atomicOr(&bits[i >> 5], 1u << (i & 31));
+    // the comb primes' first multiples, 32 mod q (the shift from one word to the next) and combs
+    if (tid < tab.n_comb) {
+        const SievePrime sp = tab.sp[tid];
+        c_sp[tid] = sp;
+        c_first[tid] = seg_first(v0, sp);
+        c_step[tid] = 32u % sp.q;
+        unsigned long long comb = 0;
+        for (uint32_t b = 0; b < 64; b += sp.q) comb |= 1ull << b;
+        c_comb[tid] = sp.q < 32 ? comb : 1ull;
     }
     __syncthreads();
-
-    // compaction: lane t owns words t, t + 256, ...
-    const uint32_t words = (len + 31) >> 5;
-    uint32_t mine = 0;
-    for (uint32_t w = tid; w < words; w += kSieveThreads) {
-        uint32_t x = ~bits[w];
-        const uint32_t valid = len - (w << 5);
-        if (valid < 32) x &= (1u << valid) - 1u;
-        mine += __popc(x);
-    }
-    const uint32_t lane = lane_id(), wave = tid >> 6;
-    uint32_t incl = mine;
-    for (uint32_t d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += y;
-    }
-    if (lane == 63) wtot[wave] = incl;
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t tot = 0;
-        for (uint32_t i = 0; i < kSieveThreads / 64; ++i) tot += wtot[i];
-        gbase = atomicAdd(&pb.cnt->n_surv, tot);
-    }
-    __syncthreads();
-    uint32_t pos = gbase + incl - mine;
-    for (uint32_t i = 0; i < wave; ++i) pos += wtot[i];
-    for (uint32_t w = tid; w < words; w += kSieveThreads) {
-        uint32_t x = ~bits[w];
-        const uint32_t valid = len - (w << 5);
-        if (valid < 32) x &= (1u << valid) - 1u;
-        while (x) {
-            const uint32_t b = __builtin_ctz(x);
-            if (pos < pb.surv_cap) pb.surv[pos] = (uint32_t)(seg0 + (w << 5) + b);
-            ++pos;
-            x &= x - 1u;
+    {
+        const uint32_t x = kOwn * 32 * tid;  // the thread's first bit
+        uint32_t w[kOwn] = {0u, 0u, 0u, 0u};
+        for (uint32_t k = 0; k < tab.n_comb; ++k) {
+            const uint32_t q = c_sp[k].q, step = c_step[k];
+            const unsigned long long comb = c_comb[k];
+            // offset of the first multiple at or after bit x: (first - x) mod q
+            uint32_t o = c_first[k] + q - mod16_f(x, q, c_sp[k].inv);
+            o = o >= q ? o - q : o;
+#pragma unroll
+            for (uint32_t i = 0; i < kOwn; ++i) {
+                w[i] |= o < 64 ? (uint32_t)(comb << o) : 0u;
+                o = o >= step ? o - step : o + q - step;  // offset in the next word
+            }
         }
+        bits4[tid] = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+    __syncthreads();
+    // wave primes: wavefront `wave` takes every sixteenth one
+    for (uint32_t k = tab.n_comb + wave; k < tab.n_wave; k += kSieveThreads / 64) {
+        const SievePrime sp = tab.sp[k];
+        const uint32_t q = sp.q;
+        for (uint32_t i = seg_first(v0, sp) + lane * q; i < len; i += 64 * q) atomicOr(&bits[i >> 5], 1u << (i & 31));
+    }
+    // lane primes
+    for (uint32_t k = tab.n_wave + tid; k < tab.n; k += kSieveThreads) {
+        const SievePrime sp = tab.sp[k];
+        for (uint32_t i = seg_first(v0, sp); i < len; i += sp.q) atomicOr(&bits[i >> 5], 1u << (i & 31));
+    }
+    __syncthreads();
+    if (v0 < 65536) {  // the sieving primes inside the segment are primes: restore them
+        for (uint32_t k = tid; k < tab.n; k += kSieveThreads) {
+            const uint32_t q = tab.sp[k].q;
+            if (q >= v0 && (q - v0) / 2 < len) {
+                const uint32_t i = (uint32_t)((q - v0) / 2);
+                atomicAnd(&bits[i >> 5], ~(1u << (i & 31)));
+            }
+        }
+        __syncthreads();
+    }
+    // survivors: the clear bits below len, in bit order per thread, after one block reservation
+    const uint4 m = bits4[tid];
+    uint32_t cw[kOwn] = {~m.x, ~m.y, ~m.z, ~m.w};
+    const uint32_t x = kOwn * 32 * tid;
+    uint32_t c = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < kOwn; ++i) {
+        const uint32_t b0 = x + 32 * i;  // bits at or past len are not in the segment
+        if (b0 >= len) cw[i] = 0;
+        else if (len - b0 < 32) cw[i] &= (1u << (len - b0)) - 1u;
+        c += __popc(cw[i]);
+    }
+    uint32_t pos = block_reserve<kSieveThreads / 64>(c, &pb.cnt->n_surv, sh);
+    const uint32_t idx0 = (uint32_t)seg0 + x;
+#pragma unroll
+    for (uint32_t i = 0; i < kOwn; ++i) {
+        for (uint32_t v = cw[i]; v != 0; v &= v - 1u, ++pos)
+            if (pos < pb.surv_cap) pb.surv[pos] = idx0 + 32 * i + (uint32_t)__builtin_ctz(v);
     }
 }
 
@@ -235,67 +266,70 @@ __device__ __forceinline__ void put_record(const PassBuffers& pb, uint32_t slot,
     }
 }
 
-// Plain mode (prep_core decision, Gpu.hpp:231-263).  info: bit0 = stride 2, bit1 = odd k.
-__device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, bool& keep, bool& flag2,
+// The residue decisions of one number p for the item (prep_core / prep_q_core, Gpu.hpp:231-263,
+// :478-509).  Numbers the device leaves to the host (p < 1000, p even, p | b) set `host`; a number
+// that cannot divide any b^n + c (t = -c == 0 mod p, no in-class exponent) keeps nothing.  Both
+// decisions rest on the quadratic characters: b^n == t (mod p) needs (t/p) == (b/p)^n.
+
+// Plain mode: info bit 0 = stride 2 (only exponents n_start + 2k + parity can solve), bit 1 = the
+// odd k (parity 1).  gm = gcd(cm, p - 1): with gm == 2 the class exponents share one parity, so the
+// characters decide p; with gm < 2 and (b/p) == -1 they fix the parity of k; with gm > 2 the setup's
+// power test decides.
+__device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, bool& keep, bool& host,
                                              uint32_t& info) {
-    if (p < 1000 || (p & 1) == 0) { flag2 = true; return; }
-    if (mod_signed(ip.b, p) == 0) { flag2 = true; return; }
-    if (mod_signed(ip.c, p) == 0) return;  // t = -c == 0: b^n == 0 impossible
-    if (ip.kcount == 0) return;
+    if (p < 1000 || (p & 1) == 0 || mod_signed(ip.b, p) == 0) {
+        host = true;
+        return;
+    }
+    if (mod_signed(ip.c, p) == 0 || ip.kcount == 0) return;
     const uint64_t pm = pm1_mod(p, ip.cm, ip.cm_r32, ip.cm_dinv);
-    const uint64_t g = ip.cm < (1ull << 32) ? gcd32((uint32_t)ip.cm, (uint32_t)pm) : gcd64(ip.cm, pm);
-    const int jb = jacobi_item(ip.jb, ip.b, p);
-    const int jt = jacobi_item(ip.jt, -ip.c, p);
-    const int lT = jt * ((ip.n_start & 1) ? jb : 1);
-    if (g == 2) {
-        keep = lT == 1;
-    } else if (g < 2) {
-        if (jb == 1) {
-            keep = lT == 1;
-        } else {
-            keep = true;
-            info = 1u | (lT != 1 ? 2u : 0u);
-        }
-    } else {
+    const uint64_t gm = ip.cm < (1ull << 32) ? gcd32((uint32_t)ip.cm, (uint32_t)pm) : gcd64(ip.cm, pm);
+    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item(ip.jt, -ip.c, p);
+    // the character of t * b^-n_start: +1 is needed when k = 0 is possible, i.e. for even k
+    const int s0 = (ip.n_start & 1) ? st * sb : st;
+    if (gm > 2) {
         keep = true;
+    } else if (gm == 2 || sb == 1) {
+        keep = s0 == 1;  // every exponent of the progression has the character of k = 0
+    } else {
+        keep = true;  // (b/p) == -1, cm odd: k even needs s0 == 1, k odd needs s0 == -1
+        info = 1u | (s0 == 1 ? 0u : 2u);
     }
 }
 
-// Per-prime class split (prep_q_core, Gpu.hpp:491-498): g = gcd(Qmax, p-1), qp = lcm(cm, g),
-// ncls = qp / cm classes; class j starts at N0 + dj(j), dj(j) = (cres + j*cm - N0) mod qp.
-struct ClassSplit {
-    uint32_t g, qp, ncls, c0, cm, n0q;
-    __device__ __forceinline__ uint32_t dj(uint32_t j) const { return (c0 + j * cm + qp - n0q) % qp; }
+// Class mode.  With Q = lcm(cm, 24), g = gcd(Q, p - 1) and qp = lcm(cm, g), the in-class exponents
+// n_start + i*cm (i < kcount) fall into ncls = qp / cm classes by i mod ncls: class k holds
+// n_start + k*cm + l*qp (l >= 0), all of one parity since qp is even.  Bit k of a class mask stands
+// for class k; its first exponent lies k*cm past n_start.
+struct ExpClasses {
+    uint32_t g, qp, ncls;
 };
 
-__device__ __forceinline__ void class_split(uint64_t p, const ItemParams& ip, ClassSplit& cs) {
-    const uint32_t Q = (uint32_t)ip.Qmax;
-    cs.cm = (uint32_t)ip.cm;
-    cs.g = gcd32(Q, mod_small_f(p - 1, Q, ip.Q_r32, ip.Q_dinv));
-    cs.qp = cs.g / gcd32(cs.g, cs.cm) * cs.cm;
-    cs.ncls = cs.qp / cs.cm;
-    cs.n0q = ip.n0modQ % cs.qp;
-    cs.c0 = (uint32_t)(ip.cres % ip.cm);
+__device__ __forceinline__ ExpClasses exp_classes(uint64_t p, const ItemParams& ip) {
+    ExpClasses ec;
+    const uint32_t Q = (uint32_t)ip.Qmax, cm = (uint32_t)ip.cm;
+    ec.g = gcd32(Q, mod_small_f(p - 1, Q, ip.Q_r32, ip.Q_dinv));
+    ec.qp = ec.g / gcd32(ec.g, cm) * cm;
+    ec.ncls = ec.qp / cm;
+    return ec;
 }
 
-// Class mode (prep_q_core decision, Gpu.hpp:478-509).  info = class mask.
-__device__ __forceinline__ void filter_class(uint64_t p, const ItemParams& ip, bool& keep, bool& flag2,
+// info = the classes whose parity the characters allow (and that hold an exponent below N1).
+__device__ __forceinline__ void filter_class(uint64_t p, const ItemParams& ip, bool& keep, bool& host,
                                              uint32_t& info) {
-    if (p < 1000 || (p & 1) == 0) { flag2 = true; return; }
-    if (mod_signed(ip.b, p) == 0) { flag2 = true; return; }
+    if (p < 1000 || (p & 1) == 0 || mod_signed(ip.b, p) == 0) {
+        host = true;
+        return;
+    }
     if (mod_signed(ip.c, p) == 0) return;
-    ClassSplit cs;
-    class_split(p, ip, cs);
-    const uint64_t span = ip.N1 - ip.N0;
-    const int jb = jacobi_item(ip.jb, ip.b, p);
-    const int jt = jacobi_item(ip.jt, -ip.c, p);
-    const uint32_t n0odd = (uint32_t)(ip.N0 & 1);
+    const ExpClasses ec = exp_classes(p, ip);
+    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item(ip.jt, -ip.c, p);
+    const uint32_t nk = ip.kcount < ec.ncls ? (uint32_t)ip.kcount : ec.ncls;
+    const uint32_t odd0 = (uint32_t)(ip.n_start & 1), odd_step = (uint32_t)(ip.cm & 1);
     uint32_t mask = 0;
-    for (uint32_t j = 0; j < cs.ncls; ++j) {
-        const uint32_t d = cs.dj(j);
-        if ((uint64_t)d >= span) continue;
-        const int l = jt * (((n0odd ^ (d & 1u)) != 0) ? jb : 1);
-        if (l == 1) mask |= 1u << j;
+    for (uint32_t k = 0; k < nk; ++k) {
+        const bool odd = (odd0 ^ (k & odd_step)) != 0;  // parity of n_start + k*cm
+        if (st == (odd ? sb : 1)) mask |= 1u << k;
     }
     keep = mask != 0;
     info = mask;
@@ -318,23 +352,23 @@ __global__ __launch_bounds__(256) void k_filter(const PassDesc* __restrict__ d, 
 #pragma unroll
         for (uint32_t k = 0; k < kFilterPer; ++k) {
             const uint32_t i = i0 + k * 256 + threadIdx.x;
-            bool keep = false, flag2 = false;
+            bool keep = false, host = false;
             uint32_t info = 0, idx = 0;
             uint64_t p = 0;
             if (i < n) {
                 idx = pb.vals ? i : pb.surv[i];
                 p = pb.vals ? pb.vals[i] : lo + 2 * (uint64_t)idx;
                 if (ip.qmode == kModeClass)
-                    filter_class(p, ip, keep, flag2, info);
+                    filter_class(p, ip, keep, host, info);
                 else
-                    filter_plain(p, ip, keep, flag2, info);
+                    filter_plain(p, ip, keep, host, info);
             }
             kept[k] = keep;
             val[k] = (uint64_t)idx | ((uint64_t)info << 32);
             c += keep ? 1u : 0u;
-            const uint32_t oslot = wave_append(flag2, &pb.cnt->n_out);
-            if (flag2) put_record(pb, oslot, pb.out_index ? idx : p, kHostDecides);
-            nflag2 += flag2 ? 1u : 0u;
+            const uint32_t oslot = wave_append(host, &pb.cnt->n_out);
+            if (host) put_record(pb, oslot, pb.out_index ? idx : p, kHostDecides);
+            nflag2 += host ? 1u : 0u;
         }
         uint32_t slot = block_reserve<4>(c, &pb.cnt->n_cand, sh);
 #pragma unroll
@@ -615,9 +649,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 if (i < n) {
                     const uint32_t ix = (uint32_t)pb.cand[i];
                     const uint64_t pk = pb.vals ? pb.vals[ix] : lo + 2 * (uint64_t)ix;
-                    ClassSplit cs;
-                    class_split(pk, ip, cs);
-                    key = min(cs.g >> 1, 30u);
+                    key = min(exp_classes(pk, ip).g >> 1, 30u);
                 }
                 keys[q] = key;
                 atomicAdd(&s_bin[key], 1u);
@@ -777,8 +809,8 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
             const uint32_t i = t0 + li;
             bool keep = false;
             uint64_t p = 0, m = 0, ng = 0, jc = 0;
-            uint32_t idx = 0, info = 0, s = 1, mask = 0, dmin = 0;
-            ClassSplit cs;
+            uint32_t idx = 0, info = 0, s = 1, mask = 0;
+            ExpClasses ec{0, 0, 0};
             F f;
             if (i < n) {
                 const uint64_t cd = pb.cand[i];
@@ -796,32 +828,44 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                     const T rr = f.r2();
                     const T bm = f.from(bmod, rr), tm = f.from(t, rr);
                     if constexpr (QMODE) {
-                        class_split(p, ip, cs);
+                        ec = exp_classes(p, ip);
                         mask = info;
-                        if (cs.g > 2) {
-                            // class j needs t^e == (b^e)^(n_j mod g), e = (p-1)/g
-                            const uint64_t e = (p - 1) / cs.g;
+                        if (ec.g > 2) {
+                            // b^n == t forces w^n == u for w = b^e, u = t^e, e = (p - 1) / g, in the
+                            // group of g-th roots of unity: n == k0 (mod d) for d = ord(w) (a divisor
+                            // of g) and k0 = log_w(u), or no n at all.  One scan of w^k finds both.
+                            const uint64_t e = (p - 1) / ec.g;
                             T u, w;
                             fpow2(f, tm, bm, e, u, w);
-                            w_setup += 2 * bitlen64(e) + cs.g;
-                            const uint64_t ku = f.key(u);
-                            const uint32_t n0g = ip.n0modQ % cs.g;
-                            uint32_t keepm = 0;
+                            const uint64_t ku = f.key(u), k1 = f.key(f.one());
+                            uint32_t d = 0, k0 = ~0u;
                             T wk = f.one();
-                            for (uint32_t k = 0; k < cs.g; ++k) {
-                                if (f.key(wk) == ku)
-                                    for (uint32_t j = 0; j < cs.ncls; ++j)
-                                        if ((n0g + cs.dj(j)) % cs.g == k) keepm |= 1u << j;
+                            for (uint32_t k = 0; k <= ec.g; ++k) {
+                                const uint64_t kw = f.key(wk);
+                                if (k > 0 && kw == k1) {
+                                    d = k;
+                                    break;
+                                }
+                                if (k0 == ~0u && kw == ku) k0 = k;
                                 wk = f.mul(wk, w);
                             }
-                            mask &= keepm;
+                            w_setup += 2 * bitlen64(e) + d;
+                            // class k keeps n == n_start + k*cm (mod qp): n mod d from n_start mod Q
+                            uint32_t keepm = 0;
+                            if (d != 0 && k0 != ~0u) {
+                                const uint32_t step = (uint32_t)ip.cm % d;  // cm divides Q <= 48
+                                uint32_t r = ip.nsmodQ % d;
+                                for (uint32_t k = 0; k < ec.ncls; ++k) {
+                                    if (r == k0) keepm |= 1u << k;
+                                    r += step;
+                                    r = r >= d ? r - d : r;
+                                }
+                            }
+                            mask &= keepm;  // d == 0: b^(p-1) != 1, a composite
                         }
                         if (mask != 0) {
                             s = __popc(mask);
-                            dmin = cs.qp;
-                            for (uint32_t j = 0; j < cs.ncls; ++j)
-                                if (((mask >> j) & 1u) && cs.dj(j) < dmin) dmin = cs.dj(j);
-                            const uint2 sz = ip.csz[cs.qp * 32 + s];
+                            const uint2 sz = ip.csz[ec.qp * 32 + s];
                             m = sz.x;
                             ng = sz.y;
                             keep = true;
@@ -878,31 +922,33 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 E[E_MG] = m | (ng << 32);
                 E[E_SSRC] = (uint64_t)s | ((uint64_t)((QMODE ? mask : info) & 0xFFFFFFu) << 8) | ((uint64_t)idx << 32);
                 if (QMODE) {
-                    // G_j = b^(N0 + dj) = b^(N0 + dmin) * b^(dj - dmin); B and the base share squarings
-                    T base;
-                    fpow_2e(f, bm, cs.qp, ip.N0 + dmin, B, base);
+                    // baby base B = b^qp; class k starts at n_start + k*cm, so its giant start is
+                    // G0 * C^k with G0 = b^n_start and C = b^cm (B and G0 share their squarings)
+                    T G0;
+                    fpow_2e(f, bm, ec.qp, ip.n_start, B, G0);
+                    const T C = fpow(f, bm, ip.cm);
                     E[E_NB] = ip.N0;
-                    E[E_META] = cs.qp | (1u << 10);
+                    E[E_META] = ec.qp | (1u << 10);
                     E[E_KLIM] = 0;
-                    w_setup += 2 * bitlen64(ip.N0 + dmin) + 12;
+                    w_setup += 2 * bitlen64(ip.n_start) + 2 * bitlen64(ip.cm) + 12;
+                    // per kept class, in ascending k: the giant start and (8 bits each) the offset of
+                    // its first exponent from N0, n_start - N0 + k*cm < qp
+                    const uint32_t off0 = (uint32_t)(ip.n_start - ip.N0), cm = (uint32_t)ip.cm;
+                    uint64_t offs[3] = {0, 0, 0};
                     uint32_t o = 0;
-                    uint64_t djw[3] = {0, 0, 0};
-                    for (uint32_t j = 0; j < cs.ncls; ++j) {
-                        if (!((mask >> j) & 1u)) continue;
-                        const uint32_t d = cs.dj(j);
-                        E[E_G + o] = F::to_bits(f.mul(base, fpow(f, bm, d - dmin)));
-                        if (o < 8)
-                            djw[0] |= (uint64_t)d << (8 * o);
-                        else if (o < 16)
-                            djw[1] |= (uint64_t)d << (8 * (o - 8));
-                        else
-                            djw[2] |= (uint64_t)d << (8 * (o - 16));
-                        w_setup += 12;
-                        ++o;
+                    T Gk = G0;
+                    for (uint32_t k = 0; (mask >> k) != 0; ++k) {
+                        if ((mask >> k) & 1u) {
+                            E[E_G + o] = F::to_bits(Gk);
+                            offs[o >> 3] |= (uint64_t)(off0 + k * cm) << (8 * (o & 7));
+                            ++o;
+                        }
+                        Gk = f.mul(Gk, C);
+                        ++w_setup;
                     }
-                    E[E_DJ] = djw[0];
-                    E[E_DJ + 1] = djw[1];
-                    E[E_DJ + 2] = djw[2];
+                    E[E_DJ] = offs[0];
+                    E[E_DJ + 1] = offs[1];
+                    E[E_DJ + 2] = offs[2];
                 } else {
                     T G0;
                     fpow_2e(f, bm, ip.cm, ip.n_start, B, G0);
@@ -1327,10 +1373,10 @@ void launch_reset(hipStream_t s, Counters* cnt, uint32_t n_surv0) {
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, cnt, n_surv0);
 }
 
-void launch_sieve(hipStream_t s, const PassDesc* d, uint64_t nodd, const uint32_t* sp, const uint32_t* sp_r32,
-                  uint32_t nsp, uint32_t nsmall, const PassBuffers& pb, uint32_t grid_override) {
+void launch_sieve(hipStream_t s, const PassDesc* d, uint64_t nodd, const SieveTable& tab, const PassBuffers& pb,
+                  uint32_t grid_override) {
     const uint32_t grid = grid_override ? grid_override : (uint32_t)((nodd + kSegOdds - 1) / kSegOdds);
-    hipLaunchKernelGGL(k_sieve, dim3(grid), dim3(kSieveThreads), 0, s, d, sp, sp_r32, nsp, nsmall, pb);
+    hipLaunchKernelGGL(k_sieve, dim3(grid), dim3(kSieveThreads), 0, s, d, tab, pb);
 }
 
 void launch_filter(hipStream_t s, const PassDesc* d, const PassBuffers& pb, uint32_t grid) {
@@ -1434,7 +1480,7 @@ hipError_t enqueue_pass_chain(hipStream_t s, const PassChain& c) {
     hipError_t e = hipMemcpyAsync(c.d_desc, c.h_desc, sizeof(PassDesc), hipMemcpyHostToDevice, s);
     if (e != hipSuccess) return e;
     launch_reset(s, c.pb.cnt, 0);
-    launch_sieve(s, c.d_desc, 0, c.sp, c.sp_r32, c.nsp, c.nsmall, c.pb, c.grid_sieve);
+    launch_sieve(s, c.d_desc, 0, c.tab, c.pb, c.grid_sieve);
     launch_filter(s, c.d_desc, c.pb, c.grid_stage);
     launch_setup(s, c.fk, c.mode, c.d_desc, c.pb, c.grid_stage);
     launch_bsgs(s, c.fk, c.tg, c.d_desc, c.pb, 0, c.grid_bsgs);
@@ -1482,11 +1528,9 @@ hipError_t build_pass_graph(const PassChain& c, hipGraph_t* out) {
     Counters* cnt = c.pb.cnt;
     uint32_t zero = 0;
     PassBuffers pb = c.pb;
-    const uint32_t* sp = c.sp;
-    const uint32_t* sp_r32 = c.sp_r32;
-    uint32_t nsp = c.nsp, nsmall = c.nsmall;
+    SieveTable tab = c.tab;
     void* a_reset[] = {&cnt, &zero};
-    void* a_sieve[] = {&d, &sp, &sp_r32, &nsp, &nsmall, &pb};
+    void* a_sieve[] = {&d, &tab, &pb};
     void* a_stage[] = {&d, &pb};
     void* a_records[] = {&pb};
     uint32_t bsgs_threads = 0;

@@ -33,9 +33,8 @@ struct mps_gpu {
     mps::PassDesc* d_desc = nullptr;
     mps::PassDesc* h_desc = nullptr;
     hipStream_t stream = nullptr;
-    uint32_t* d_sp = nullptr;
-    uint32_t* d_sp_r32 = nullptr;
-    uint32_t nsp = 0, nsmall = 0;
+    mps::SievePrime* d_sp = nullptr;
+    mps::SieveTable tab{};  // d_sp with its counts
     mps::PassBuffers pb{};
     uint64_t buf_odds = 0;
     uint64_t* d_vals = nullptr;

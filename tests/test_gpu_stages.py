@@ -12,6 +12,8 @@ pytestmark = pytest.mark.gpu
 MCAP = 2048  # default largest baby-step count without classes
 
 SIEVE_RANGES = [
+    (5, 5 + 2 * 150_000),                  # from the smallest start the device takes: sieving primes survive
+    (2**15 + 1, 65536 + 2**18),            # across the largest sieving prime
     (1001, 1001 + 2 * 300_000),            # tiny p: survivors are the primes
     (10**9, 10**9 + 2**22),                # c1 scale: sqrt(hi) < 65521, survivors are the primes
     (10**12 + 12345, 10**12 + 12345 + 2**23),
@@ -54,6 +56,22 @@ def class_modulus(oracle, oit, mod, N0, N1, classes, policy):
     return qmode and classes, int(Q)
 
 
+def offset_order(mask, qp, mod, res, N0):
+    """The oracle's class mask (bit j: residue cres + j*cm modulo qp, prep_q_core's numbering,
+    Gpu.hpp:491-498) in the device's numbering: bit k for the class whose first exponent lies k*cm
+    past n_start, the first in-class exponent >= N0."""
+    cm = max(1, mod)
+    N0 = max(0, N0)
+    off0 = (res % cm - N0) % cm  # n_start - N0
+    out = 0
+    for j in range(qp // cm):
+        if (mask >> j) & 1:
+            dj = (res % cm + j * cm - N0) % qp  # offset of class j's first exponent from N0
+            assert (dj - off0) % cm == 0
+            out |= 1 << ((dj - off0) // cm)
+    return out
+
+
 @pytest.mark.parametrize("field,policy", [(-1, 1), (0, 1), (-1, 0)])
 @pytest.mark.parametrize("it", ITEMS)
 def test_entries_match_oracle_decisions(gpu, mps, oracle, it, field, policy):
@@ -81,7 +99,7 @@ def test_entries_match_oracle_decisions(gpu, mps, oracle, it, field, policy):
         if qmode:
             r, e = oracle.prep_q(x, oit, Q, True)
             if r == 1:
-                want[x] = (e.s, e.mask)
+                want[x] = (e.s, offset_order(e.mask, e.qp, mod, res, N0))
         else:
             r, e = oracle.prep(x, oit, True)
             if r == 1:
