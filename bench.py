@@ -26,7 +26,9 @@ FLOP_PER_PRODUCT_F64 = 9
 # m = t0 * n0: 1, m * p: 2)
 MUL_PER_PRODUCT_MONT = 10
 PEAK_FP64_TFLOPS = 78.6          # MI355X published FP64 vector/matrix peak
-PEAK_INT_MUL_TOPS = 31.3         # measured full-rate v_mul_lo_u32 issue over the chip (mps_gpu_measure_valu)
+# 32x32-bit multiplies per 64-bit Montgomery product (mmul), the work unit of the integer roofline:
+# 4 for the full a*b, 3 for lo*ninv mod 2^64, 4 for the full m*p
+MUL_PER_MMUL = 11
 # HBM bytes per tuned k_bsgs launch on this workload: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scale factors
 # from a 1 GiB streaming copy), from profiles/k_bsgs_c2_hbm_traffic.txt
 K_BSGS_TRAFFIC_BYTES = 2.932e8
@@ -54,21 +56,72 @@ def prime_count(P0, P1):
     return counts[f"{P0}:{P1}"]
 
 
+def host_cpu():
+    """The host's CPU model, logical CPU count and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = os.cpu_count() or 1
+    return {"model": model, "nproc": os.cpu_count() or 1, "affinity": affinity}
+
+
+# Host threads of the CPU legs: the CPUs this process may use, at most 16 (one GPU's share of a
+# GPU node's host cores).
+CPU_THREADS_MAX = 16
+
+
+def cpu_time_range(O, b, c, P0, P1, N0, N1, mod, res, threads):
+    item = O.make_item(b, c, P0, P1, N0, N1, mod, res)
+    chunk = max(1 << 20, (P1 - P0) // (4 * threads))
+    t0 = time.perf_counter()
+    rels, primes = O.process_range(item, P0, P1, chunk, threads)
+    return time.perf_counter() - t0, primes, len(rels)
+
+
 def cpu_baseline(threads, slice_ints):
-    """The oracle's CPU path (segmented sieve + run_bsgs per prime) on the first slice_ints
-    integers of the workload."""
+    """The oracle's CPU path (segmented sieve + run_bsgs per prime, the reference's CPU branch) on
+    the first slice_ints integers of the workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     w = WORKLOAD
-    P1 = w["P0"] + slice_ints
-    item = O.make_item(w["b"], w["c"], w["P0"], P1, w["N0"], w["N1"], w["mod"], w["res"])
-    chunk = max(1 << 20, slice_ints // (4 * threads))
-    t0 = time.perf_counter()
-    rels, primes = O.process_range(item, w["P0"], P1, chunk, threads)
-    dt = time.perf_counter() - t0
+    dt, primes, nrel = cpu_time_range(O, w["b"], w["c"], w["P0"], w["P0"] + slice_ints, w["N0"], w["N1"], w["mod"],
+                                      w["res"], threads)
+    hc = host_cpu()
     return {"value": primes / dt, "unit": "primes/s", "cores": threads, "kind": "port",
             "sample": f"first {slice_ints} integers of the workload's P range ({primes} primes, "
-                      f"{len(rels)} relations) in {dt:.2f} s on {threads} threads"}
+                      f"{nrel} relations) in {dt:.2f} s on {threads} threads",
+            "cpu_model": hc["model"], "nproc": hc["nproc"], "affinity": hc["affinity"]}
+
+
+def cpu_baseline_configs(threads, slice_ints=1 << 26):
+    """The oracle's CPU path on c1 (whole) and the first slice_ints integers of c3 and c4, as
+    primes/s; c4's sieving primes up to sqrt(P1) ~ 2e9 (sieve_small_primes, a one-off per item)
+    are timed on their own first."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    out = {}
+    for name, (b, c, P0, P1, N0, N1, mod, res, _) in CONFIGS.items():
+        hi = min(P1, P0 + slice_ints)
+        setup_s = None
+        if name == "c4":
+            t = time.perf_counter()
+            O.sieve_primes(P1 - 2, P1)  # the sieving primes up to sqrt(P1), cached in the oracle
+            setup_s = time.perf_counter() - t
+        dt, primes, nrel = cpu_time_range(O, b, c, P0, hi, N0, N1, mod, res, threads)
+        out[name] = {"value": primes / dt, "unit": "primes/s", "cores": threads, "kind": "port",
+                     "sample": f"first {hi - P0} integers ({primes} primes, {nrel} relations) in {dt:.2f} s"}
+        if setup_s is not None:
+            out[name]["sieving_primes_setup_s"] = setup_s
+    return out
 
 
 def load_mix(n):
@@ -189,7 +242,44 @@ CONFIGS = {  # the other configurations of the metric, as (b, c, P0, P1, N0, N1,
 }
 
 
-def run_configs(mps, g, log, reps=3):
+def item_stats(st, reps):
+    """Per-item device counts and stage times of `reps` runs' accumulated stats."""
+    return {"entries": st["entries"] // reps, "candidates": st["candidates"] // reps,
+            "survivors": st["survivors"] // reps, "flagged": st["flagged"] // reps,
+            "products": {k: st[k] // reps for k in ("w_fermat", "w_setup", "w_bsgs")},
+            "stage_ms": {k: st[k] / reps for k in ("ms_sieve", "ms_filter", "ms_setup", "ms_bsgs")},
+            "field": "fp64" if st["field"] == 1 else "montgomery"}
+
+
+def int_roofline(st, reps, t_plain, t_batch, rates):
+    """The item's modular products W (Fermat, setup, baby and giant steps, counted on the device)
+    as 32x32-bit multiplies (MUL_PER_MMUL each) per second of item time, against the measured
+    v_mad_u64_u32 issue rate of the chip."""
+    W = (st["w_fermat"] + st["w_setup"] + st["w_bsgs"]) / reps
+    peak = rates["mad_u64_u32_per_s"]
+    return {"W_mmul": W, "unit": "32x32 multiplies/s", "peak": peak,
+            "achieved_plain": MUL_PER_MMUL * W / t_plain, "frac_plain": MUL_PER_MMUL * W / t_plain / peak,
+            "achieved_batched": MUL_PER_MMUL * W / t_batch, "frac_batched": MUL_PER_MMUL * W / t_batch / peak}
+
+
+def kernel_roofline(st, rates):
+    """k_bsgs alone, from its event times: FP64 field products x 9 FLOP against the FP64 peak, or
+    Montgomery products x 10 32-bit multiplies against the measured v_mad_u64_u32 rate."""
+    launches = max(1, st["n_bsgs_launches"])
+    ms = st["ms_bsgs"] / launches
+    prod = st["w_bsgs"] / launches
+    if st["field"] == 1:
+        achieved = prod * FLOP_PER_PRODUCT_F64 / (ms * 1e-3)
+        return {"kernel": "k_bsgs<F64Field>", "bound": "valu", "unit": "FLOP/s", "achieved": achieved,
+                "peak": PEAK_FP64_TFLOPS * 1e12, "frac": achieved / (PEAK_FP64_TFLOPS * 1e12),
+                "ms_per_launch": ms, "products_per_launch": prod}
+    achieved = prod * MUL_PER_PRODUCT_MONT / (ms * 1e-3)
+    return {"kernel": "k_bsgs<MontField>", "bound": "valu", "unit": "32x32 multiplies/s", "achieved": achieved,
+            "peak": rates["mad_u64_u32_per_s"], "frac": achieved / rates["mad_u64_u32_per_s"],
+            "ms_per_launch": ms, "products_per_launch": prod}
+
+
+def run_configs(mps, g, log, rates, reps=3):
     """Item latency and primes/s of c1, c3 and c4 on the plain worker and the batched engine."""
     out = {}
     for name, (b, c, P0, P1, N0, N1, mod, res, classes) in CONFIGS.items():
@@ -198,10 +288,12 @@ def run_configs(mps, g, log, reps=3):
         try:
             g.run_item(w, chunk_ints=CHUNK)  # warm-up
             t_plain = []
+            g.reset_stats()
             for _ in range(reps):
                 t = time.perf_counter()
                 rels, _ = g.run_item(w, chunk_ints=CHUNK)
                 t_plain.append(time.perf_counter() - t)
+            st = g.stats()
             eng = mps.Batch(g)
             t_batch, same = [], True
             for _ in range(reps):  # the first builds the slots' graphs
@@ -220,7 +312,10 @@ def run_configs(mps, g, log, reps=3):
                      "primes": n, "relations": len(rels),
                      "plain_item_s": min(t_plain), "plain_primes_per_s": n / min(t_plain),
                      "batched_item_s": min(t_batch), "batched_primes_per_s": n / min(t_batch),
-                     "relations_identical": same}
+                     "relations_identical": same,
+                     "device": item_stats(st, reps),
+                     "int_roofline": int_roofline(st, reps, min(t_plain), min(t_batch), rates),
+                     "kernel_roofline": kernel_roofline(st, rates)}
         log(f"{name}: plain {min(t_plain)*1e3:.1f} ms, batched {min(t_batch)*1e3:.1f} ms per item")
     return out
 
@@ -257,6 +352,9 @@ def main():
                     help="only the tuned plain c2 leg (no untuned or batched legs): for kernel profiles")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB streaming copy (k_copy), the byte-count reference of HBM counters")
+    ap.add_argument("--profile-config", choices=sorted(CONFIGS), default=None,
+                    help="only `steps` tuned plain runs of this configuration (for kernel profiles)")
+    ap.add_argument("--no-cpu-configs", action="store_true", help="skip the CPU legs of c1, c3 and c4")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this worker runs on one GPU; --gpus must be 1 "
@@ -273,6 +371,18 @@ def main():
     if not g.init(0):
         raise SystemExit(g.banner)
     g.clear_tuning()
+    if args.profile_config:
+        tune.load(g, args.tuning)
+        b, c, P0, P1, N0, N1, mod, res, classes = CONFIGS[args.profile_config]
+        g.set_classes(classes)
+        it = mps.work_item(b, c, P0, P1, N0, N1, mod, res)
+        ms, _, rels, st = time_plain(g, it, args.steps, args.warmup)
+        log(f"{args.profile_config} plain {ms:.2f} ms/item")
+        print(json.dumps({"config": args.profile_config, "ms_per_item": ms, "relations": len(rels),
+                          "device": item_stats(st, args.steps)}), flush=True)
+        return
+    rates = mps.measure_valu(0)
+    log(f"VALU rates: mad_u64_u32 {rates['mad_u64_u32_per_s']:.3e}/s, fma64 {rates['fma64_per_s']:.3e}/s")
     item = mps.work_item(w["b"], w["c"], w["P0"], w["P1"], w["N0"], w["N1"], w["mod"], w["res"])
     nprimes = prime_count(w["P0"], w["P1"])
     copy_gbps = mps.copy_probe(1 << 30, 3) if args.copy_probe else None
@@ -345,6 +455,8 @@ def main():
                    "products_per_step": {k: s[k] // args.steps for k in ("w_fermat", "w_setup", "w_bsgs")},
                    "flagged_per_step": s["flagged"] // args.steps},
         "roofline": roof,
+        "int_roofline": int_roofline(s, args.steps, ms_per_step / 1000.0, ms_per_step / 1000.0, rates),
+        "valu_rates": rates,
     }
     if ms_untuned is not None:
         line["config"]["untuned_ms_per_step"] = ms_untuned
@@ -374,14 +486,18 @@ def main():
     if copy_gbps is not None:
         line["config"]["hbm_copy_gbps"] = copy_gbps
     if not args.no_configs:
-        line["configs"] = run_configs(mps, g, log)
+        line["configs"] = run_configs(mps, g, log, rates)
     if args.mix:
         line["mix"] = run_mix(mps, g, args.mix, args.mix_load, log, mix_untuned)
         line["p50_item_latency_s"] = line["mix"]["served_p50_s"]
         line["p99_item_latency_s"] = line["mix"]["served_p99_s"]
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = max(1, min(CPU_THREADS_MAX, host_cpu()["affinity"]))
         line["cpu_baseline"] = cpu_baseline(threads, args.cpu_slice)
+        log(f"cpu baseline: {line['cpu_baseline']['value']:.3e} primes/s on {threads} threads")
+        if not args.no_cpu_configs:
+            line["cpu_baseline_configs"] = cpu_baseline_configs(threads)
+            log(f"cpu baseline configs: { {k: round(v['value']) for k, v in line['cpu_baseline_configs'].items()} }")
     print(json.dumps(line), flush=True)
     g.close()
 

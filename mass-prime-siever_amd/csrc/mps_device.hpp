@@ -199,36 +199,31 @@ struct MontField {
     }
     MPS_HD T one() const { return one_; }
     // a * b * 2^-64 mod p, as a Montgomery product over 32-bit words (CIOS: each word of b adds
-    // a * b_i, then m * p with m = t0 * (-p^-1) mod 2^32 clears the low word, which is shifted
-    // out).  With p < 2^62 the partial sums stay below 2^63 after each shift, so two words hold
-    // them and one final subtraction brings the result into [0, p): the same value as the 64-bit
-    // REDC (lo * ninv, high halves of a*b and m*p), in about half its instructions (each product
-    // of 32-bit words with a 64-bit addend is one v_mad_u64_u32).
+    // a * b_i, then m * p with m = t0 * (-p^-1) mod 2^32 clears the low word, which is shifted out).
+    // With a, b < p < 2^62 every partial sum fits 64 bits: a0 * b_i + (the 64-bit accumulator) <
+    // 2^62 + 2^63, so the second word takes the whole accumulator as the addend of one
+    // v_mad_u64_u32 (its low word is the accumulator's low word plus a0 * b1, its high word carries
+    // the accumulator's high word on); m * p0 + u0 < 2^64; and the result is below 2p, so one
+    // subtraction, selected by its borrow, makes it canonical: the same value as the 64-bit REDC
+    // (lo * ninv, high halves of a*b and m*p), in 10 32-bit multiplies.
     MPS_HD T mul(T a, T b) const {
         const uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
         const uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
         const uint32_t p0 = (uint32_t)p, p1 = (uint32_t)(p >> 32), n0 = (uint32_t)ninv;
-        uint64_t s = (uint64_t)a0 * b0;
-        uint32_t t0 = (uint32_t)s;
-        s = (uint64_t)a1 * b0 + (s >> 32);
-        uint32_t t1 = (uint32_t)s, t2 = (uint32_t)(s >> 32);
-        uint32_t m = t0 * n0;
-        s = (uint64_t)m * p0 + t0;
-        s = (uint64_t)m * p1 + t1 + (s >> 32);
-        t0 = (uint32_t)s;
-        t1 = t2 + (uint32_t)(s >> 32);
-        s = (uint64_t)a0 * b1 + t0;
-        t0 = (uint32_t)s;
-        s = (uint64_t)a1 * b1 + t1 + (s >> 32);
-        t1 = (uint32_t)s;
-        t2 = (uint32_t)(s >> 32);
-        m = t0 * n0;
-        s = (uint64_t)m * p0 + t0;
-        s = (uint64_t)m * p1 + t1 + (s >> 32);
-        t0 = (uint32_t)s;
-        t1 = t2 + (uint32_t)(s >> 32);
-        const uint64_t r = ((uint64_t)t1 << 32) | t0;
-        return r >= p ? r - p : r;
+        // word 0 of b: t = (a * b0 + m * p) / 2^32
+        uint64_t u = (uint64_t)a0 * b0;
+        uint32_t m = (uint32_t)u * n0;
+        uint64_t v = (uint64_t)a1 * b0 + (u >> 32);
+        uint64_t w = (uint64_t)m * p0 + (uint32_t)u;  // low word 0
+        uint64_t t = (uint64_t)m * p1 + v + (w >> 32);
+        // word 1: t = (t + a * b1 + m * p) / 2^32
+        u = (uint64_t)a0 * b1 + t;
+        m = (uint32_t)u * n0;
+        v = (uint64_t)a1 * b1 + (u >> 32);
+        w = (uint64_t)m * p0 + (uint32_t)u;
+        t = (uint64_t)m * p1 + v + (w >> 32);
+        uint64_t d;
+        return __builtin_sub_overflow(t, p, &d) ? t : d;
     }
     MPS_HD T dbl(T a) const {
         const uint64_t r = a + a;
