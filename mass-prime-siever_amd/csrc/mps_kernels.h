@@ -168,7 +168,7 @@ size_t bsgs_lds_bytes(const TableGeom& tg);
 
 // Miller-Rabin over the pass's relation records (after launch_bsgs, same stream): composite
 // numbers are replaced by 0.
-constexpr uint32_t kPrimeRecordBlocks = 256;  // one workgroup per CU: a pass may have a record per prime
+constexpr uint32_t kPrimeRecordBlocks = 64;  // a pass may have a record per prime; 64 workgroups keep a short launch
 void launch_prime_records(hipStream_t s, const PassBuffers& pb);
 void launch_bsgs(hipStream_t s, FieldKind fk, const TableGeom& tg, const PassDesc* d, const PassBuffers& pb,
                  uint32_t n_ent, uint32_t grid);
