@@ -33,6 +33,8 @@ MUL_PER_MMUL = 11
 # from a 1 GiB streaming copy), from profiles/k_bsgs_c2_hbm_traffic.txt
 K_BSGS_TRAFFIC_BYTES = 2.932e8
 TUNING_FILE = os.path.join(ROOT, "mass-prime-siever_amd", "tuning_mi355x.json")
+# the autotuner's item set: other items of the same tuning keys as the timed workloads
+TUNE_WORKTODO = os.path.join(ROOT, "mass-prime-siever_amd", "tune_worktodo.txt")
 
 
 def load_tune():
@@ -169,11 +171,28 @@ def plain_mix(g, wis, log, label):
     return svc, rels_all
 
 
+def plain_served(g, wis, arrivals, log):
+    """The plain worker as one FIFO server under an arrival schedule, on the wall clock: an item
+    starts once it has arrived and the item before it has finished; its latency runs from its
+    arrival to its finish."""
+    lat, rels = [], []
+    start = time.perf_counter()
+    for k, (a, w) in enumerate(zip(arrivals, wis)):
+        delay = a - (time.perf_counter() - start)
+        if delay > 0:
+            time.sleep(delay)
+        rels.append(g.run_item(w, chunk_ints=CHUNK)[0])
+        lat.append(time.perf_counter() - start - a)
+        if k % 64 == 63:
+            log(f"mix plain served: {k + 1} items")
+    return lat, rels
+
+
 def run_mix(mps, g, n_items, load, log, plain_untuned):
     """The serving mix (seed 0): the plain per-chunk worker one item after another, the batched
-    engine with every item submitted at once (throughput), and the batched engine under Poisson
-    arrivals at `load` times the plain worker's capacity (latency).  Plain-worker latencies under
-    the same arrivals follow from its measured service times (one FIFO server)."""
+    engine with every item submitted at once (throughput), then the batched engine and the plain
+    worker each under the same Poisson arrivals at `load` times the plain worker's capacity
+    (latency, measured from each item's arrival to its finish)."""
     items, wis, primes = mix_items(mps, n_items)
     svc, rel_plain = plain_mix(g, wis, log, "mix plain")
     t_plain = sum(svc)
@@ -207,13 +226,10 @@ def run_mix(mps, g, n_items, load, log, plain_untuned):
     served_same = all(eng.relations(i) == rel_plain[k] for k, i in enumerate(ids))
     eng.close()
     lat = [d["t_done"] - d["t_submit"] for d in served]
-    # plain worker under the same arrivals: Lindley recursion over its service times
-    plain_lat, free_at = [], 0.0
-    for a, s_ in zip(arrivals, svc):
-        begin = max(a, free_at)
-        free_at = begin + s_
-        plain_lat.append(free_at - a)
     log(f"mix served: p50 {pct(lat, 0.5):.3f} s p99 {pct(lat, 0.99):.3f} s, identical={served_same}")
+    plain_lat, rel_plain_served = plain_served(g, wis, arrivals, log)
+    served_same = served_same and rel_plain_served == rel_plain
+    log(f"mix plain served: p50 {pct(plain_lat, 0.5):.3f} s p99 {pct(plain_lat, 0.99):.3f} s")
     out = {
         "workload": f"serving mix seed 0, {len(wis)} items, {primes} primes",
         "primes": primes,
@@ -224,7 +240,7 @@ def run_mix(mps, g, n_items, load, log, plain_untuned):
         "statuses_ok": all(d["status"] == 0 for d in done + served),
         "served_load": load, "arrival_rate_items_per_s": rate,
         "served_p50_s": pct(lat, 0.5), "served_p99_s": pct(lat, 0.99),
-        "plain_p50_s": pct(plain_lat, 0.5), "plain_p99_s": pct(plain_lat, 0.99),
+        "plain_served_p50_s": pct(plain_lat, 0.5), "plain_served_p99_s": pct(plain_lat, 0.99),
         "batch_stats": bst,
     }
     if plain_untuned is not None:
@@ -399,36 +415,37 @@ def main():
             mix_untuned = (sum(svc_u), rels_mix_u)
     tuning_info = {"source": "defaults"}
     if args.tune:
-        items = [tuple(w[k] for k in ("b", "c", "P0", "P1", "N0", "N1", "mod", "res"))]
-        if args.mix:
-            items += [tuple(d[k] for k in ("b", "c", "P0", "P1", "N0", "N1", "mod", "res"))
-                      for d in load_mix(args.mix)]
+        # tuned live on the held-out set (none of its items is timed here)
+        with open(TUNE_WORKTODO) as f:
+            items = tune.parse_worktodo(f.read())
         t = time.perf_counter()
         table = tune.autotune_items(g, items, log=log)
-        tuning_info = {"source": "autotuned live", "keys": len(table), "tune_s": time.perf_counter() - t}
+        tuning_info = {"source": "autotuned live on " + os.path.relpath(TUNE_WORKTODO, ROOT), "keys": len(table),
+                       "tune_s": time.perf_counter() - t}
+        table_keys = {r["key"] for r in table}
     elif args.tuning != "none":
         if not os.path.exists(args.tuning):  # a missing table would leave the tuned legs untuned
             raise SystemExit(f"tuning table {args.tuning} not found")
         tuning_info = {"source": os.path.relpath(args.tuning, ROOT), "keys": tune.load(g, args.tuning)}
+        with open(args.tuning) as f:
+            table_keys = {r["key"] for r in json.load(f)["entries"]}
+    if tuned:
+        # how much of the timed work has a tuned key (the rest runs with the defaults)
+        tuning_info["c2_key_tuned"] = g.get_tuning(item)[1] in table_keys
+        if args.mix:
+            _, wis_t, _ = mix_items(mps, args.mix)
+            tuning_info["mix_items_with_tuned_key"] = sum(g.get_tuning(x)[1] in table_keys for x in wis_t)
     log(f"tuning: {tuning_info}")
 
     ms_per_step, times, rels, s = time_plain(g, item, args.steps, args.warmup)
     log(f"c2 plain {ms_per_step:.2f} ms/step")
 
     # roofline of the dominant kernel (k_bsgs), from HIP events on the worker's stream
-    launches = max(1, s["n_bsgs_launches"])
-    ms_launch = s["ms_bsgs"] / launches
-    products_launch = s["w_bsgs"] / launches
-    if s["field"] == 1:
-        achieved = products_launch * FLOP_PER_PRODUCT_F64 / (ms_launch * 1e-3) / 1e12
-        roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP64_TFLOPS}
-    else:
-        achieved = products_launch * MUL_PER_PRODUCT_MONT / (ms_launch * 1e-3) / 1e12
-        roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_INT_MUL_TOPS, "unit": "TOP/s",
-                "frac": achieved / PEAK_INT_MUL_TOPS}
-    roof.update({"traffic": K_BSGS_TRAFFIC_BYTES, "kernel": "k_bsgs", "ms_per_launch": ms_launch,
-                 "products_per_launch": products_launch, "tuning": g.get_tuning(item)[0]})
+    kr = kernel_roofline(s, rates)
+    roof = {"bound": "valu", "achieved": kr["achieved"] / 1e12, "peak": kr["peak"] / 1e12,
+            "unit": "TFLOP/s" if s["field"] == 1 else "T 32x32 multiplies/s", "frac": kr["frac"],
+            "traffic": K_BSGS_TRAFFIC_BYTES, "kernel": kr["kernel"], "ms_per_launch": kr["ms_per_launch"],
+            "products_per_launch": kr["products_per_launch"], "tuning": g.get_tuning(item)[0]}
 
     line = {
         "metric": "primes/sec sieved+BSGS'd at p~1e12 (1 GPU)",

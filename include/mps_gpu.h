@@ -31,7 +31,10 @@ typedef struct mps_item {
 /* Status codes.  DECLINED: this chunk must take the caller's CPU path (reference: sieve_filter
  * returning false for a range, table or buffer limit, Gpu.hpp:858, :875, :925).  ERROR: a device
  * error; the GPU sieve stays disabled for the rest of the process (sv_fail, Gpu.hpp:982-987).
- * CAPACITY: the output buffer is too small; *out_n holds the count needed.  STOPPED: the caller
+ * CAPACITY: the output buffer is too small; *out_n holds the count needed (a repeated
+ * mps_gpu_sieve_filter call can need a few more where a pass filled the device's entry list: the
+ * numbers past it are flagged for the host, and which ones depends on the order of the device's
+ * slot reservations; mps_gpu_filter sizes its passes so that this cannot happen).  STOPPED: the caller
  * asked the run to stop (mps_run_opts.stop); the chunks that finished are reported. */
 enum { MPS_OK = 0, MPS_DECLINED = 1, MPS_ERROR = 2, MPS_CAPACITY = 3, MPS_STOPPED = 4 };
 

@@ -44,8 +44,8 @@ public:
         std::vector<uint64_t> out(primes.size() / 16 + 1024);
         size_t n = 0;
         int st = mps_gpu_filter(h_, primes.data(), primes.size(), &it, out.data(), out.size(), &n);
-        if (st == MPS_CAPACITY) {
-            out.resize(n);
+        for (int k = 0; st == MPS_CAPACITY && k < kRetries; ++k) {
+            out.resize(n + n / 8 + 64);
             st = mps_gpu_filter(h_, primes.data(), primes.size(), &it, out.data(), out.size(), &n);
         }
         if (st != MPS_OK) return false;
@@ -61,8 +61,8 @@ public:
         std::vector<uint64_t> out(4096);
         size_t n = 0;
         int st = mps_gpu_sieve_filter(h_, lo, hi, &it, out.data(), out.size(), &n);
-        if (st == MPS_CAPACITY) {
-            out.resize(n);
+        for (int k = 0; st == MPS_CAPACITY && k < kRetries; ++k) {
+            out.resize(n + n / 8 + 64);
             st = mps_gpu_sieve_filter(h_, lo, hi, &it, out.data(), out.size(), &n);
         }
         if (st != MPS_OK) return false;
@@ -73,6 +73,11 @@ public:
     mps_gpu* handle() const { return h_; }
 
 private:
+    // A second call returns a count at least as large only where the device's entry list was full
+    // (the numbers past it go to the host, and which ones depends on the order of slot
+    // reservations), so the capacity retry takes a margin and a few rounds.
+    static constexpr int kRetries = 4;
+
     template <class WorkItemT>
     static mps_item convert(const WorkItemT& item) {
         mps_item it;

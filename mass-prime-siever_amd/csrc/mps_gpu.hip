@@ -827,9 +827,12 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
     }
     const FieldKind fk = pass_field(g, pmax + 1);
     const TableGeom tg = pl.tg;
-    const size_t batch = (size_t)std::min<uint64_t>(g->pass_odds, 1ull << 22);
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
-    if (!ensure_buffers(g, batch)) return MPS_ERROR;
+    if (!ensure_buffers(g, g->pass_odds)) return MPS_ERROR;
+    // every value is a prime and may take an entry, so a batch is at most the entry list: no entry
+    // overflows to the host, and the flagged set does not depend on the order of the device's
+    // slot reservations
+    const size_t batch = (size_t)std::min<uint64_t>(g->pb.ent_cap, 1ull << 22);
     if (g->vals_cap < batch) {
         hipFree(g->d_vals);
         g->d_vals = nullptr;

@@ -108,6 +108,8 @@ int main(int argc, char** argv) {
             if (gpu_sieve) {
                 std::vector<uint64_t> flagged;
                 if (!gpu.sieve_filter(lo, hi, item, flagged)) {  // GPU_strict=1: a decline stops the run
+                    std::fprintf(stderr, "chunk [%llu, %llu) declined: %s\n", (unsigned long long)lo,
+                                 (unsigned long long)hi, mps_gpu_last_error(gpu.handle()));
                     declined.fetch_add(1);
                     continue;
                 }
@@ -121,6 +123,8 @@ int main(int argc, char** argv) {
                 if (n) or_sieve_primes(lo, hi, primes.data(), n);
                 std::vector<std::size_t> todo;
                 if (!gpu.filter(primes, item, todo)) {
+                    std::fprintf(stderr, "chunk [%llu, %llu) declined: %s\n", (unsigned long long)lo,
+                                 (unsigned long long)hi, mps_gpu_last_error(gpu.handle()));
                     declined.fetch_add(1);
                     continue;
                 }

@@ -74,6 +74,24 @@ def test_filter_path_matches_oracle(gpu, mps, oracle):
     assert rels == want
 
 
+def test_filter_full_chunk_is_exact_and_repeatable(gpu, mps, oracle):
+    """GpuSiever::filter on a whole 2^26-integer chunk (3.2M primes, about half of them BSGS
+    entries): the passes fit the device's entry list, so nothing overflows to the host, the flagged
+    indices are the same on every call, and they carry exactly the oracle's relations."""
+    P0, P1 = 10**9, 10**9 + 2**26
+    item = mps.work_item(2, -25, P0, P1, 3_600_000, 5_000_000, 2, 1)
+    primes = oracle.sieve_primes(P0, P1)
+    first, second = [], []
+    assert gpu.filter(primes, item, first), gpu.last_error()
+    assert gpu.filter(primes, item, second), gpu.last_error()
+    assert first == second
+    assert len(first) < 4 * 2224  # the relations plus the few left to the host
+    rels = [r for r in (mps.run_bsgs(int(primes[i]), item) for i in first) if r]
+    want, _ = oracle.process_range(oracle.make_item(2, -25, P0, P1, 3_600_000, 5_000_000, 2, 1), P0, P1, 1 << 22, 8)
+    assert len(want) == 2224
+    assert rels == want
+
+
 @pytest.mark.parametrize("it", ITEMS[:5])
 def test_device_relations_exact(gpu, mps, oracle, it):
     """The device's own verdicts are exact: every record it decides is an oracle relation with
