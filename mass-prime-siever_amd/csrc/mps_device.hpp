@@ -183,6 +183,7 @@ struct MontField {
     static constexpr bool kKeepsOne = true;  // set() takes the Montgomery form of 1
     static constexpr bool kSmallMul = false;  // no mul_small (operands are in Montgomery form)
     static constexpr int kOrderSetupWaves = 6;  // order-mode setup: its LDS allows 6 workgroups per CU
+    static constexpr uint32_t kGiantGroup = 1;  // k_bsgs: giant steps per branch on the lookups
     uint64_t p, ninv, one_;
 
     MPS_HD void init(uint64_t mod) {
@@ -251,6 +252,7 @@ struct F64Field {
     static constexpr bool kKeepsOne = false;  // set() ignores its third argument
     static constexpr bool kSmallMul = true;   // mul_small: a times a small integer
     static constexpr int kOrderSetupWaves = 7;  // order-mode setup: 72 registers, 7 waves per SIMD
+    static constexpr uint32_t kGiantGroup = 2;  // k_bsgs: giant steps per branch on the lookups
     static constexpr uint32_t kHashMul = 0x9E3779B1u;
     double p, pinv;
     uint32_t plo;  // low word of p
@@ -403,6 +405,43 @@ MPS_HD bool fermat2(const F& f, uint64_t p) {
         if ((e >> k) & 1) x = f.dbl(x);
     }
     return f.key(x) == f.key(f.one());
+}
+
+// ---------------------------------------------------------------- BSGS table buckets
+// A bucket is 8 slots of 16-bit fingerprints in four words (slot 2k in the low half of word k),
+// filled in order; 0xFFFF marks an empty slot, so a stored fingerprint is clamped to 0xFFFE, and
+// slot 7 taken means the bucket is full and its keys may go on in the next bucket.
+
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+
+// The low half of hash word hw in both halves of a word (one byte permute on the device).
+MPS_HD uint32_t fp_pair(uint32_t hw) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_perm(hw, hw, 0x01000100u);
+#else
+    return (hw & 0xFFFFu) * 0x10001u;
+#endif
+}
+
+// A slot of the bucket holds the fingerprint that f2 carries in both halves, or the bucket is full
+// (the chain goes on): a slot matches when its half of w ^ f2 is zero, so the packed minimum over
+// the four words has a zero half.  Packed 16-bit operations and one compare stand in for eight
+// compares and the seven scalar ORs that joined their lane masks.  The lookup may use the raw low
+// half of the hash word: a raw 0xFFFF matches every empty slot, and a bucket without one is full,
+// so such a lookup always goes on to the exact walk, which uses the clamped value.
+MPS_HD bool bucket_needs_look(uint4 w, uint32_t f2) {
+    auto x = [f2](uint32_t a) { return __builtin_bit_cast(u16x2, a ^ f2); };
+    const u16x2 m = __builtin_elementwise_min(__builtin_elementwise_min(x(w.x), x(w.y)),
+                                              __builtin_elementwise_min(x(w.z), x(w.w)));
+    return (umin(m.x, m.y) == 0u) | (w.w < 0xFFFF0000u);
+}
+
+// Bit k set when slot k of the bucket holds fingerprint fp.
+MPS_HD uint32_t match_mask(uint4 w, uint32_t fp) {
+    return (((w.x & 0xFFFFu) == fp) ? 1u : 0u) | (((w.x >> 16) == fp) ? 2u : 0u) |
+           (((w.y & 0xFFFFu) == fp) ? 4u : 0u) | (((w.y >> 16) == fp) ? 8u : 0u) |
+           (((w.z & 0xFFFFu) == fp) ? 16u : 0u) | (((w.z >> 16) == fp) ? 32u : 0u) |
+           (((w.w & 0xFFFFu) == fp) ? 64u : 0u) | (((w.w >> 16) == fp) ? 128u : 0u);
 }
 
 }  // namespace mps

@@ -1056,6 +1056,7 @@ __device__ __forceinline__ void tab_insert(const BsgsTable& T, uint32_t boff, ui
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(3))) const u32x4 lds_u32x4;
+typedef __attribute__((address_space(3))) uint16_t lds_u16;
 
 // The bucket at byte offset boff.  The table starts the kernel's dynamic LDS, which starts at LDS
 // address 0 because k_bsgs has no static LDS (bsgs_lds_at_zero checks that on the host), so boff is
@@ -1065,21 +1066,6 @@ __device__ __forceinline__ uint4 read_bucket(const BsgsTable&, uint32_t boff) {
     return make_uint4(w.x, w.y, w.z, w.w);
 }
 
-// Some slot of the bucket holds fingerprint fp, or the bucket is full (the chain goes on).  fp is
-// never 0xFFFF, so slot 7 holding fp implies slot 7 not empty: the full test covers slot 7.
-__device__ __forceinline__ bool bucket_needs_look(uint4 w, uint32_t fp) {
-    const bool m = ((w.x & 0xFFFFu) == fp) | ((w.x >> 16) == fp) | ((w.y & 0xFFFFu) == fp) | ((w.y >> 16) == fp) |
-                   ((w.z & 0xFFFFu) == fp) | ((w.z >> 16) == fp) | ((w.w & 0xFFFFu) == fp);
-    return m | (w.w < 0xFFFF0000u);
-}
-
-// Bit k set when slot k of the bucket holds fingerprint fp.
-__device__ __forceinline__ uint32_t match_mask(uint4 w, uint32_t fp) {
-    return (((w.x & 0xFFFFu) == fp) ? 1u : 0u) | (((w.x >> 16) == fp) ? 2u : 0u) |
-           (((w.y & 0xFFFFu) == fp) ? 4u : 0u) | (((w.y >> 16) == fp) ? 8u : 0u) |
-           (((w.z & 0xFFFFu) == fp) ? 16u : 0u) | (((w.z >> 16) == fp) ? 32u : 0u) |
-           (((w.w & 0xFFFFu) == fp) ? 64u : 0u) | (((w.w >> 16) == fp) ? 128u : 0u);
-}
 
 template <class F>
 struct BsgsEntry {
@@ -1204,6 +1190,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         const uint32_t nbe = (uint32_t)max(4, min((int)tg.nbbits, nbe_m));
         const uint32_t nbk_e = 1u << nbe;
         Tb.omask = (nbk_e - 1u) << 4;
+        const uint32_t spare = 20u * nbk + 2u * (uint32_t)sizeof(HitList);  // a halfword past the table
 
         // baby steps t * B^i, i == tid (mod L); a baby value equal to t for some 0 < i < m means
         // the order of B is below m: the giant steps might not reach the smallest solution, and
@@ -1232,8 +1219,17 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                 const uint32_t w = f.hash_word(v);
                 uint32_t fp, bo;
                 hash_word(w, Tb.omask, fp, bo);
-                tab_insert(Tb, bo, fp);
-                if (w == wt) nt += f.key(v) == f.key(be.t);  // rare: the full key decides
+                // one try in the key's own bucket, written to a spare halfword when the bucket is
+                // full (no branch around the write); a full bucket (rare) sends the key on along the
+                // chain
+                const uint32_t old = atomicAdd(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(Tb.cnt) + (bo >> 2)), 1u);
+                const bool placed = old < 8;
+                *reinterpret_cast<lds_u16*>((size_t)(placed ? bo + 2u * old : spare)) = (uint16_t)fp;
+                if (!placed) tab_insert(Tb, (bo + 16u) & Tb.omask, fp);
+                // a baby value with t's hash word (equal words: equal low words of the residues);
+                // one other than lane 0's first sends the prime to the host, which is exact, so a
+                // word match of two different values costs time only
+                nt += w == wt;
                 v = f.mul(v, bstep);
             };
             for (uint32_t r = 0; r < rounds; ++r) baby();
@@ -1251,16 +1247,46 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             const uint32_t rounds = be.ng / L, rem = be.ng - rounds * L;
             for (uint32_t j = 0; j < be.s; ++j) {
                 T v = gfold ? Mt : f.mul(F::from_bits(Ec[E_G + j]), Mt);
-                auto giant = [&](uint32_t g) {
-                    uint32_t fp, bo;
-                    hash_word(f.hash_word(v), Tb.omask, fp, bo);
+                auto probe = [&](uint32_t& hw, uint32_t& bo, T& vg) {
+                    hw = f.hash_word(v);
+                    bo = (hw >> 16) & Tb.omask;
                     const uint4 w = read_bucket(Tb, bo);
-                    const T vg = v;
+                    vg = v;
                     v = f.mul(v, mstep);  // the next giant value while the LDS read is in flight
-                    if (bucket_needs_look(w, fp)) collect_hit(Tb, bo, fp, f.key(vg), g, j, hits);
+                    return bucket_needs_look(w, fp_pair(hw));
                 };
-                for (uint32_t r = 0; r < rounds; ++r) giant(tid + r * L);
-                if (tid < rem) giant(tid + rounds * L);
+                auto look = [&](uint32_t hw, uint32_t bo, T vg, uint32_t g) {
+                    collect_hit(Tb, bo, min(hw & 0xFFFFu, 0xFFFEu), f.key(vg), g, j, hits);
+                };
+                uint32_t r = 0;
+                constexpr uint32_t G = F::kGiantGroup;
+                if constexpr (G > 1) {
+                    for (; r + G <= rounds; r += G) {  // G steps, then one branch for all of them
+                        uint32_t h[G], b[G];
+                        T vs[G];
+                        bool l[G], any = false;
+#pragma unroll
+                        for (uint32_t k = 0; k < G; ++k) {
+                            l[k] = probe(h[k], b[k], vs[k]);
+                            any |= l[k];
+                        }
+                        if (any) {
+#pragma unroll
+                            for (uint32_t k = 0; k < G; ++k)
+                                if (l[k]) look(h[k], b[k], vs[k], tid + (r + k) * L);
+                        }
+                    }
+                }
+                for (; r < rounds; ++r) {
+                    uint32_t h, b;
+                    T vg;
+                    if (probe(h, b, vg)) look(h, b, vg, tid + r * L);
+                }
+                if (tid < rem) {
+                    uint32_t h, b;
+                    T vg;
+                    if (probe(h, b, vg)) look(h, b, vg, tid + rounds * L);
+                }
             }
             __syncthreads();
             // exact check of every match: the solution k = g*m - i, when in range, gives
@@ -1413,8 +1439,9 @@ TableGeom table_geom(uint64_t m_max, uint64_t waves, uint32_t load) {
     return tg;
 }
 
-// fingerprints (16 B) + a fill counter (4 B) per bucket
-size_t bsgs_lds_bytes(const TableGeom& tg) { return ((size_t)20 << tg.nbbits) + 2 * sizeof(HitList); }
+// fingerprints (16 B) + a fill counter (4 B) per bucket, two hit lists, and 16 bytes that hold the
+// spare halfword of the baby steps' inserts
+size_t bsgs_lds_bytes(const TableGeom& tg) { return ((size_t)20 << tg.nbbits) + 2 * sizeof(HitList) + 16; }
 
 // n_ent: entries, one workgroup each (0 = a persistent grid of `grid` workgroups that drains the
 // pass's entries).

@@ -294,6 +294,51 @@ static void check_jacobi(int scale) {
     std::printf("jacobi: %ld symbols\n", n);
 }
 
+// The BSGS bucket test: bucket_needs_look (packed 16-bit minimum over w ^ fp_pair(hw)) against
+// slot-by-slot compares, on buckets with 0 to 8 filled slots, for lookups of stored, absent,
+// 0xFFFE and raw 0xFFFF fingerprints under random high halves of the hash word; and the property
+// the kernel relies on: a lookup whose clamped fingerprint is stored in the bucket, or whose bucket
+// is full, always looks (so no stored key is missed, and a raw 0xFFFF always looks).
+static void check_bucket(int scale) {
+    const long n = 400000L * scale;
+    for (long it = 0; it < n; ++it) {
+        uint16_t sl[8];
+        const int fill = (int)rnd_below(9);
+        for (int k = 0; k < 8; ++k) {
+            uint16_t v = 0xFFFF;
+            if (k < fill) {
+                const uint64_t r = rnd_below(8);
+                v = r == 0 ? 0xFFFE : r == 1 ? 0 : (uint16_t)umin((uint32_t)(rnd() & 0xFFFF), 0xFFFEu);
+            }
+            sl[k] = v;
+        }
+        uint4 w;
+        w.x = sl[0] | (uint32_t)sl[1] << 16;
+        w.y = sl[2] | (uint32_t)sl[3] << 16;
+        w.z = sl[4] | (uint32_t)sl[5] << 16;
+        w.w = sl[6] | (uint32_t)sl[7] << 16;
+        const uint64_t pick = rnd_below(6);
+        uint32_t lo = pick == 0 ? 0xFFFF : pick == 1 ? 0xFFFE : pick == 2 && fill ? sl[rnd_below(fill)] : rnd() & 0xFFFF;
+        const uint32_t hw = (uint32_t)(rnd() & 0xFFFF0000u) | lo;
+        CHECK(fp_pair(hw) == (lo | lo << 16), "fp_pair %08x", hw);
+        bool any = false;
+        uint32_t mm = 0;
+        for (int k = 0; k < 8; ++k) {
+            any |= sl[k] == lo;
+            if (sl[k] == lo) mm |= 1u << k;
+        }
+        const bool full = sl[7] != 0xFFFF;
+        const bool look = bucket_needs_look(w, fp_pair(hw));
+        CHECK(look == (any || full), "needs_look fill %d lo %04x: %d", fill, lo, (int)look);
+        CHECK(match_mask(w, lo) == mm, "match_mask fill %d lo %04x", fill, lo);
+        const uint32_t clamped = umin(lo, 0xFFFEu);
+        bool stored = false;
+        for (int k = 0; k < fill; ++k) stored |= sl[k] == clamped;
+        if (stored || full || lo == 0xFFFF) CHECK(look, "missed lookup fill %d lo %04x", fill, lo);
+    }
+    std::printf("bucket: %ld lookups on 0-8 filled slots\n", n);
+}
+
 int main(int argc, char** argv) {
     const int scale = argc > 1 ? std::atoi(argv[1]) : 1;
     check_mont(scale);
@@ -301,6 +346,7 @@ int main(int argc, char** argv) {
     check_small_mod(scale);
     check_wide(scale);
     check_jacobi(scale);
+    check_bucket(scale);
     std::printf("ok: %ld checks\n", g_checks);
     return 0;
 }
