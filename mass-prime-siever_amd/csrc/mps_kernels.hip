@@ -271,10 +271,15 @@ __device__ __forceinline__ void put_record(const PassBuffers& pb, uint32_t slot,
 // that cannot divide any b^n + c (t = -c == 0 mod p, no in-class exponent) keeps nothing.  Both
 // decisions rest on the quadratic characters: b^n == t (mod p) needs (t/p) == (b/p)^n.
 
-// Plain mode: info bit 0 = stride 2 (only exponents n_start + 2k + parity can solve), bit 1 = the
-// odd k (parity 1).  gm = gcd(cm, p - 1): with gm == 2 the class exponents share one parity, so the
-// characters decide p; with gm < 2 and (b/p) == -1 they fix the parity of k; with gm > 2 the setup's
-// power test decides.
+// Plain and order mode: info bit 0 = stride 2 (only exponents n_start + 2k + parity can solve), bit
+// 1 = the odd k (parity 1).  gm = gcd(cm, p - 1).  Where every in-class exponent has n_start's
+// parity (cm even) or (b/p) == 1, b^n has one character over the class, which (t/p) must match:
+// such a p is kept only when the character of t * b^-n_start is +1.  This holds whatever gm is.
+// With gm > 2 the setup's power test decides as well; for an even cm (so an even gm) that test
+// implies the character, and order mode's 2-power step tests it, so the filter only drops earlier
+// what the setup would drop.  Plain mode with an odd cm and gm > 2 keeps every p, as the
+// reference's prep_core does (its gm-th power test is then of odd degree).  With (b/p) == -1 and
+// an odd cm the characters fix the parity of k instead (gm == 1; gm > 2 leaves it to the setup).
 __device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, bool& keep, bool& host,
                                              uint32_t& info) {
     if (p < 1000 || (p & 1) == 0 || mod_signed(ip.b, p) == 0) {
@@ -287,10 +292,12 @@ __device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, b
     const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item(ip.jt, -ip.c, p);
     // the character of t * b^-n_start: +1 is needed when k = 0 is possible, i.e. for even k
     const int s0 = (ip.n_start & 1) ? st * sb : st;
-    if (gm > 2) {
+    const bool even_cm = (ip.cm & 1) == 0;
+    if (even_cm || sb == 1) {
+        // one character over the class
+        keep = (gm > 2 && !even_cm && ip.qmode != kModeOrder) || s0 == 1;
+    } else if (gm > 2) {
         keep = true;
-    } else if (gm == 2 || sb == 1) {
-        keep = s0 == 1;  // every exponent of the progression has the character of k = 0
     } else {
         keep = true;  // (b/p) == -1, cm odd: k even needs s0 == 1, k odd needs s0 == -1
         info = 1u | (s0 == 1 ? 0u : 2u);
