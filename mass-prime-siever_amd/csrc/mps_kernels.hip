@@ -1,11 +1,12 @@
 // HIP kernels of the MI355X sieve worker (gfx950, wave64).
 //
 // Stages, one pass = one range of odd numbers [lo, lo + 2*nodd):
-//   k_sieve   segmented odd sieve, one workgroup per 2^17-odd LDS bitmap; odd primes 3..65521 cross
-//             off their odd multiples from max(q^2, first multiple in the segment), so a prime is
-//             never crossed off; survivors are compacted by a block scan of popcounts.
-//   k_filter  per survivor: Jacobi-symbol residue filter (plain mode) or per-prime class split with
-//             a quadratic-residue mask (class mode); wave ballot compaction of what is left.
+//   k_sieve   segmented odd sieve, one 1024-thread workgroup per 2^17-odd LDS bitmap: the primes
+//             3..61 as register combs, 67..4093 a wavefront per prime, 4099..65521 a lane per prime
+//             (LDS atomic ORs); every odd multiple is crossed off and a segment restores the
+//             sieving primes it holds; survivors are compacted by one block reservation.
+//   k_filter  per survivor: quadratic-character filter (plain and order mode) or per-prime class
+//             split with a quadratic-residue mask (class mode); block compaction of what is left.
 //   k_setup   per candidate: base-2 Fermat test, g-th power residue test (class mode), BSGS sizes
 //             and the entry (powers tables, giant starts), compacted by block ballot.
 //   k_setup_order  order mode: per candidate g, b^e, t^e; per (candidate, prime of g) pair a
