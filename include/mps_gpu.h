@@ -139,7 +139,9 @@ int mps_gpu_get_stats(const mps_gpu* g, mps_gpu_stats* s);
 void mps_gpu_reset_stats(mps_gpu* g);
 
 /* Tuning and test hooks (no counterpart in the reference).
- * field: -1 = automatic (FP64 when hi <= 2^50, else Montgomery), 0 = Montgomery, 1 = FP64.
+ * field: -1 = automatic (FP64 when hi <= 2^50, else Montgomery), 0 = Montgomery, 1 = FP64 where
+ * the range allows it (hi <= 2^50; above that the Montgomery field, as automatic).  Any other value,
+ * or an argument outside its range, is an error and leaves every setting as it was.
  * pass_odds: odd numbers per device pass (buffers are sized from it; 0 keeps the current).
  * mcap: largest baby-step count without classes, 256..4096 (0 keeps the current; default 2048).
  * waves: wavefronts that share one BSGS entry, 1, 2 or 4 (0 = chosen per item). */

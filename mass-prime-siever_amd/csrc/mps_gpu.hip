@@ -630,7 +630,9 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
 int mps_gpu_ok(const mps_gpu* g) { return g != nullptr && g->ok ? 1 : 0; }
 
 void mps_gpu_set_classes(mps_gpu* g, int on) {
-    if (g) g->classes_on = on != 0;
+    if (g == nullptr) return;
+    std::lock_guard<std::mutex> lock(g->mu);
+    g->classes_on = on != 0;
 }
 
 int mps_gpu_set_class_policy(mps_gpu* g, int policy) {
@@ -642,20 +644,16 @@ int mps_gpu_set_class_policy(mps_gpu* g, int policy) {
 
 int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap, uint32_t waves) {
     if (g == nullptr) return MPS_ERROR;
+    // every argument is checked before any takes effect
+    if (field < -1 || field > 1) return MPS_ERROR;
+    if (waves && waves != 1 && waves != 2 && waves != 4) return MPS_ERROR;
+    if (mcap && (mcap < 256 || mcap > 4096)) return MPS_ERROR;
+    if (pass_odds && (pass_odds < 4096 || pass_odds > (1ull << 31))) return MPS_ERROR;
     std::lock_guard<std::mutex> lock(g->mu);
     g->field_mode = field;
-    if (waves) {
-        if (waves != 1 && waves != 2 && waves != 4) return MPS_ERROR;
-        g->waves = waves;
-    }
-    if (mcap) {
-        if (mcap < 256 || mcap > 4096) return MPS_ERROR;
-        g->mcap = mcap;
-    }
-    if (pass_odds) {
-        if (pass_odds < 4096 || pass_odds > (1ull << 31)) return MPS_ERROR;
-        g->pass_odds = pass_odds;
-    }
+    if (waves) g->waves = waves;
+    if (mcap) g->mcap = mcap;
+    if (pass_odds) g->pass_odds = pass_odds;
     return MPS_OK;
 }
 

@@ -59,3 +59,19 @@ def test_invalid_shapes_rejected(gpu, mps):
     for bad in (dict(waves=3), dict(load=3), dict(cmax=4096), dict(mcap=100), dict(policy=3)):
         with pytest.raises(ValueError):
             gpu.set_tuning(w, **bad)
+
+
+def test_invalid_configuration_rejected_whole(gpu, mps, oracle):
+    """mps_gpu_configure checks every argument before any takes effect: a call with one bad
+    argument changes nothing (here the Montgomery field a good field argument would force), and the
+    handle then still gives the oracle's relations."""
+    for bad in (dict(field=2), dict(field=-2), dict(field=0, waves=3), dict(field=0, mcap=100),
+                dict(field=0, pass_odds=100), dict(field=0, pass_odds=1 << 32)):
+        assert gpu.configure(**bad) == mps.MPS_ERROR, bad
+    P0, P1 = 10**12, 10**12 + 2**20
+    item = (10, 111, P0, P1, 0, 10_000_000, 6, 4)
+    gpu.reset_stats()
+    rels, _ = gpu.run_item(mps.work_item(*item), chunk_ints=1 << 20)
+    assert gpu.stats()["field"] == 1  # still the automatic choice: FP64 below 2^50
+    want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 8)
+    assert rels == want
