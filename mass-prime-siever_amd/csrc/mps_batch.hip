@@ -16,6 +16,8 @@
 // records into relations (q = ord_p(b)), so host and device work of different items overlap.
 #include <hip/hip_runtime.h>
 
+#include <sched.h>
+
 #include <algorithm>
 #include <array>
 #include <atomic>
@@ -110,7 +112,7 @@ struct mps_batch {
     std::string err;
     std::vector<Slot> slots;
     uint32_t grid_sieve = 0, grid_stage = 0;
-    int in_flight = 0;
+    std::atomic<int> in_flight{0};  // chunks on the slots (changed under mu)
     double busy_since = 0;
     mps_batch_stats st{};
 };
@@ -334,6 +336,16 @@ std::string chunk_name(const Task& t) {
     return buf;
 }
 
+// The CPUs this process may run on (at least 1).
+int host_cpus() {
+    static const int n = [] {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        return sched_getaffinity(0, sizeof(set), &set) == 0 ? std::max(1, CPU_COUNT(&set)) : 4;
+    }();
+    return n;
+}
+
 // One chunk [lo, hi) of an item on slot s: returns the chunk's status.
 int run_chunk(mps_batch* b, Slot& s, const Task& t, std::vector<mps_relation>& rels, uint64_t& prime_flagged,
               double& dev_s, double& host_s) {
@@ -365,8 +377,10 @@ int run_chunk(mps_batch* b, Slot& s, const Task& t, std::vector<mps_relation>& r
     s.recs.resize(n_out);
     for (uint32_t i = 0; i < n_out; ++i) s.recs[i] = mps_dev_relation{s.h_out[2 * i], s.h_out[2 * i + 1]};
     // a pass with many relations (small b and c over long exponent ranges: thousands, each with
-    // q = ord_p(b) from a factorisation of p - 1) is finished on a few threads
-    const int st = finish_records_parallel(&it->w, lo, t.hi, s.recs.data(), n_out, rels, &prime_flagged, 4, 512);
+    // q = ord_p(b) from a factorisation of p - 1) is finished on several threads: the CPUs shared
+    // among the slots that are running chunks, at least 4 and at most 12
+    const int threads = std::min(12, std::max(4, host_cpus() / std::max(1, b->in_flight.load())));
+    const int st = finish_records_parallel(&it->w, lo, t.hi, s.recs.data(), n_out, rels, &prime_flagged, threads, 512);
     host_s += now_s() - t1;
     return st;
 }
