@@ -110,7 +110,7 @@ MPS_HD int jacobi_small(uint32_t a, uint64_t p) {
 
 // Jacobi symbol (x mod p / p) for x given as a signed 64-bit value (b or -c of an item).
 MPS_HD int jacobi_signed(int64_t x, uint64_t p) {
-    const uint64_t ax = x < 0 ? (uint64_t)(-x) : (uint64_t)x;
+    const uint64_t ax = x < 0 ? 0ull - (uint64_t)x : (uint64_t)x;  // 2^63 for INT64_MIN
     int s = 1;
     if (x < 0 && (p & 3) == 3) s = -1;  // (-1/p)
     if (ax < 65536u) return s * jacobi_small((uint32_t)ax, p);

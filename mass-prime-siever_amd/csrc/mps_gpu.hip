@@ -200,17 +200,18 @@ bool order_mode(const mps_item* it, bool classes_on, int class_policy, uint64_t&
     return classes_on && class_policy == 2 && it->N0 >= 0 && it->N1 > it->N0;
 }
 
-JacobiPre jacobi_pre(int64_t x) {
+// The per-item constants of (x/p) for x = (neg ? -1 : 1) * mag.
+JacobiPre jacobi_pre(bool neg, uint64_t mag) {
     JacobiPre J{0, 0, 0, 0};
-    if (x == 0) return J;
-    uint64_t ax = x < 0 ? 0ull - (uint64_t)x : (uint64_t)x;
+    if (mag == 0) return J;
+    uint64_t ax = mag;
     const uint32_t e = (uint32_t)__builtin_ctzll(ax);
     ax >>= e;
     if (ax >= 65536) return J;
     J.a = (uint32_t)ax;
     J.r32 = (uint32_t)((1ull << 32) % ax);
     J.dinv = 1.0 / (double)ax;
-    J.neg = x < 0 ? 1u : 0u;
+    J.neg = neg ? 1u : 0u;
     J.e_odd = e & 1u;
     return J;
 }
@@ -230,8 +231,9 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
     ip.cres = it->cls_res;
     ip.n_start = N0 + ((it->cls_res % cm) + cm - (N0 % cm)) % cm;
     ip.kcount = ip.n_start >= N1 ? 0 : (N1 - ip.n_start + cm - 1) / cm;
-    ip.jb = jacobi_pre(it->b);
-    ip.jt = jacobi_pre(it->c == INT64_MIN ? 0 : -it->c);
+    // b and t = -c by sign and magnitude (|INT64_MIN| = 2^63 is a magnitude like any other)
+    ip.jb = jacobi_pre(it->b < 0, it->b < 0 ? 0ull - (uint64_t)it->b : (uint64_t)it->b);
+    ip.jt = jacobi_pre(it->c > 0, it->c < 0 ? 0ull - (uint64_t)it->c : (uint64_t)it->c);
     ip.cm_r32 = cm < 65536 ? (uint32_t)((1ull << 32) % cm) : 0;
     ip.cm_dinv = cm < 65536 ? 1.0 / (double)cm : 0.0;
     ip.mcap = mcap;

@@ -228,8 +228,19 @@ __device__ __forceinline__ uint64_t mod_signed(int64_t x, uint64_t p) {  // x mo
         const uint64_t u = (uint64_t)x;
         return u < p ? u : u % p;
     }
-    const uint64_t a = (uint64_t)(-x);
+    const uint64_t a = 0ull - (uint64_t)x;  // |x|, 2^63 for INT64_MIN
     const uint64_t r = a < p ? a : a % p;
+    return r == 0 ? 0 : p - r;
+}
+
+// (-x) mod p in [0, p): t = -c of the item, without negating c in 64 bits (-INT64_MIN is 2^63).
+__device__ __forceinline__ uint64_t neg_mod(int64_t x, uint64_t p) {
+    if (x <= 0) {
+        const uint64_t u = 0ull - (uint64_t)x;
+        return u < p ? u : u % p;
+    }
+    const uint64_t u = (uint64_t)x;
+    const uint64_t r = u < p ? u : u % p;
     return r == 0 ? 0 : p - r;
 }
 
@@ -290,7 +301,7 @@ __device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, b
     if (mod_signed(ip.c, p) == 0 || ip.kcount == 0) return;
     const uint64_t pm = pm1_mod(p, ip.cm, ip.cm_r32, ip.cm_dinv);
     const uint64_t gm = ip.cm < (1ull << 32) ? gcd32((uint32_t)ip.cm, (uint32_t)pm) : gcd64(ip.cm, pm);
-    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item(ip.jt, -ip.c, p);
+    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item(ip.jt, (int64_t)(0ull - (uint64_t)ip.c), p);
     // the character of t * b^-n_start: +1 is needed when k = 0 is possible, i.e. for even k
     const int s0 = (ip.n_start & 1) ? st * sb : st;
     const bool even_cm = (ip.cm & 1) == 0;
@@ -331,7 +342,7 @@ __device__ __forceinline__ void filter_class(uint64_t p, const ItemParams& ip, b
     }
     if (mod_signed(ip.c, p) == 0) return;
     const ExpClasses ec = exp_classes(p, ip);
-    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item(ip.jt, -ip.c, p);
+    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item(ip.jt, (int64_t)(0ull - (uint64_t)ip.c), p);
     const uint32_t nk = ip.kcount < ec.ncls ? (uint32_t)ip.kcount : ec.ncls;
     const uint32_t odd0 = (uint32_t)(ip.n_start & 1), odd_step = (uint32_t)(ip.cm & 1);
     uint32_t mask = 0;
@@ -689,7 +700,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 F f;
                 f.init(pc);
                 const uint64_t g = order_g(pc, ip, gx);
-                const uint64_t bmod = mod_signed(ip.b, pc), tv = mod_signed(-ip.c, pc);
+                const uint64_t bmod = mod_signed(ip.b, pc), tv = neg_mod(ip.c, pc);
                 const T rr = f.r2();
                 const uint32_t tz = __builtin_ctzll(g);
                 const uint64_t e = exact_div_odd((pc - 1) >> tz, g >> tz);
@@ -792,7 +803,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 const uint64_t jc = R.Be[cc], mg = R.Te[cc], oL = R.acc[cc], onst = R.lx[cc];
                 const uint64_t m = (uint32_t)mg;
                 const T rr = f.r2();
-                const T bm = f.from(mod_signed(ip.b, pc), rr), tm = f.from(mod_signed(-ip.c, pc), rr);
+                const T bm = f.from(mod_signed(ip.b, pc), rr), tm = f.from(neg_mod(ip.c, pc), rr);
                 uint64_t* E = pb.ent + (uint64_t)slot * kEntWords;
                 E[E_P] = pc;
                 E[E_AUX] = f.aux();
@@ -832,7 +843,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 if (fermat_here && !fermat2(f, p)) {
                     ++fermat_fail;
                 } else {
-                    const uint64_t bmod = mod_signed(ip.b, p), t = mod_signed(-ip.c, p);
+                    const uint64_t bmod = mod_signed(ip.b, p), t = neg_mod(ip.c, p);
                     const T rr = f.r2();
                     const T bm = f.from(bmod, rr), tm = f.from(t, rr);
                     if constexpr (QMODE) {
@@ -918,7 +929,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
             }
             if (keep) {
                 uint64_t* E = pb.ent + (uint64_t)slot * kEntWords;
-                const uint64_t bmod = mod_signed(ip.b, p), t = mod_signed(-ip.c, p);
+                const uint64_t bmod = mod_signed(ip.b, p), t = neg_mod(ip.c, p);
                 const T rr = f.r2();
                 const T bm = f.from(bmod, rr), tm = f.from(t, rr);
                 T B;

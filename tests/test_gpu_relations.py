@@ -287,3 +287,34 @@ def test_order_mode_unsupported_modulus_falls_back(gpu, mps, oracle, mod, res):
     want, _ = oracle.process_range(oracle.make_item(*item), P0, P1, 1 << 20, 16)
     assert st.declined == 0
     assert want and rels == want
+
+
+INT64_MIN, INT64_MAX = -(1 << 63), (1 << 63) - 1
+EXTREME_C = [
+    (2, INT64_MIN, 10**4, 10**4 + 2**16, 0, 10**6, 1, 0),
+    (3, INT64_MAX, 10**5, 10**5 + 2**16, 0, 10**6, 2, 1),
+    (5, INT64_MIN, 3 * 10**5, 3 * 10**5 + 2**16, 10**5, 2 * 10**6, 6, 5),
+]
+
+
+@pytest.mark.parametrize("mode", [(False, 1, -1), (True, 1, -1), (True, 2, -1), (True, 2, 0)],
+                         ids=["plain", "class", "order", "order-mont"])
+@pytest.mark.parametrize("it", EXTREME_C)
+def test_extreme_constants(gpu, mps, oracle, it, mode):
+    """c at the ends of int64: t = -c is 2^63 for c = INT64_MIN, which a 64-bit negation would turn
+    into -2^63 (the wrong residue, and the wrong quadratic character for p == 3 mod 4)."""
+    classes, policy, field = mode
+    print(f"extreme c {it} mode {mode}", flush=True)
+    gpu.set_classes(classes)
+    gpu.set_class_policy(policy)
+    gpu.configure(field, 0)
+    try:
+        rels, st = gpu.run_item(mps.work_item(*it), chunk_ints=1 << 16)
+    finally:
+        gpu.set_classes(True)
+        gpu.set_class_policy(1)
+        gpu.configure(-1, 0)
+    assert st.declined == 0
+    want, _ = oracle.process_range(oracle.make_item(*it), it[2], it[3], 1 << 16, 8)
+    assert want
+    assert rels == want

@@ -171,3 +171,21 @@ def test_prep_plain_is_residue_filter(oracle, it):
                 assert (r == 1) == qr_T, p
             else:
                 assert r == 1 and e.stride == 2 and e.parity == (0 if qr_T else 1), p
+
+
+def test_oracle_extreme_constants(oracle):
+    """c = INT64_MIN and INT64_MAX: the oracle's (p, M) equal brute force with exact integers
+    (t = -c = 2^63 for INT64_MIN), and q = ord_p(b) by sympy."""
+    import sympy
+    for b, c, P0, P1, N0, N1 in [(2, -(1 << 63), 10**4, 10**4 + 2**11, 0, 3000),
+                                 (3, (1 << 63) - 1, 10**4, 10**4 + 2**11, 0, 20000)]:
+        rels, _ = oracle.process_range(oracle.make_item(b, c, P0, P1, N0, N1, 1, 0), P0, P1, 1 << 12, 1)
+        want = []
+        for p in sympy.primerange(P0, P1):
+            t, x = (-c) % p, pow(b, N0, p)
+            for n in range(N0, N1):
+                if x == t:
+                    want.append((p, n, int(sympy.n_order(b, p))))
+                    break
+                x = x * b % p
+        assert want and rels == want
