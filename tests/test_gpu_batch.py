@@ -156,3 +156,27 @@ def test_batch_order_mode_matches_oracle(gpu, mps, want):
     finally:
         eng.close()
         gpu.set_class_policy(1)
+
+
+@pytest.mark.parametrize("policy", [1, 2])
+def test_batch_edge_items(gpu, mps, oracle, policy):
+    """The degenerate and boundary items of the plain worker's edge test (b = 0 and 1, a negative
+    base, N0 < 0, cls_res >= cls_mod, one in-class exponent, cm = 2 * 65537, an empty exponent
+    range) and c = INT64_MIN / INT64_MAX, all in one batch: every item finishes with the oracle's
+    relations and none is declined."""
+    from test_gpu_relations import EDGE_ITEMS, EXTREME_C
+    items = EDGE_ITEMS + EXTREME_C
+    print(f"batch edge items, class policy {policy}", flush=True)
+    gpu.set_class_policy(policy)
+    eng = mps.Batch(gpu, slots=4, pass_odds=PASS_ODDS)
+    try:
+        ids = [eng.submit(mps.work_item(*it), tag=k) for k, it in enumerate(items)]
+        eng.wait_all()
+        done = {d["id"]: d for d in eng.poll()}
+        for i, it in zip(ids, items):
+            assert done[i]["status"] == mps.MPS_OK, (it, eng.last_error())
+            want, _ = oracle.process_range(oracle.make_item(*it), it[2], it[3], 1 << 16, 8)
+            assert eng.relations(i) == want, it
+    finally:
+        eng.close()
+        gpu.set_class_policy(1)

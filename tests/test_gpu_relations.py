@@ -318,3 +318,37 @@ def test_extreme_constants(gpu, mps, oracle, it, mode):
     want, _ = oracle.process_range(oracle.make_item(*it), it[2], it[3], 1 << 16, 8)
     assert want
     assert rels == want
+
+
+EDGE_ITEMS = [
+    (0, -5, 10**4, 10**4 + 2**14, 0, 1000, 1, 0),  # b = 0: every prime divides b, the host decides
+    (1, -1, 10**4, 10**4 + 2**14, 10, 1000, 1, 0),  # b = 1, c = -1: every prime, M = N0, q = 1
+    (1, 4, 10**4, 10**4 + 2**14, 0, 1000, 1, 0),  # b = 1: only primes dividing 1 + c
+    (-3, 7, 10**5, 10**5 + 2**16, 0, 10**5, 2, 1),  # negative base
+    (2, -25, 10**5, 10**5 + 2**16, -100, 10**5, 7, 12),  # N0 < 0, cls_res >= cls_mod
+    (3, -2, 10**5, 10**5 + 2**16, 0, 10**6, 1 << 40, 5),  # one in-class exponent
+    (12, 5, 10**12, 10**12 + 2**16, 0, 10**7, 131074, 3),  # cm = 2 * 65537
+    (10, -1, 10**6, 10**6 + 2**16, 5, 5, 1, 0),  # empty exponent range
+]
+
+
+@pytest.mark.parametrize("mode", [(False, 1, -1), (True, 1, -1), (True, 2, -1), (True, 2, 0)],
+                         ids=["plain", "class", "order", "order-mont"])
+@pytest.mark.parametrize("it", EDGE_ITEMS)
+def test_edge_items(gpu, mps, oracle, it, mode):
+    """Degenerate and boundary work items, in every setup mode, against the oracle; none of them
+    may be declined."""
+    classes, policy, field = mode
+    print(f"edge item {it} mode {mode}", flush=True)
+    gpu.set_classes(classes)
+    gpu.set_class_policy(policy)
+    gpu.configure(field, 0)
+    try:
+        rels, st = gpu.run_item(mps.work_item(*it), chunk_ints=1 << 16)
+    finally:
+        gpu.set_classes(True)
+        gpu.set_class_policy(1)
+        gpu.configure(-1, 0)
+    assert st.declined == 0
+    want, _ = oracle.process_range(oracle.make_item(*it), it[2], it[3], 1 << 16, 8)
+    assert rels == want

@@ -189,3 +189,20 @@ def test_oracle_extreme_constants(oracle):
                     break
                 x = x * b % p
         assert want and rels == want
+
+
+def test_oracle_degenerate_items(oracle):
+    """b = 0, b = 1 and a negative base, an N0 below 0 and cls_res >= cls_mod: the oracle's relations
+    equal brute force with exact integers (M the smallest n >= max(N0, 0), n == res (mod mod),
+    n < N1 with p | b^n + c; q = ord_p(b))."""
+    import sympy
+    items = [(0, -5, 10**4, 10**4 + 2**10, 0, 200, 1, 0), (1, -1, 10**4, 10**4 + 2**10, 10, 200, 1, 0),
+             (-3, 7, 10**3 + 1, 10**3 + 2**11, 0, 3000, 2, 1), (2, -25, 10**3 + 1, 10**3 + 2**11, -100, 3000, 7, 12)]
+    for b, c, P0, P1, N0, N1, mod, res in items:
+        rels, _ = oracle.process_range(oracle.make_item(b, c, P0, P1, N0, N1, mod, res), P0, P1, 1 << 12, 1)
+        want = []
+        for p in sympy.primerange(P0, P1):
+            n = next((n for n in range(max(N0, 0), N1) if n % mod == res % mod and (pow(b, n, p) + c) % p == 0), None)
+            if n is not None:
+                want.append((p, n, int(sympy.n_order(b % p, p)) if b % p else 1))
+        assert rels == want, (b, c)
