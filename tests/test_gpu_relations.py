@@ -352,3 +352,32 @@ def test_edge_items(gpu, mps, oracle, it, mode):
     assert st.declined == 0
     want, _ = oracle.process_range(oracle.make_item(*it), it[2], it[3], 1 << 16, 8)
     assert rels == want
+
+
+MULTIPASS_ITEMS = [
+    (2, -3, 5, 2**16, 0, 1000, 1, 0),  # from the smallest allowed lo: primes below 1000 go to the host
+    (2, -25, 10**9, 10**9 + 2**20, 3_600_000, 5_000_000, 2, 1),
+    (2, -25, 10**9 + 7, 10**9 + 8, 0, 10_000_000, 1, 0),  # one odd number
+    (2, -25, 10**9 + 8, 10**9 + 9, 0, 10_000_000, 1, 0),  # no odd number
+]
+
+
+@pytest.mark.parametrize("pass_odds", [4096, 5000, (1 << 17) + 12345])
+def test_chunks_of_many_passes(gpu, mps, oracle, pass_odds):
+    """Chunks cut into many device passes (pass sizes that are not multiples of the 2^17-odd sieve
+    segment), windows of one odd number or none, the smallest lo and the top of the 62-bit range
+    (a planted relation at p ~ 4.6e18): the relations equal the oracle's."""
+    import json
+    import os
+    planted = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "planted.json")))["items"]
+    items = MULTIPASS_ITEMS + [tuple(it["item"]) for it in planted if it["tag"].startswith("top62")]
+    print(f"pass_odds {pass_odds}", flush=True)
+    assert gpu.configure(-1, pass_odds) == mps.MPS_OK
+    try:
+        for it in items:
+            rels, st = gpu.run_item(mps.work_item(*it), chunk_ints=1 << 18)
+            assert st.declined == 0, it
+            want, _ = oracle.process_range(oracle.make_item(*it), it[2], it[3], 1 << 18, 8)
+            assert rels == want, it
+    finally:
+        gpu.configure(-1, 1 << 25)
