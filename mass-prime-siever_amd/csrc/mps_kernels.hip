@@ -262,6 +262,20 @@ __device__ __forceinline__ int jacobi_item(const JacobiPre& J, int64_t x, uint64
     return s * jacobi32(mod_small_f(p, J.a, J.r32, J.dinv), J.a);
 }
 
+// (x/p) for x = -c with a table of (r/a), r < a, in LDS (k_filter builds it once per workgroup for
+// the odd part a of |c| up to kJacTab): the same signs as jacobi_item, then one lookup of p mod a
+// in place of the Jacobi loop.
+constexpr uint32_t kJacTab = 4096;
+__device__ __forceinline__ int jacobi_item_tab(const JacobiPre& J, int64_t x, uint64_t p, const int8_t* tab) {
+    if (tab == nullptr) return jacobi_item(J, x, p);
+    int s = 1;
+    const uint32_t p8 = (uint32_t)p & 7u;
+    if (J.neg && (p8 & 3u) == 3u) s = -s;
+    if (J.e_odd && (p8 == 3u || p8 == 5u)) s = -s;
+    if ((J.a & 3u) == 3u && (p8 & 3u) == 3u) s = -s;
+    return s * tab[mod_small_f(p, J.a, J.r32, J.dinv)];
+}
+
 __device__ __forceinline__ uint64_t gcd64(uint64_t a, uint64_t b) {
     while (b) {
         const uint64_t t = a % b;
@@ -292,8 +306,8 @@ __device__ __forceinline__ void put_record(const PassBuffers& pb, uint32_t slot,
 // what the setup would drop.  Plain mode with an odd cm and gm > 2 keeps every p, as the
 // reference's prep_core does (its gm-th power test is then of odd degree).  With (b/p) == -1 and
 // an odd cm the characters fix the parity of k instead (gm == 1; gm > 2 leaves it to the setup).
-__device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, bool& keep, bool& host,
-                                             uint32_t& info) {
+__device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, const int8_t* jtab, bool& keep,
+                                             bool& host, uint32_t& info) {
     if (p < 1000 || (p & 1) == 0 || mod_signed(ip.b, p) == 0) {
         host = true;
         return;
@@ -301,7 +315,7 @@ __device__ __forceinline__ void filter_plain(uint64_t p, const ItemParams& ip, b
     if (mod_signed(ip.c, p) == 0 || ip.kcount == 0) return;
     const uint64_t pm = pm1_mod(p, ip.cm, ip.cm_r32, ip.cm_dinv);
     const uint64_t gm = ip.cm < (1ull << 32) ? gcd32((uint32_t)ip.cm, (uint32_t)pm) : gcd64(ip.cm, pm);
-    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item(ip.jt, (int64_t)(0ull - (uint64_t)ip.c), p);
+    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item_tab(ip.jt, (int64_t)(0ull - (uint64_t)ip.c), p, jtab);
     // the character of t * b^-n_start: +1 is needed when k = 0 is possible, i.e. for even k
     const int s0 = (ip.n_start & 1) ? st * sb : st;
     const bool even_cm = (ip.cm & 1) == 0;
@@ -334,15 +348,15 @@ __device__ __forceinline__ ExpClasses exp_classes(uint64_t p, const ItemParams& 
 }
 
 // info = the classes whose parity the characters allow (and that hold an exponent below N1).
-__device__ __forceinline__ void filter_class(uint64_t p, const ItemParams& ip, bool& keep, bool& host,
-                                             uint32_t& info) {
+__device__ __forceinline__ void filter_class(uint64_t p, const ItemParams& ip, const int8_t* jtab, bool& keep,
+                                             bool& host, uint32_t& info) {
     if (p < 1000 || (p & 1) == 0 || mod_signed(ip.b, p) == 0) {
         host = true;
         return;
     }
     if (mod_signed(ip.c, p) == 0) return;
     const ExpClasses ec = exp_classes(p, ip);
-    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item(ip.jt, (int64_t)(0ull - (uint64_t)ip.c), p);
+    const int sb = jacobi_item(ip.jb, ip.b, p), st = jacobi_item_tab(ip.jt, (int64_t)(0ull - (uint64_t)ip.c), p, jtab);
     const uint32_t nk = ip.kcount < ec.ncls ? (uint32_t)ip.kcount : ec.ncls;
     const uint32_t odd0 = (uint32_t)(ip.n_start & 1), odd_step = (uint32_t)(ip.cm & 1);
     uint32_t mask = 0;
@@ -358,6 +372,15 @@ __global__ __launch_bounds__(256) void k_filter(const PassDesc* __restrict__ d, 
     const uint64_t lo = d->lo;
     const ItemParams& ip = d->ip;  // read in place: uniform fields load as scalars, tables by index
     __shared__ uint32_t sh[5];
+    // (r/a) for the odd part a of |c|, r < a, when a is small enough
+    __shared__ int8_t s_jt[kJacTab];
+    const uint32_t ja = ip.jt.a;
+    const int8_t* jtab = nullptr;
+    if (ja > 1 && ja <= kJacTab) {  // uniform
+        for (uint32_t r = threadIdx.x; r < ja; r += 256) s_jt[r] = (int8_t)jacobi32(r, ja);
+        __syncthreads();
+        jtab = s_jt;
+    }
     const uint32_t n = min(pb.cnt->n_surv, pb.surv_cap);
     const uint32_t lane = lane_id();
     uint32_t nflag2 = 0;
@@ -378,9 +401,9 @@ __global__ __launch_bounds__(256) void k_filter(const PassDesc* __restrict__ d, 
                 idx = pb.vals ? i : pb.surv[i];
                 p = pb.vals ? pb.vals[i] : lo + 2 * (uint64_t)idx;
                 if (ip.qmode == kModeClass)
-                    filter_class(p, ip, keep, host, info);
+                    filter_class(p, ip, jtab, keep, host, info);
                 else
-                    filter_plain(p, ip, keep, host, info);
+                    filter_plain(p, ip, jtab, keep, host, info);
             }
             kept[k] = keep;
             val[k] = (uint64_t)idx | ((uint64_t)info << 32);
