@@ -156,6 +156,10 @@ typedef struct mps_tuning {
     uint32_t load;    /* keys per 8-slot LDS table bucket the table is sized for: 2 or 4 (default 2) */
     uint32_t policy;  /* class policy for this kind of item (see mps_gpu_set_class_policy), 0 = the
                          handle's */
+    uint32_t ext;     /* class policy 2: largest medium prime r of p - 1 (19 <= r, at most the
+                         first 192 primes past Qo's) whose power residue narrows a prime's exponent
+                         class, in both fields; 1 = none; 0 = the handle's default: 1024 for
+                         passes in the Montgomery field (p > 2^50), none in the FP64 field */
 } mps_tuning;
 
 /* Autotuner.  Items are keyed by whether classes are on, kind of class modulus, p magnitude
@@ -163,7 +167,8 @@ typedef struct mps_tuning {
  * of `item` whose P range starts at P0, times every candidate shape (with classes on: class
  * policy 1 or 2; waves 1/2/4 x load 2/4 x the baby-step cap of the resulting mode: 512 to 4096
  * plain and order mode, 256 to 2048 class mode; shapes whose table exceeds 64 KB of LDS are
- * skipped) on one pass over the first 2 * sample_odds integers from P0 (device
+ * skipped; then, in order mode, the winner with the medium primes of p - 1 off and on) on one pass
+ * over the first 2 * sample_odds integers from P0 (device
  * time of the pass chain, best of two runs), requires every candidate's records to agree with the
  * default's (the same smallest solution wherever both decided a number; a number may be left to
  * the host under one shape only) and installs the fastest for the key.  best, best_ms, default_ms

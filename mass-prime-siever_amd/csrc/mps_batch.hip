@@ -361,7 +361,7 @@ int run_chunk(mps_batch* b, Slot& s, const Task& t, std::vector<mps_relation>& r
     s.h_desc->ip = it->ip;
     s.h_desc->tg = it->tg;
     const double t0 = now_s();
-    if (!launch_pass(b, s, fk, it->ip.qmode, it->tg)) return MPS_ERROR;
+    if (!launch_pass(b, s, fk, setup_variant(it->ip, fk), it->tg)) return MPS_ERROR;
     hipError_t e = hipStreamSynchronize(s.stream);
     if (e != hipSuccess) return batch_fail(b, e, "pass" + chunk_name(t)), MPS_ERROR;
     const uint32_t n_out = s.h_cnt->n_out;

@@ -151,6 +151,33 @@ MPS_HD uint32_t mod16(uint32_t a, uint32_t m) {
     return (uint32_t)r;
 }
 
+// The inverse of a modulo a prime r < 2^16 (a not a multiple of r), as a^(r-2) mod r: products
+// below r^2 < 2^32, each reduced by mod16_f with dinv = 1.0 / r.
+MPS_HD uint32_t inv_mod_prime(uint32_t a, uint32_t r, double dinv) {
+    uint32_t e = r - 2, x = a, y = 1;
+    while (e) {
+        if (e & 1) y = mod16_f(y * x, r, dinv);
+        e >>= 1;
+        if (e) x = mod16_f(x * x, r, dinv);
+    }
+    return y;
+}
+
+// Narrows the exponent class n == nst (mod L) by n == k (mod r), r a prime below 2^16 (r32 = 2^32
+// mod r, dinv = 1.0 / r): the admissible n are n == nst + L * j (mod L * r) with
+// j = (k - nst) * L^-1 mod r, and nst + L * j is the first of them from nst on.  False (nothing
+// changed) when r divides L.  The caller keeps L * r and nst + L * (r - 1) below 2^64.
+MPS_HD bool narrow_class(uint64_t& L, uint64_t& nst, uint32_t k, uint32_t r, uint32_t r32, double dinv) {
+    const uint32_t Lr = mod_small_f(L, r, r32, dinv);
+    if (Lr == 0) return false;
+    const uint32_t nr = mod_small_f(nst, r, r32, dinv);
+    const uint32_t d = k >= nr ? k - nr : k + r - nr;
+    const uint32_t j = mod16_f(d * inv_mod_prime(Lr, r, dinv), r, dinv);
+    nst += L * j;
+    L *= r;
+    return true;
+}
+
 // a mod m and a / m for a and m below 2^52 (m >= 1): both are exact doubles, so the double quotient
 // is within one of the true one.
 MPS_HD uint64_t divmod52(uint64_t a, uint64_t m, uint64_t& rem) {

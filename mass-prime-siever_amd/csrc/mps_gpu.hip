@@ -217,7 +217,7 @@ JacobiPre jacobi_pre(bool neg, uint64_t mag) {
 }
 
 bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t cmax,
-                 uint32_t waves, ItemParams& ip, uint64_t& m_max, uint2* csz, std::string& why) {
+                 uint32_t waves, uint32_t ext_bound, ItemParams& ip, uint64_t& m_max, uint2* csz, std::string& why) {
     std::memset(&ip, 0, sizeof(ip));
     ip.b = it->b;
     ip.c = it->c;
@@ -275,6 +275,20 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
             ip.oinv[j] = rinv;
             ip.olim[j] = ~0ull / ip.opr[j];
         }
+        // medium primes: from the first table prime that does not divide Qo up to the bound, at most
+        // kExtTested of them (a table index of at most 254, so that a pair slot never reads as a
+        // free one); ext_bound 0 is the default bound in the Montgomery field only (in the FP64
+        // field the narrowing costs the setup about what it saves the BSGS), 1 is none
+        static constexpr ExtTable kExt = make_ext_table();
+        const uint32_t bound = ext_bound == 0 ? kExtDefaultBound : ext_bound;
+        uint32_t e0 = 0, e1 = 0;
+        while (e0 < kExtPrimes - 1 && Qo % kExt.e[e0].r == 0) ++e0;
+        for (e1 = e0; e1 < kExtPrimes - 1 && e1 < e0 + kExtTested && kExt.e[e1].r <= bound; ++e1) {
+        }
+        ip.ext_first = e0;
+        ip.ext_end = e1;
+        ip.ext_kmin = kExtDefaultKmin;
+        ip.ext_fields = ext_bound == 0 ? 1u << FIELD_MONT : (1u << FIELD_MONT) | (1u << FIELD_F64);
         // an entry's class has at most kcount members: m <= sqrt(kcount) in whole rounds, <= mcap
         uint64_t m = 1;
         while (m * m < ip.kcount) ++m;
@@ -328,7 +342,7 @@ uint32_t tune_key(const mps_gpu* g, const mps_item* item, uint64_t P0, bool clas
 thread_local const mps_tuning* t_candidate = nullptr;
 
 mps_tuning item_tuning(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on) {
-    mps_tuning t{g->waves, g->mcap, kClassMaxM, 2, (uint32_t)g->class_policy};
+    mps_tuning t{g->waves, g->mcap, kClassMaxM, 2, (uint32_t)g->class_policy, g->ext_bound};
     std::lock_guard<std::mutex> l(g->tune_mu);
     const mps_tuning* o = nullptr;
     if (t_candidate) {
@@ -343,6 +357,7 @@ mps_tuning item_tuning(const mps_gpu* g, const mps_item* item, uint64_t P0, bool
         if (o->cmax) t.cmax = o->cmax;
         if (o->load) t.load = o->load;
         if (o->policy) t.policy = o->policy;
+        if (o->ext) t.ext = o->ext;
     }
     return t;
 }
@@ -354,11 +369,11 @@ bool plan_item(const mps_gpu* g, const mps_item* item, uint64_t P0, bool classes
     if (waves == 0) {
         // measured on MI355X: two wavefronts per entry for the class tables (m <= 1024) and short
         // plain ranges; four when plain-mode entries run up to 2048 baby steps
-        if (!item_params(item, classes_on, policy, t.mcap, t.cmax, 2, pl.ip, pl.m_max, pl.csz.data(), why)) return false;
+        if (!item_params(item, classes_on, policy, t.mcap, t.cmax, 2, t.ext, pl.ip, pl.m_max, pl.csz.data(), why)) return false;
         // order-mode entries are mostly short (their classes are narrow), whatever the item's cap
         waves = pl.m_max > 1024 && pl.ip.qmode != kModeOrder ? 4 : 2;
     }
-    if (!item_params(item, classes_on, policy, t.mcap, t.cmax, waves, pl.ip, pl.m_max, pl.csz.data(), why)) return false;
+    if (!item_params(item, classes_on, policy, t.mcap, t.cmax, waves, t.ext, pl.ip, pl.m_max, pl.csz.data(), why)) return false;
     pl.tg = table_geom(pl.m_max, waves, t.load);
     if (bsgs_lds_bytes(pl.tg) > kMaxBsgsLds) {
         why = "BSGS table would exceed the LDS budget";
@@ -503,7 +518,7 @@ bool run_pass(mps_gpu* g, const PassBuffers& pb, uint64_t* h_rec, size_t rec_pre
     // class mode sorts tiles of kSetupTile candidates, a workgroup a tile at a time; order mode runs
     // rounds of 256
     const uint32_t setup_grid = blocks_for(g, pc, ip.qmode == kModeClass ? kSetupTile : 256);
-    launch_setup(s, fk, ip.qmode, d, pb, setup_grid);
+    launch_setup(s, fk, setup_variant(ip, fk), d, pb, setup_grid);
     HIPCHK(hipGetLastError(), "setup launch");
     HIPCHK(hipEventRecord(g->ev[3], s), "event");
     if (stop_stage == 2) {
@@ -1041,7 +1056,7 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
     };
     // the defaults (no table entry for the key) give the reference records and time
     const uint32_t key = tune_key(g, item, P0, g->classes_on);
-    force(true, mps_tuning{g->waves, g->mcap, kClassMaxM, 2, (uint32_t)g->class_policy});
+    force(true, mps_tuning{g->waves, g->mcap, kClassMaxM, 2, (uint32_t)g->class_policy, g->ext_bound});
     std::vector<uint64_t> ref, rec;
     double ms = 0, t_default = 1e300;
     int st = MPS_OK;
@@ -1054,8 +1069,30 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
         g->st = keep;
         return st;
     }
-    mps_tuning win{0, 0, 0, 0, 0};
+    mps_tuning win{0, 0, 0, 0, 0, 0};
     double t_win = t_default;
+    // times one candidate shape (skipped when its table would not fit); false on an error
+    auto try_shape = [&](const mps_tuning& t) {
+        Plan pl;
+        std::string why;
+        force(true, t);
+        if (!plan_item(g, item, P0, g->classes_on, pl, why)) return true;  // the table would not fit
+        double tbest = 1e300;
+        for (int rep = 0; rep < 2 && st == MPS_OK; ++rep) {
+            st = timed_sample(g, lo, hi, item, rec, ms);
+            tbest = std::min(tbest, ms);
+            if (st == MPS_OK && !records_agree(rec, ref)) {
+                g->err = "autotune: a candidate shape contradicts the default's records";
+                st = MPS_ERROR;
+            }
+        }
+        if (st != MPS_OK) return false;
+        if (tbest < t_win) {
+            t_win = tbest;
+            win = t;
+        }
+        return true;
+    };
     // with classes on, both class policies that change the setup are candidates; policy 2 only
     // where the item has an order-mode reduction (otherwise it runs as policy 1)
     std::vector<uint32_t> policies{(uint32_t)g->class_policy};
@@ -1070,31 +1107,27 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
     for (uint32_t waves : {1u, 2u, 4u}) {
         for (uint32_t load : {2u, 4u}) {
             for (uint32_t cap : {0u, 1u, 2u, 3u}) {
-                mps_tuning t{waves, qmode ? g->mcap : 512u << cap, qmode ? 256u << cap : kClassMaxM, load, policy};
-                Plan pl;
-                std::string why;
-                force(true, t);
-                if (!plan_item(g, item, P0, g->classes_on, pl, why)) continue;  // the table would not fit
-                double tbest = 1e300;
-                for (int rep = 0; rep < 2 && st == MPS_OK; ++rep) {
-                    st = timed_sample(g, lo, hi, item, rec, ms);
-                    tbest = std::min(tbest, ms);
-                    if (st == MPS_OK && !records_agree(rec, ref)) {
-                        g->err = "autotune: a candidate shape contradicts the default's records";
-                        st = MPS_ERROR;
-                    }
-                }
-                if (st != MPS_OK) break;
-                if (tbest < t_win) {
-                    t_win = tbest;
-                    win = t;
-                }
+                const mps_tuning t{waves, qmode ? g->mcap : 512u << cap, qmode ? 256u << cap : kClassMaxM, load, policy,
+                                   g->ext_bound};
+                if (!try_shape(t)) break;
             }
             if (st != MPS_OK) break;
         }
         if (st != MPS_OK) break;
     }
     if (st != MPS_OK) break;
+    }
+    // the medium primes of p - 1 off and on (in both fields) for the winning shape, in order mode
+    if (st == MPS_OK) {
+        mps_tuning base = win;
+        if (base.waves == 0 && base.mcap == 0 && base.load == 0 && base.policy == 0)  // the defaults won
+            base = mps_tuning{g->waves, g->mcap, kClassMaxM, 2, (uint32_t)g->class_policy, g->ext_bound};
+        if (order_mode(item, g->classes_on, (int)base.policy, Qo))
+            for (uint32_t ext : {1u, kExtDefaultBound}) {
+                mps_tuning t = base;
+                t.ext = ext;
+                if (!try_shape(t)) break;
+            }
     }
     force(false, mps_tuning{});
     g->st = keep;

@@ -21,6 +21,9 @@ constexpr uint32_t kCszEntries = 49 * 32;        // class-size table: (m, giant 
 constexpr uint32_t kSetupTile = 1024;             // candidates a class-mode setup workgroup sorts at a time
 // Setup modes (ItemParams.qmode).
 constexpr uint32_t kModePlain = 0, kModeClass = 1, kModeOrder = 2;
+// Setup kernel variant of order mode with the medium primes of p - 1 (setup_variant); the variant
+// without them is the order-mode kernel with none of their code.
+constexpr uint32_t kModeOrderExt = 3;
 // Order mode: every p - 1 is split by Qo = lcm(cm, B), B the longest of the prefix products
 // 367567200 = 2^5 3^3 5^2 7 11 13 17, then times 19, 23, 29, 31, 37, with Qo below 2^52 (the CRT
 // then stays exact in doubles).
@@ -28,6 +31,40 @@ constexpr uint64_t kOrderQ = 367567200ull;
 constexpr uint32_t kOrderMorePrimes[5] = {19, 23, 29, 31, 37};
 constexpr uint64_t kOrderQLimit = 1ull << 52;
 constexpr uint32_t kOrderMaxPrimes = 16;          // distinct primes of Qo
+// Order mode, medium primes: a prime r of p - 1 outside Qo (r from 19 up to the item's bound) whose
+// power residue b^((p-1)/r) is not 1 fixes n modulo r as well (the discrete log of t^((p-1)/r) to
+// that base, a search of at most r steps), dividing the class's exponent count by r.  The table
+// holds the odd primes from 19 on with the constants of exact tests and remainders by them:
+// r^-1 mod 2^64 and floor((2^64 - 1) / r) (x is a multiple of r exactly when x * r^-1 mod 2^64 is
+// at most that bound), 2^32 mod r, 1/r rounded and r as a double (for x < 2^53, x is a multiple of
+// r exactly when x - rint(x * (1/r)) * r, one FMA, is zero).
+struct ExtPrime {
+    uint32_t r, r32;
+    double dinv;
+    uint64_t rinv, lim;
+    double rd;
+};
+constexpr uint32_t kExtPrimes = 256;              // 19 .. 1669; a table index fits 8 bits
+constexpr uint32_t kExtTested = 3 * 64;           // primes an item may use: three per lane of a wavefront
+constexpr uint32_t kExtDefaultBound = 1024;       // default largest medium prime
+constexpr uint32_t kExtDefaultKmin = 131072;      // classes of fewer exponents are not narrowed
+struct ExtTable {
+    ExtPrime e[kExtPrimes];
+};
+constexpr ExtTable make_ext_table() {
+    ExtTable t{};
+    uint32_t n = 0;
+    for (uint32_t r = 19; n < kExtPrimes; r += 2) {
+        bool prime = true;
+        for (uint32_t d = 3; d * d <= r; d += 2)
+            if (r % d == 0) prime = false;
+        if (!prime) continue;
+        uint64_t inv = r;  // r * r == 1 (mod 8): 3 bits, doubled by each Newton step
+        for (int k = 0; k < 5; ++k) inv *= 2 - (uint64_t)r * inv;
+        t.e[n++] = ExtPrime{r, (uint32_t)((1ull << 32) % r), 1.0 / (double)r, inv, ~0ull / r, (double)r};
+    }
+    return t;
+}
 constexpr uint64_t kHostDecides = ~0ull;          // M of a record the host must decide exactly
 // Relation records a pass of `odds` odd numbers can produce: every prime of the pass in the worst
 // case (an item whose b^n + c vanishes for some in-class n has a relation at every prime), and
@@ -114,6 +151,12 @@ struct ItemParams {
     // per odd prime r of Qo: r^-1 mod 2^64 and floor((2^64 - 1) / r); x is a multiple of r exactly
     // when x * r^-1 mod 2^64 <= that bound, and the product is then x / r
     uint64_t oinv[kOrderMaxPrimes], olim[kOrderMaxPrimes];
+    // order mode, medium primes: table indices [ext_first, ext_end) (ext_first: the first prime not
+    // dividing Qo; empty: off) for classes of at least ext_kmin exponents, in the fields whose bit
+    // (1 << FieldKind) is set in ext_fields
+    uint32_t ext_first, ext_end;
+    uint64_t ext_kmin;
+    uint32_t ext_fields;
 };
 
 // BSGS table geometry of a pass: 2^nbbits buckets of 8 slots; a slot holds a fingerprint of
@@ -144,6 +187,13 @@ struct PassBuffers {
 };
 
 enum FieldKind : int { FIELD_MONT = 0, FIELD_F64 = 1 };
+
+// The setup kernel variant of a pass in field fk: kModeOrderExt for order mode with medium primes
+// in that field, else the item's mode.
+inline uint32_t setup_variant(const ItemParams& ip, FieldKind fk) {
+    const bool ext = ip.qmode == kModeOrder && ip.ext_end > ip.ext_first && ((ip.ext_fields >> fk) & 1u);
+    return ext ? kModeOrderExt : ip.qmode;
+}
 
 // One odd sieving prime with the constants of remainders by it: q, 2^32 mod q and 1/q (rounded).
 struct SievePrime {

@@ -535,7 +535,175 @@ struct OrderRound {
     uint16_t pair[kOrderPairs];                     // candidate | prime index << 8
     uint32_t bin[kOrderBins];
     uint32_t npair, nkeep, base;
+    uint32_t nxp, nq, nkeep2;                       // medium primes: pair slots, candidates, entries
 };
+
+// Medium primes (phases 3a-3c): pair slots of a round (kept index | table index << 8), in the pair
+// array behind the list of kept candidates, with the results behind them; a candidate whose pairs
+// do not fit takes none (dropping a congruence never drops a solution).  A round of 256 candidates
+// has about 50 pairs at 1e12 (fewer than a wavefront, so they are not binned by prime).
+constexpr uint32_t kExtPairs = 512;
+static_assert(256 + 2 * kExtPairs <= kOrderPairs, "medium-prime pairs fit behind the kept list");
+constexpr uint16_t kExtFree = 0xFFFFu;    // slot of a candidate whose pairs did not fit
+constexpr uint32_t kExtNone = 0xFFFFu;    // b^((p-1)/r) == 1 == t^((p-1)/r): no congruence
+constexpr uint32_t kExtDead = 0xFFFEu;    // no n: t^((p-1)/r) is not a power of b^((p-1)/r)
+
+// The medium primes 19 .. 1669 and their constants (r^-1 mod 2^64, floor((2^64 - 1) / r), 2^32 mod
+// r, 1/r), read by uniform index as scalar loads.
+__constant__ ExtTable c_ext = make_ext_table();
+
+// v_readlane of a 64-bit value from lane j (j the same in every lane)
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, uint32_t j) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, (int)j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), (int)j);
+    return (uint64_t)hi << 32 | lo;
+}
+
+// One lane's medium prime in phase 3a: the constants of the exact test "r divides x" (x > 0).  In
+// the FP64 field (x < 2^50) x - rint(x * (1/r)) * r is zero exactly for multiples of r (rint gives
+// x / r then, and the FMA is exact); the Montgomery field tests x * r^-1 mod 2^64 <=
+// floor((2^64 - 1) / r).  A lane past the table's range holds r = 0 and 1/r = 0 (x - 0 is not zero),
+// or r^-1 = 1 and a bound of 0 (x is not at most 0).
+template <class F>
+struct ExtTest {
+    double a;         // FP64 field: 1/r
+    uint32_t rr;      // and r
+    uint64_t ia, ib;  // Montgomery field: r^-1 mod 2^64 and floor((2^64 - 1) / r)
+    __device__ __forceinline__ void load(uint32_t e, uint32_t end) {
+        const bool on = e < end;
+        const ExtPrime& P = c_ext.e[on ? e : 0];
+        if constexpr (F::kSmallMul) {
+            a = on ? P.dinv : 0.0;
+            rr = on ? P.r : 0u;
+        } else {
+            ia = on ? P.rinv : 1ull;
+            ib = on ? P.lim : 0ull;
+        }
+    }
+    __device__ __forceinline__ bool divides(uint64_t x, double xd) const {
+        if constexpr (F::kSmallMul)
+            return fma_rn(-rint(xd * a), (double)rr, xd) == 0.0;
+        else
+            return x * ia <= ib;
+    }
+};
+
+
+// b^e and t^e (t = -c) modulo p in the field f: with the FP64 field, b and -c as small integers
+// when their products with residues stay exact (fpow2_small), else from their field forms.
+template <class F>
+__device__ __forceinline__ void pow_b_t(const F& f, uint64_t p, const ItemParams& ip, uint64_t e, typename F::T& Be,
+                                        typename F::T& Te) {
+    const uint64_t bmod = mod_signed(ip.b, p), tv = neg_mod(ip.c, p);
+    if constexpr (F::kSmallMul) {
+        const double sb = (double)bmod, sc = -(double)ip.c, pd = (double)p;
+        if (sb * pd < 0x1p52 && fabs(sc) * pd < 0x1p52) {
+            fpow2_small(f, sb, sc, e, Be, Te);
+            return;
+        }
+    }
+    const typename F::T rr = f.r2();
+    fpow2(f, f.from(bmod, rr), f.from(tv, rr), e, Be, Te);
+}
+
+// log_x(y) for x of prime order r and y a power of x, by baby and giant steps in registers, shared
+// by the S lanes of one pair (lane j of S, S a power of two): the low words of the keys of the B = 8
+// baby values y * x^i (i < B), in every lane, against the giant values x^(B g), lane j taking
+// g == j + 1 (mod S); about B + 3 + ceil(r / B) / S dependent products, where a plain search would
+// chain r of them (more baby words spill registers in the order-mode setup).  A lane stops at its
+// first word match (g, i) and, after the loop, every lane checks its match exactly (y * x^i against
+// x^(B g), at most B - 1 more products, run once for the whole wavefront); the window of g holds
+// just one log below r (the logs differ by r > B).  Returns the log when this lane found it, r when
+// it found none (x is not of order r, which only a composite modulus allows) and r + 1 when its
+// word match was not exact (the congruence is then not used; a miss of the right log is about
+// 2^-32 per compared word).
+template <class F>
+__device__ __forceinline__ uint32_t small_log(const F& f, typename F::T x, typename F::T y, uint32_t r, uint32_t j,
+                                              uint32_t S, unsigned long long& work) {
+    typedef typename F::T T;
+    constexpr uint32_t B = 8;
+    const uint64_t ky = f.key(y);
+    if (ky == f.key(f.one())) return j == 0 ? 0 : r;
+    uint32_t bw[B];
+    T v = y;
+#pragma unroll
+    for (uint32_t i = 0; i < B; ++i) {
+        bw[i] = (uint32_t)f.key(v);
+        if (i + 1 < B) v = f.mul(v, x);
+    }
+    T X = x;
+    for (uint32_t q = 1; q < B; q <<= 1) X = f.mul(X, X);      // x^B
+    T gv = X, XS = X;
+    for (uint32_t q = 1; q < S; q <<= 1) XS = f.mul(XS, XS);  // x^(B S)
+    for (uint32_t q = 0; q < j; ++q) gv = f.mul(gv, X);        // x^(B (j + 1))
+    const uint32_t ng = (r + B - 1) / B;  // logs 1 .. r - 1 lie in B g - i, g <= ng, i < B
+    work += B + 3;
+    uint32_t g = j + 1, hit = B;
+    for (; g <= ng; g += S) {
+        const uint32_t w = (uint32_t)f.key(gv);
+#pragma unroll
+        for (uint32_t i = 0; i < B; ++i) hit = w == bw[i] ? i : hit;
+        ++work;
+        if (hit < B) break;
+        gv = f.mul(gv, XS);
+    }
+    if (hit >= B) return r;
+    T yx = y;  // y * x^hit, against gv = x^(B g)
+    for (uint32_t i = 1; i < B; ++i)
+        if (i <= hit) yx = f.mul(yx, x);
+    const uint32_t k = g * B - hit;
+    return f.key(yx) == f.key(gv) && k < r ? k : r + 1;
+}
+
+// Phase 3b for the pair (candidate cc, medium prime r of p - 1), lane j of the pair's S lanes: with
+// x = b^((p-1)/r) and y = t^((p-1)/r), b^n == t forces x^n == y.  x of order r (a prime) gives
+// n == k = log_x(y) (mod r) (small_log), which with the candidate's class n == n_start (mod L) means
+// n = n_start + L * J for J == (k - n_start) * L^-1 (mod r): the lane that finds k stores that
+// residue of J.  x == 1 gives nothing when y == 1 and no n otherwise (stored by lane 0), as does an
+// r that divides L already (a prime of cm) or an inexact word match.  The slot starts as kExtDead,
+// so a search that fails (only a composite number can fail it) drops the number.
+template <class F>
+__device__ __forceinline__ void ext_log(const OrderRound<F>& R, uint32_t cc, const ExtPrime& P, const ItemParams& ip,
+                                        uint32_t j, uint32_t S, uint16_t& res, unsigned long long& work) {
+    typedef typename F::T T;
+    F f;
+    const uint64_t pc = R.p[cc];
+    f.set(pc, R.aux[cc], F::kKeepsOne ? R.one[cc] : 0);
+    const uint64_t h = (pc - 1) * P.rinv;  // (p - 1) / r, exact: r divides p - 1
+    T x, y;
+    pow_b_t(f, pc, ip, h, x, y);
+    if (j == 0) work += 2 * bitlen64(h);
+    const uint64_t k1 = f.key(f.one());
+    if (f.key(x) == k1) {
+        if (j == 0) res = f.key(y) == k1 ? kExtNone : kExtDead;
+        return;
+    }
+    const uint32_t k = small_log(f, x, y, P.r, j, S, work);
+    if (k > P.r) res = kExtNone;
+    if (k >= P.r) return;
+    const uint32_t Lr = mod_small_f(R.acc[cc], P.r, P.r32, P.dinv);
+    if (Lr == 0) {
+        res = kExtNone;
+        return;
+    }
+    const uint32_t nr = mod_small_f(R.lx[cc], P.r, P.r32, P.dinv);
+    const uint32_t d = k >= nr ? k - nr : k + P.r - nr;
+    res = (uint16_t)mod16_f(d * inv_mod_prime(Lr, P.r, P.dinv), P.r, P.dinv);
+}
+
+// Baby-step count m (whole rounds of the workgroup, at most mcap) and giant steps ng of an
+// order-mode entry whose class has K exponents below N1 (jc = min(K, p): the exponents repeat with
+// a period dividing p - 1).
+__device__ __forceinline__ void order_sizes(uint64_t K, uint64_t p, const ItemParams& ip, uint64_t& jc, uint64_t& mg) {
+    jc = K < p ? K : p;
+    uint64_t m = isqrt_ceil64(jc);
+    m = (m + ip.lanes - 1) / ip.lanes * ip.lanes;
+    if (m > ip.mcap) m = ip.mcap;
+    if (m == 0) m = 1;
+    uint64_t rr;
+    const uint64_t ng = (jc < (1ull << 51) ? divmod52(jc + m - 1, m, rr) : (jc + m - 1) / m) + 1;
+    mg = m | (ng << 32);
+}
 
 // Phase 2 for the pair (candidate c, j-th prime r of Qo, r^a exactly dividing g): with h = g / r^a,
 // x = (b^e)^h and y = (t^e)^h give n == k (mod d), d = ord(x), k = log_x(y); with n == cres modulo
@@ -659,7 +827,7 @@ __device__ __forceinline__ void write_step_tables(const F& f, uint64_t* E, typen
     E[E_BML] = F::to_bits(lane_stride_pow(f, Bm64, lanes));
 }
 
-template <class F, int MODE>
+template <class F, int MODE, bool EXT = false>
 __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassBuffers& pb) {
     constexpr bool QMODE = MODE == kModeClass;
     const uint64_t lo = d->lo;
@@ -723,21 +891,10 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 F f;
                 f.init(pc);
                 const uint64_t g = order_g(pc, ip, gx);
-                const uint64_t bmod = mod_signed(ip.b, pc), tv = neg_mod(ip.c, pc);
-                const T rr = f.r2();
                 const uint32_t tz = __builtin_ctzll(g);
                 const uint64_t e = exact_div_odd((pc - 1) >> tz, g >> tz);
                 T Be, Te;
-                // b and -c as small integers when their products with residues stay exact
-                const double sb = (double)bmod, sc = -(double)ip.c, pd = (double)pc;
-                if constexpr (F::kSmallMul) {
-                    if (sb * pd < 0x1p52 && fabs(sc) * pd < 0x1p52)
-                        fpow2_small(f, sb, sc, e, Be, Te);
-                    else
-                        fpow2(f, f.from(bmod, rr), f.from(tv, rr), e, Be, Te);
-                } else {
-                    fpow2(f, f.from(bmod, rr), f.from(tv, rr), e, Be, Te);
-                }
+                pow_b_t(f, pc, ip, e, Be, Te);
                 w_setup += 2 * bitlen64(e);
                 R.p[c] = pc;
                 R.aux[c] = f.aux();
@@ -791,30 +948,165 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
             {
                 uint64_t oL = 0, onst = 0, K = 0;
                 if (live && order_finish(R, c, ip, oL, onst, K)) {
-                    const uint64_t pc = R.p[c];
-                    const uint64_t cd = pb.cand[i];
-                    const uint64_t jc = K < pc ? K : pc;  // exponents repeat with period dividing p - 1
-                    uint64_t m = isqrt_ceil64(jc);
-                    m = (m + ip.lanes - 1) / ip.lanes * ip.lanes;  // whole rounds of the workgroup
-                    if (m > ip.mcap) m = ip.mcap;
-                    if (m == 0) m = 1;
-                    const uint64_t ng = (jc + m - 1) / m + 1;
+                    uint64_t jc, mg;
+                    order_sizes(K, R.p[c], ip, jc, mg);
                     R.Be[c] = jc;
-                    R.Te[c] = m | (ng << 32);
+                    R.Te[c] = mg;
                     R.acc[c] = oL;
                     R.lx[c] = onst;
-                    R.g[c] = cd;  // index | info << 32
+                    R.g[c] = pb.cand[i];  // index | info << 32
                     R.pair[atomicAdd(&R.nkeep, 1u)] = (uint16_t)c;
                 }
             }
             __syncthreads();
-            if (c == 0) R.base = R.nkeep ? atomicAdd(&pb.cnt->n_ent, R.nkeep) : 0u;  // one reservation a round
+            const uint32_t nk = R.nkeep;
+            // medium primes of p - 1 (phases 3a-3c, the kernel variant EXT) narrow the classes of at
+            // least ext_kmin exponents; the entries are then the candidates that phase 3c keeps, listed
+            // in R.fail
+            const bool ext = EXT && nk != 0;  // the same in every lane
+            if constexpr (EXT) if (ext) {
+                uint16_t* xp = R.pair + 256;     // pair slots: kept index | table index << 8
+                uint16_t* xr = xp + kExtPairs;   // per slot: n mod r, kExtNone or kExtDead
+                if (c == 0) {
+                    R.nxp = 0;
+                    R.nq = 0;
+                    R.nkeep2 = 0;
+                }
+                __syncthreads();
+                // the kept candidates whose class has at least ext_kmin exponents, listed (in R.fail,
+                // free until 3c) so that the trial divisions run on whole wavefronts of them
+                for (uint32_t k = c; k < nk; k += 256) {
+                    R.gexp[k] = 0;  // no pairs
+                    if (R.Be[R.pair[k]] >= ip.ext_kmin) R.fail[atomicAdd(&R.nq, 1u)] = (uint8_t)k;
+                }
+                __syncthreads();
+                // 3a, a wavefront per listed candidate and a lane per medium prime: lane l tests the table's
+                // primes ext_first + l + 64 u (u < 3), whose constants it loads once a round; the
+                // wavefront's candidates (listed ones wv, wv + 4, ...) are gathered one per lane and
+                // taken one by one with v_readlane, and the primes that divide p - 1 come back as a
+                // ballot.  A candidate keeps at most 8 of them, ascending, while L times their product
+                // stays below 2^62 (so that n_start + L * j and L * r fit).
+                {
+                    const uint32_t nq = R.nq, wv = c >> 6, ln = c & 63u;
+                    ExtTest<F> tests[3];
+#pragma unroll
+                    for (uint32_t u = 0; u < 3; ++u) tests[u].load(ip.ext_first + ln + 64 * u, ip.ext_end);
+                    const uint32_t qj = wv + 4 * ln;  // this lane's candidate of the wavefront
+                    uint64_t xj = 0, Lj = 0;
+                    uint32_t kj = 0;
+                    if (qj < nq) {
+                        kj = R.fail[qj];
+                        const uint32_t cj = R.pair[kj];
+                        xj = R.p[cj] - 1;
+                        Lj = R.acc[cj];
+                    }
+                    const uint32_t nw = nq > wv ? (nq - wv + 3) / 4 : 0;  // the same in the wavefront
+                    for (uint32_t j = 0; j < nw; ++j) {
+                        const uint64_t x = readlane64(xj, j);
+                        uint64_t lp = readlane64(Lj, j);  // L times the primes taken
+                        const uint32_t k = (uint32_t)__builtin_amdgcn_readlane((int)kj, (int)j);
+                        const double xd = (double)x;       // exact in the FP64 field (p < 2^50)
+                        uint64_t es = 0;
+                        uint32_t ne = 0;
+#pragma unroll
+                        for (uint32_t u = 0; u < 3; ++u) {
+                            uint64_t m = __ballot(tests[u].divides(x, xd));
+                            while (m != 0) {  // ascending primes
+                                const uint32_t e = ip.ext_first + 64 * u + (uint32_t)__builtin_ctzll(m);
+                                m &= m - 1;
+                                const ExtPrime P = c_ext.e[e];
+                                if (lp <= (P.lim >> 2) && ne < 8) {
+                                    es |= (uint64_t)e << (8 * ne);
+                                    ++ne;
+                                    lp *= P.r;
+                                }
+                            }
+                        }
+                        uint32_t start = 0;
+                        if (ne != 0) {
+                            if (ln == 0) start = atomicAdd(&R.nxp, ne);
+                            start = (uint32_t)__builtin_amdgcn_readfirstlane((int)start);
+                            const bool fits = start + ne <= kExtPairs;
+                            if (ln < ne && start + ln < kExtPairs) {
+                                const uint32_t e = (uint32_t)(es >> (8 * ln)) & 255u;
+                                xp[start + ln] = fits ? (uint16_t)(k | e << 8) : kExtFree;
+                                xr[start + ln] = (uint16_t)kExtDead;
+                            }
+                            if (!fits) ne = 0;
+                        }
+                        if (ln == 0) R.gexp[k] = start | (uint64_t)ne << 32;
+                    }
+                }
+                __syncthreads();
+                // 3b, S lanes per pair (the lanes of a pair share one wavefront and each computes the
+                // pair's powers): in the FP64 field, whose searches are latency-bound, up to 4 while
+                // they fit the workgroup (a round has about 30 pairs at 1e12); in the Montgomery field,
+                // whose 62-bit powers make the pair instruction-bound, one (measured: c2 setup 2.12
+                // ms per item with one lane, 2.01 with four; c4 22.7 ms with one, 23.9 with four)
+                const uint32_t nxs = min(R.nxp, kExtPairs);
+                const uint32_t lsh = F::kSmallMul ? (nxs <= 64 ? 2 : nxs <= 128 ? 1 : 0) : 0;
+                for (uint32_t t = c; t < (nxs << lsh); t += 256) {
+                    const uint32_t s = t >> lsh, v = xp[s];
+                    if (v != kExtFree)
+                        ext_log(R, R.pair[v & 255u], c_ext.e[v >> 8], ip, t & ((1u << lsh) - 1), 1u << lsh, xr[s], w_setup);
+                }
+                __syncthreads();
+                // 3c, a lane per kept candidate: its primes' residues J == J_r (mod r) combine into
+                // J == J0 (mod M) (narrow_class on the pair (M, J0); M is the product of the primes,
+                // J0 < M), and the class n == n_start (mod L) narrows to n == n_start + L * J0
+                // (mod L * M), whose first member from N0 is n_start + L * J0
+                for (uint32_t k = c; k < nk; k += 256) {
+                    const uint32_t cc = R.pair[k];
+                    const uint64_t se = R.gexp[k];
+                    const uint32_t start = (uint32_t)se, ne = (uint32_t)(se >> 32);
+                    bool alive = true;
+                    uint64_t M = 1, J0 = 0;
+                    for (uint32_t q = 0; q < ne; ++q) {
+                        const uint32_t s = start + q, res = xr[s];
+                        if (res == kExtDead) {
+                            alive = false;
+                            break;
+                        }
+                        if (res == kExtNone) continue;
+                        const ExtPrime P = c_ext.e[xp[s] >> 8];
+                        if (M == 1) {
+                            M = P.r;
+                            J0 = res;
+                        } else {
+                            narrow_class(M, J0, res, P.r, P.r32, P.dinv);
+                        }
+                    }
+                    uint64_t L = R.acc[cc], nst = R.lx[cc];
+                    const bool narrowed = M > 1;
+                    nst += L * J0;
+                    L *= M;
+                    if (alive && narrowed) {
+                        if (nst >= ip.N1) {
+                            alive = false;
+                        } else {
+                            const uint64_t rest = ip.N1 - 1 - nst;
+                            uint64_t rr;
+                            const uint64_t K = (L > rest ? 0 : rest < (1ull << 52) ? divmod52(rest, L, rr) : rest / L) + 1;
+                            uint64_t jc, mg;
+                            order_sizes(K, R.p[cc], ip, jc, mg);
+                            R.Be[cc] = jc;
+                            R.Te[cc] = mg;
+                            R.acc[cc] = L;
+                            R.lx[cc] = nst;
+                        }
+                    }
+                    if (alive) R.fail[atomicAdd(&R.nkeep2, 1u)] = (uint8_t)cc;
+                }
+                __syncthreads();
+            }
+            const uint32_t nent = ext ? R.nkeep2 : nk;
+            if (c == 0) R.base = nent ? atomicAdd(&pb.cnt->n_ent, nent) : 0u;  // one reservation a round
             __syncthreads();
             // phase 4: the entries, a lane per kept candidate (a quarter of the candidates or so: a
             // lane per candidate left most lanes idle through the entry's hundred-odd products)
-            const uint32_t nk = R.nkeep, base = R.base;
-            for (uint32_t k = c; k < nk; k += 256) {
-                const uint32_t cc = R.pair[k], slot = base + k;
+            const uint32_t base = R.base;
+            for (uint32_t k = c; k < nent; k += 256) {
+                const uint32_t cc = ext ? R.fail[k] : R.pair[k], slot = base + k;
                 const uint64_t pc = R.p[cc];
                 const uint32_t idx = (uint32_t)R.g[cc], info = (uint32_t)(R.g[cc] >> 32);
                 if (slot >= pb.ent_cap) {  // no room: the host decides this one
@@ -1047,10 +1339,10 @@ __global__ __launch_bounds__(256) void k_setup(const PassDesc* __restrict__ d, P
 // Order mode at F::kOrderSetupWaves waves per SIMD (FP64 field: 7, i.e. 72 registers, measured
 // faster than 82 registers at 5 waves, c3 setup 1.23 -> 1.13 ms; the Montgomery field's LDS allows
 // 6); the other modes keep their registers.
-template <class F>
+template <class F, bool EXT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(F::kOrderSetupWaves))) void k_setup_order(
     const PassDesc* __restrict__ d, PassBuffers pb) {
-    setup_pass<F, kModeOrder>(d, pb);
+    setup_pass<F, kModeOrder, EXT>(d, pb);
 }
 
 // ------------------------------------------------------------------------------------- bsgs
@@ -1453,7 +1745,8 @@ void launch_filter(hipStream_t s, const PassDesc* d, const PassBuffers& pb, uint
 
 template <class F>
 const void* setup_kernel_f(uint32_t mode) {
-    return mode == kModeOrder   ? reinterpret_cast<const void*>(&k_setup_order<F>)
+    return mode == kModeOrderExt ? reinterpret_cast<const void*>(&k_setup_order<F, true>)
+           : mode == kModeOrder ? reinterpret_cast<const void*>(&k_setup_order<F, false>)
            : mode == kModeClass ? reinterpret_cast<const void*>(&k_setup<F, kModeClass>)
                                 : reinterpret_cast<const void*>(&k_setup<F, kModePlain>);
 }

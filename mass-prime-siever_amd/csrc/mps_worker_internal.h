@@ -24,6 +24,7 @@ struct mps_gpu {
     uint32_t mcap = 2048;
     uint32_t waves = 0;  // 0: by item (4 for plain-mode tables above 1024 baby steps, else 2)
     int class_policy = 1;  // 0: classes for an even cm only (gpu_classes, Math.hpp:324-337); 1: any cm
+    uint32_t ext_bound = 0;  // order mode: largest medium prime of p - 1 used (0: by field, 1: none)
     // autotuned table shapes by item key
     mutable std::mutex tune_mu;
     std::map<uint32_t, mps_tuning> tuned;
@@ -59,7 +60,7 @@ namespace mps {
 // class-mode size table csz[qp * 32 + s] = (m, giant steps per class).
 // Returns false, with the reason, when the BSGS table would not fit.
 bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t mcap, uint32_t cmax,
-                 uint32_t waves, ItemParams& ip, uint64_t& m_max, uint2* csz, std::string& why);
+                 uint32_t waves, uint32_t ext_bound, ItemParams& ip, uint64_t& m_max, uint2* csz, std::string& why);
 
 // Everything a pass of one item needs besides the range: constants, class-size table, geometry.
 struct Plan {

@@ -67,7 +67,7 @@ class GpuStats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     """mps_tuning (include/mps_gpu.h): the BSGS shape of one kind of item."""
     _fields_ = [("waves", ctypes.c_uint32), ("mcap", ctypes.c_uint32), ("cmax", ctypes.c_uint32),
-                ("load", ctypes.c_uint32), ("policy", ctypes.c_uint32)]
+                ("load", ctypes.c_uint32), ("policy", ctypes.c_uint32), ("ext", ctypes.c_uint32)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

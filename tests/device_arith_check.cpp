@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "mps_device.hpp"
+#include "mps_kernels.h"
 
 using namespace mps;
 typedef unsigned __int128 u128;
@@ -339,6 +340,52 @@ static void check_bucket(int scale) {
     std::printf("bucket: %ld lookups on 0-8 filled slots\n", n);
 }
 
+// ------------------------------------------------------------------ medium primes of p - 1
+// The table of the order-mode setup (make_ext_table), the exact divisibility test and quotient by
+// r^-1 mod 2^64, inv_mod_prime for every residue, and narrow_class against the CRT in 128 bits.
+static void check_ext(int scale) {
+    static constexpr ExtTable T = make_ext_table();
+    long n = 0;
+    uint32_t prev = 17;
+    for (uint32_t e = 0; e < kExtPrimes; ++e) {
+        const ExtPrime& P = T.e[e];
+        CHECK(P.r > prev && is_prime(P.r), "ext table: %u after %u", P.r, prev);
+        for (uint32_t q = prev + 2; q < P.r; q += 2) CHECK(!is_prime(q), "ext table skips the prime %u", q);
+        prev = P.r;
+        CHECK(P.rinv * P.r == 1 && P.lim == ~0ull / P.r && P.r32 == (uint32_t)((1ull << 32) % P.r) &&
+                  P.dinv == 1.0 / (double)P.r,
+              "ext constants r=%u", P.r);
+        for (int k = 0; k < 200 * scale; ++k) {
+            const uint64_t x = (k & 1) ? rnd() : P.r * rnd_below(~0ull / P.r + 1);
+            const bool div = x * P.rinv <= P.lim;
+            CHECK(div == (x % P.r == 0), "ext divisibility x=%llu r=%u", (unsigned long long)x, P.r);
+            if (div) CHECK(x * P.rinv == x / P.r, "ext exact quotient");
+            n += 2;
+        }
+        for (uint32_t a = 1; a < P.r; ++a) {
+            CHECK(a * inv_mod_prime(a, P.r, P.dinv) % P.r == 1, "inv_mod_prime a=%u r=%u", a, P.r);
+            ++n;
+        }
+        for (int k = 0; k < 100 * scale; ++k) {
+            uint64_t L = 1 + rnd_below(1ull << (1 + rnd_below(52)));
+            if (k == 0) L = (1ull << 62) / P.r - 1;  // the largest L * r the setup allows
+            const uint64_t nst = rnd_below(1ull << 62);
+            const uint32_t kr = (uint32_t)rnd_below(P.r);
+            uint64_t L2 = L, n2 = nst;
+            const bool ch = narrow_class(L2, n2, kr, P.r, P.r32, P.dinv);
+            if (L % P.r == 0) {
+                CHECK(!ch && L2 == L && n2 == nst, "narrow_class changed a class r | L");
+            } else {
+                CHECK(ch && L2 == L * P.r && n2 >= nst && (n2 - nst) % L == 0 && (u128)(n2 - nst) < (u128)L * P.r &&
+                          n2 % P.r == kr,
+                      "narrow_class L=%llu nst=%llu k=%u r=%u", (unsigned long long)L, (unsigned long long)nst, kr, P.r);
+            }
+            ++n;
+        }
+    }
+    std::printf("medium primes: %u primes %u..%u, %ld checks\n", kExtPrimes, T.e[0].r, prev, n);
+}
+
 int main(int argc, char** argv) {
     const int scale = argc > 1 ? std::atoi(argv[1]) : 1;
     check_mont(scale);
@@ -347,6 +394,7 @@ int main(int argc, char** argv) {
     check_wide(scale);
     check_jacobi(scale);
     check_bucket(scale);
+    check_ext(scale);
     std::printf("ok: %ld checks\n", g_checks);
     return 0;
 }

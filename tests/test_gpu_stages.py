@@ -139,11 +139,29 @@ def order_modulus(cm):
     return Qo
 
 
-def order_class(p, b, c, cm, cres, N0, N1):
+# the library's defaults for the medium primes of p - 1 (order mode): the bound of the Montgomery
+# field's passes (the FP64 field's use none unless a tuning sets a bound) and the smallest class
+# they narrow
+EXT_BOUND, EXT_KMIN = 1024, 131072
+
+
+def ext_primes(Qo, bound):
+    """The medium primes the device tries: the primes from 19 on, from the first that does not
+    divide Qo, up to bound, at most 192 of them."""
+    import sympy
+    ps = list(sympy.primerange(19, 2000))
+    i = 0
+    while i < len(ps) and Qo % ps[i] == 0:
+        i += 1
+    return [r for r in ps[i:i + 192] if r <= bound]
+
+
+def order_class(p, b, c, cm, cres, N0, N1, ext_bound=EXT_BOUND):
     """The exponent class the device must find in order mode, restated from the mathematics: with
     g = gcd(Qo, p - 1) (order_modulus), every solution n of b^n == -c (mod p) has (b^e)^n == (-c)^e,
     e = (p-1)/g; per prime power of g this fixes n modulo the order of the projected b^e (or rules
-    p out), and with n == cres (mod cm) the admissible n are one class mod L.  Returns
+    p out), and with n == cres (mod cm) the admissible n are one class mod L; a class of at least
+    EXT_KMIN exponents is then narrowed by the medium primes of p - 1 (below).  Returns
     (L, first n >= N0, count below N1) or None."""
     import math
     import sympy
@@ -170,7 +188,35 @@ def order_class(p, b, c, cm, cres, N0, N1):
     n0, L = int(sol[0]), int(sol[1])
     first = N0 + (n0 - N0) % L
     K = (N1 - 1 - first) // L + 1 if first < N1 else 0
-    return (L, first, K) if K > 0 else None
+    if K <= 0:
+        return None
+    if ext_bound is None or min(K, p) < EXT_KMIN:
+        return (L, first, K)
+    # medium primes r of p - 1 (ascending, at most 8, while L times their product stays below
+    # 2^62): every solution has x^n == y for x = b^((p-1)/r), y = (-c)^((p-1)/r); x of order r fixes
+    # n mod r, x == 1 rules p out unless y == 1
+    taken, lp = [], L
+    for r in ext_primes(Qo, ext_bound):
+        if (p - 1) % r == 0 and lp <= ((2**64 - 1) // r) >> 2 and len(taken) < 8:
+            taken.append(r)
+            lp *= r
+    for r in taken:
+        x, y = pow(b % p, (p - 1) // r, p), pow((-c) % p, (p - 1) // r, p)
+        if x == 1:
+            if y != 1:
+                return None
+            continue
+        if L % r == 0:
+            continue
+        k, v = 0, 1
+        while v != y and k < r:
+            v, k = v * x % p, k + 1
+        assert k < r  # x has prime order r and y is a power of x (p is prime)
+        n0, L = (int(v) for v in solve_congruence((first, L), (k, r)))
+        first = N0 + (n0 - N0) % L
+        if first >= N1:
+            return None
+    return (L, first, (N1 - 1 - first) // L + 1)
 
 
 ORDER_ITEMS = [
@@ -181,24 +227,29 @@ ORDER_ITEMS = [
 ]
 
 
+@pytest.mark.parametrize("ext", [0, 1024, 1, 300])
 @pytest.mark.parametrize("it", ORDER_ITEMS)
-def test_order_mode_entries_match_power_residues(gpu, mps, oracle, it):
+def test_order_mode_entries_match_power_residues(gpu, mps, oracle, it, ext):
     """Under class policy 2 the prime entries are exactly the primes whose power-residue class is
-    non-empty, and each entry's baby/giant split covers the class."""
+    non-empty (narrowed by the medium primes of p - 1 where they are on: by default in the
+    Montgomery field's passes, p > 2^50; in both fields with an explicit bound), and each entry's
+    baby/giant split covers the class."""
     import sympy
     b, c, lo, hi, N0, N1, mod, res = it
-    print(f"order entries {it}", flush=True)
-    gpu.set_class_policy(2)
+    print(f"order entries {it} ext {ext}", flush=True)
+    w = mps.work_item(b, c, lo, hi, N0, N1, mod, res)
+    gpu.set_tuning(w, policy=2, ext=ext)
     try:
-        ent = gpu.debug_stage(2, lo, hi, mps.work_item(b, c, lo, hi, N0, N1, mod, res))
+        ent = gpu.debug_stage(2, lo, hi, w)
     finally:
-        gpu.set_class_policy(1)
+        gpu.clear_tuning()
+    bound = (EXT_BOUND if hi > 2**50 else None) if ext == 0 else (None if ext == 1 else ext)
     got = {int(p): (int(m), int(ng)) for p, m, ng, _ in ent.tolist() if sympy.isprime(int(p))}
     want = {}
     for x in oracle.sieve_primes(lo, hi).tolist():
         if x < 1000 or b % x == 0 or c % x == 0:
             continue
-        oc = order_class(x, b, c, max(1, mod), res, N0, N1)
+        oc = order_class(x, b, c, max(1, mod), res, N0, N1, bound)
         if oc:
             want[x] = min(oc[2], x)
     assert set(got) == set(want)

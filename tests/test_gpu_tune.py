@@ -24,11 +24,14 @@ def clean_tuning(gpu):
 
 
 @pytest.mark.parametrize("it", [CLASS_ITEM, PLAIN_ITEM, SHORT_ITEM])
-@pytest.mark.parametrize("shape", [dict(waves=1, load=4, cmax=512, mcap=1024, policy=1),
-                                   dict(waves=4, load=2, cmax=2048, mcap=2048, policy=2),
-                                   dict(waves=2, load=4, cmax=1024, mcap=4096, policy=1),
-                                   dict(waves=1, load=2, cmax=512, mcap=1024, policy=2)])
+@pytest.mark.parametrize("shape", [dict(waves=1, load=4, cmax=512, mcap=1024, policy=1, ext=1024),
+                                   dict(waves=4, load=2, cmax=2048, mcap=2048, policy=2, ext=1),
+                                   dict(waves=2, load=4, cmax=1024, mcap=4096, policy=1, ext=1024),
+                                   dict(waves=1, load=2, cmax=512, mcap=1024, policy=2, ext=1669),
+                                   dict(waves=2, load=2, cmax=1024, mcap=512, policy=2, ext=100)])
 def test_every_shape_gives_oracle_relations(gpu, mps, oracle, it, shape):
+    """Every BSGS shape, class policy and medium-prime bound (ext: 1 = none, up to the whole
+    table) gives the oracle's relations."""
     print(f"shape {shape} item {it}", flush=True)
     w = mps.work_item(*it)
     gpu.set_tuning(w, **shape)
