@@ -535,7 +535,7 @@ struct OrderRound {
     uint16_t pair[kOrderPairs];                     // candidate | prime index << 8
     uint32_t bin[kOrderBins];
     uint32_t npair, nkeep, base;
-    uint32_t nxp, nq, nkeep2;                       // medium primes: pair slots, candidates, entries
+    uint32_t nxp, nq, na, nb, nkeep2;               // medium primes: pair slots, candidates, lists
 };
 
 // Medium primes (phases 3a-3c): pair slots of a round (kept index | table index << 8), in the pair
@@ -543,7 +543,7 @@ struct OrderRound {
 // do not fit takes none (dropping a congruence never drops a solution).  A round of 256 candidates
 // has about 50 pairs at 1e12 (fewer than a wavefront, so they are not binned by prime).
 constexpr uint32_t kExtPairs = 512;
-static_assert(256 + 2 * kExtPairs <= kOrderPairs, "medium-prime pairs fit behind the kept list");
+static_assert(256 + 2 * kExtPairs + 256 <= kOrderPairs, "medium-prime pairs and list B fit behind the kept list");
 constexpr uint16_t kExtFree = 0xFFFFu;    // slot of a candidate whose pairs did not fit
 constexpr uint32_t kExtNone = 0xFFFFu;    // b^((p-1)/r) == 1 == t^((p-1)/r): no congruence
 constexpr uint32_t kExtDead = 0xFFFEu;    // no n: t^((p-1)/r) is not a power of b^((p-1)/r)
@@ -827,6 +827,45 @@ __device__ __forceinline__ void write_step_tables(const F& f, uint64_t* E, typen
     E[E_BML] = F::to_bits(lane_stride_pow(f, Bm64, lanes));
 }
 
+// Phase 4 of order mode: the entry of candidate cc of the round (its class n_start + k*L, k < jc,
+// and its sizes from phase 3 or 3c) at entry slot `slot`, or a record for the host when the entry
+// buffer is full.
+template <class F>
+__device__ __forceinline__ void order_entry(const OrderRound<F>& R, uint32_t cc, uint32_t slot, const ItemParams& ip,
+                                            const PassBuffers& pb, unsigned long long& w_setup,
+                                            unsigned long long& w_bsgs) {
+    typedef typename F::T T;
+    const uint64_t pc = R.p[cc];
+    const uint32_t idx = (uint32_t)R.g[cc], info = (uint32_t)(R.g[cc] >> 32);
+    if (slot >= pb.ent_cap) {  // no room: the host decides this one
+        put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? idx : pc, kHostDecides);
+        return;
+    }
+    F f;
+    f.set(pc, R.aux[cc], F::kKeepsOne ? R.one[cc] : 0);
+    const uint64_t jc = R.Be[cc], mg = R.Te[cc], oL = R.acc[cc], onst = R.lx[cc];
+    const uint64_t m = (uint32_t)mg;
+    const T rr = f.r2();
+    const T bm = f.from(mod_signed(ip.b, pc), rr), tm = f.from(neg_mod(ip.c, pc), rr);
+    uint64_t* E = pb.ent + (uint64_t)slot * kEntWords;
+    E[E_P] = pc;
+    E[E_AUX] = f.aux();
+    E[E_ONE] = F::to_bits(f.one());
+    E[E_T] = F::to_bits(tm);
+    E[E_MG] = mg;
+    E[E_SSRC] = 1u | ((uint64_t)(info & 0xFFFFFFu) << 8) | ((uint64_t)idx << 32);
+    T B, G1;
+    fpow_2e(f, bm, oL, onst, B, G1);
+    w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
+    E[E_G] = F::to_bits(G1);
+    E[E_NB] = onst;
+    E[E_META] = (1u << 11) | (1u << 12);
+    E[E_KLIM] = jc;
+    E[E_DJ] = oL;
+    write_step_tables(f, E, B, G1, tm, m, ip.lanes, w_setup);
+    w_bsgs += m + (mg >> 32);
+}
+
 template <class F, int MODE, bool EXT = false>
 __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassBuffers& pb) {
     constexpr bool QMODE = MODE == kModeClass;
@@ -970,6 +1009,8 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 if (c == 0) {
                     R.nxp = 0;
                     R.nq = 0;
+                    R.na = 0;
+                    R.nb = 0;
                     R.nkeep2 = 0;
                 }
                 __syncthreads();
@@ -1038,25 +1079,53 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                     }
                 }
                 __syncthreads();
+                // the kept candidates without pairs are final (list A, in R.fail: their entries are
+                // written now, beside the pairs' searches); the others (list B, kept indices behind the
+                // pair slots) wait for 3c
+                uint16_t* lb = xr + kExtPairs;
+                for (uint32_t k = c; k < nk; k += 256) {
+                    if ((R.gexp[k] >> 32) == 0)
+                        R.fail[atomicAdd(&R.na, 1u)] = (uint8_t)R.pair[k];
+                    else
+                        lb[atomicAdd(&R.nb, 1u)] = (uint16_t)k;
+                }
+                __syncthreads();
+                const uint32_t na = R.na;
+                if (c == 0) R.base = na ? atomicAdd(&pb.cnt->n_ent, na) : 0u;
+                __syncthreads();
                 // 3b, S lanes per pair (the lanes of a pair share one wavefront and each computes the
                 // pair's powers): in the FP64 field, whose searches are latency-bound, up to 4 while
                 // they fit the workgroup (a round has about 30 pairs at 1e12); in the Montgomery field,
                 // whose 62-bit powers make the pair instruction-bound, one (measured: c2 setup 2.12
-                // ms per item with one lane, 2.01 with four; c4 22.7 ms with one, 23.9 with four)
-                const uint32_t nxs = min(R.nxp, kExtPairs);
-                const uint32_t lsh = F::kSmallMul ? (nxs <= 64 ? 2 : nxs <= 128 ? 1 : 0) : 0;
-                for (uint32_t t = c; t < (nxs << lsh); t += 256) {
-                    const uint32_t s = t >> lsh, v = xp[s];
-                    if (v != kExtFree)
-                        ext_log(R, R.pair[v & 255u], c_ext.e[v >> 8], ip, t & ((1u << lsh) - 1), 1u << lsh, xr[s], w_setup);
+                // ms per item with one lane, 2.01 with four; c4 22.7 ms with one, 23.9 with four).
+                // The pairs take the first wavefronts and list A's entries the others, so that the
+                // two kinds of dependent chains run side by side.
+                {
+                    const uint32_t nxs = min(R.nxp, kExtPairs);
+                    const uint32_t lsh = F::kSmallMul ? (nxs <= 64 ? 2 : nxs <= 128 ? 1 : 0) : 0;
+                    const uint32_t pt = nxs << lsh, wp = min(4u, (pt + 63) / 64), ew = 4 - wp;
+                    if (c < wp * 64) {
+                        for (uint32_t t = c; t < pt; t += wp * 64) {
+                            const uint32_t s = t >> lsh, v = xp[s];
+                            if (v != kExtFree)
+                                ext_log(R, R.pair[v & 255u], c_ext.e[v >> 8], ip, t & ((1u << lsh) - 1), 1u << lsh, xr[s],
+                                        w_setup);
+                        }
+                    }
+                    if (ew == 0 || c >= wp * 64) {
+                        const uint32_t base = R.base;
+                        for (uint32_t q = ew ? c - wp * 64 : c; q < na; q += ew ? ew * 64 : 256)
+                            order_entry(R, R.fail[q], base + q, ip, pb, w_setup, w_bsgs);
+                    }
                 }
                 __syncthreads();
-                // 3c, a lane per kept candidate: its primes' residues J == J_r (mod r) combine into
-                // J == J0 (mod M) (narrow_class on the pair (M, J0); M is the product of the primes,
-                // J0 < M), and the class n == n_start (mod L) narrows to n == n_start + L * J0
+                // 3c, a lane per candidate of list B: its primes' residues J == J_r (mod r) combine
+                // into J == J0 (mod M) (narrow_class on the pair (M, J0); M is the product of the
+                // primes, J0 < M), and the class n == n_start (mod L) narrows to n == n_start + L * J0
                 // (mod L * M), whose first member from N0 is n_start + L * J0
-                for (uint32_t k = c; k < nk; k += 256) {
-                    const uint32_t cc = R.pair[k];
+                const uint32_t nb = R.nb;
+                for (uint32_t qb = c; qb < nb; qb += 256) {
+                    const uint32_t k = lb[qb], cc = R.pair[k];
                     const uint64_t se = R.gexp[k];
                     const uint32_t start = (uint32_t)se, ne = (uint32_t)(se >> 32);
                     bool alive = true;
@@ -1105,38 +1174,8 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
             // phase 4: the entries, a lane per kept candidate (a quarter of the candidates or so: a
             // lane per candidate left most lanes idle through the entry's hundred-odd products)
             const uint32_t base = R.base;
-            for (uint32_t k = c; k < nent; k += 256) {
-                const uint32_t cc = ext ? R.fail[k] : R.pair[k], slot = base + k;
-                const uint64_t pc = R.p[cc];
-                const uint32_t idx = (uint32_t)R.g[cc], info = (uint32_t)(R.g[cc] >> 32);
-                if (slot >= pb.ent_cap) {  // no room: the host decides this one
-                    put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? idx : pc, kHostDecides);
-                    continue;
-                }
-                F f;
-                f.set(pc, R.aux[cc], F::kKeepsOne ? R.one[cc] : 0);
-                const uint64_t jc = R.Be[cc], mg = R.Te[cc], oL = R.acc[cc], onst = R.lx[cc];
-                const uint64_t m = (uint32_t)mg;
-                const T rr = f.r2();
-                const T bm = f.from(mod_signed(ip.b, pc), rr), tm = f.from(neg_mod(ip.c, pc), rr);
-                uint64_t* E = pb.ent + (uint64_t)slot * kEntWords;
-                E[E_P] = pc;
-                E[E_AUX] = f.aux();
-                E[E_ONE] = F::to_bits(f.one());
-                E[E_T] = F::to_bits(tm);
-                E[E_MG] = mg;
-                E[E_SSRC] = 1u | ((uint64_t)(info & 0xFFFFFFu) << 8) | ((uint64_t)idx << 32);
-                T B, G1;
-                fpow_2e(f, bm, oL, onst, B, G1);
-                w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
-                E[E_G] = F::to_bits(G1);
-                E[E_NB] = onst;
-                E[E_META] = (1u << 11) | (1u << 12);
-                E[E_KLIM] = jc;
-                E[E_DJ] = oL;
-                write_step_tables(f, E, B, G1, tm, m, ip.lanes, w_setup);
-                w_bsgs += m + (mg >> 32);
-            }
+            for (uint32_t k = c; k < nent; k += 256)
+                order_entry(R, ext ? R.fail[k] : R.pair[k], base + k, ip, pb, w_setup, w_bsgs);
         }
         for (uint32_t q = 0; q < (OMODE ? 0 : kTile / 256); ++q) {
             const uint32_t li = SORT ? (uint32_t)s_order[q * 256 + threadIdx.x] : q * 256 + threadIdx.x;
