@@ -244,6 +244,8 @@ def test_order_mode_entries_match_power_residues(gpu, mps, oracle, it, ext):
     finally:
         gpu.clear_tuning()
     bound = (EXT_BOUND if hi > 2**50 else None) if ext == 0 else (None if ext == 1 else ext)
+    nums = [int(p) for p, _, _, _ in ent.tolist()]
+    assert len(set(nums)) == len(nums), "an entry was written twice (or a slot kept an old entry)"
     got = {int(p): (int(m), int(ng)) for p, m, ng, _ in ent.tolist() if sympy.isprime(int(p))}
     want = {}
     for x in oracle.sieve_primes(lo, hi).tolist():
