@@ -396,6 +396,19 @@ MPS_HD void fpow2_small(const F& f, double sa, double sb, uint64_t e, double& ra
     }
 }
 
+// s^e for an integer s with |s| * p < 2^52 (F64Field), left to right with mul_small, as
+// fpow2_small for one base.
+template <class F>
+MPS_HD double fpow_small(const F& f, double s, uint64_t e) {
+    if (e == 0) return 1.0;
+    double r = f.mul_small(1.0, s);
+    for (int k = 62 - __builtin_clzll(e); k >= 0; --k) {
+        r = f.mul(r, r);
+        if ((e >> k) & 1) r = f.mul_small(r, s);
+    }
+    return r;
+}
+
 // a^e and b^e in one loop (the exponent's bits are walked once for both); E is uint32_t or
 // uint64_t.
 template <class F, class E>

@@ -655,6 +655,27 @@ __device__ __forceinline__ uint32_t small_log(const F& f, typename F::T x, typen
     return f.key(yx) == f.key(gv) && k < r ? k : r + 1;
 }
 
+// t^e (tbase) or b^e modulo p, in the form pow_b_t picks for the pair (b, t): the small-integer
+// form only when both bases allow it, so that the two lanes of a pair run the same instructions.
+template <class F>
+__device__ __forceinline__ typename F::T pow_b_or_t(const F& f, uint64_t p, const ItemParams& ip, uint64_t e, bool tbase) {
+    const uint64_t bmod = mod_signed(ip.b, p), tv = neg_mod(ip.c, p);
+    if constexpr (F::kSmallMul) {
+        const double sb = (double)bmod, sc = -(double)ip.c, pd = (double)p;
+        if (sb * pd < 0x1p52 && fabs(sc) * pd < 0x1p52) return fpow_small(f, tbase ? sc : sb, e);
+    }
+    const typename F::T rr = f.r2();
+    return fpow(f, f.from(tbase ? tv : bmod, rr), e);
+}
+
+// A value of lane `src` of the wavefront (every lane of the pair active).
+template <class T>
+__device__ __forceinline__ T lane_value(T v, uint32_t src) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = (uint32_t)__shfl((int)(uint32_t)u, (int)src), hi = (uint32_t)__shfl((int)(uint32_t)(u >> 32), (int)src);
+    return __builtin_bit_cast(T, (uint64_t)hi << 32 | lo);
+}
+
 // Phase 3b for the pair (candidate cc, medium prime r of p - 1), lane j of the pair's S lanes: with
 // x = b^((p-1)/r) and y = t^((p-1)/r), b^n == t forces x^n == y.  x of order r (a prime) gives
 // n == k = log_x(y) (mod r) (small_log), which with the candidate's class n == n_start (mod L) means
@@ -671,7 +692,14 @@ __device__ __forceinline__ void ext_log(const OrderRound<F>& R, uint32_t cc, con
     f.set(pc, R.aux[cc], F::kKeepsOne ? R.one[cc] : 0);
     const uint64_t h = (pc - 1) * P.rinv;  // (p - 1) / r, exact: r divides p - 1
     T x, y;
-    pow_b_t(f, pc, ip, h, x, y);
+    if (S >= 2) {  // even lanes b^h, odd lanes t^h: one exponentiation's instructions for both
+        const T v = pow_b_or_t(f, pc, ip, h, (j & 1u) != 0);
+        const uint32_t lane = lane_id();
+        x = lane_value(v, lane & ~1u);
+        y = lane_value(v, lane | 1u);
+    } else {
+        pow_b_t(f, pc, ip, h, x, y);
+    }
     if (j == 0) work += 2 * bitlen64(h);
     const uint64_t k1 = f.key(f.one());
     if (f.key(x) == k1) {
@@ -1102,7 +1130,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 // two kinds of dependent chains run side by side.
                 {
                     const uint32_t nxs = min(R.nxp, kExtPairs);
-                    const uint32_t lsh = F::kSmallMul ? (nxs <= 64 ? 2 : nxs <= 128 ? 1 : 0) : 0;
+                    const uint32_t lsh = F::kSmallMul ? (nxs <= 64 ? 2 : nxs <= 128 ? 1 : 0) : (nxs <= 128 ? 1 : 0);
                     const uint32_t pt = nxs << lsh, wp = min(4u, (pt + 63) / 64), ew = 4 - wp;
                     if (c < wp * 64) {
                         for (uint32_t t = c; t < pt; t += wp * 64) {

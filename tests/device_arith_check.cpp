@@ -198,6 +198,11 @@ static void check_f64(int scale) {
                 CHECK(f.key(r1) == powmod(sa % p, e1, p) && f.key(r2) == powmod(p - sb % p, e1, p),
                       "fpow2_small p=%llu sa=%llu sb=%llu", (unsigned long long)p, (unsigned long long)sa,
                       (unsigned long long)sb);
+                CHECK(f.key(fpow_small(f, (double)sa, e1)) == powmod(sa % p, e1, p) &&
+                          f.key(fpow_small(f, -(double)sb, e2)) == powmod(p - sb % p, e2, p) &&
+                          f.key(fpow_small(f, -(double)sb, 0)) == 1 % p,
+                      "fpow_small p=%llu sa=%llu sb=%llu", (unsigned long long)p, (unsigned long long)sa,
+                      (unsigned long long)sb);
             }
         }
         if (p > 37) CHECK(fermat2(f, p) == (powmod(2, p - 1, p) == 1), "fermat2 f64 p=%llu", (unsigned long long)p);
