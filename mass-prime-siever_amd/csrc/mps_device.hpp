@@ -211,6 +211,8 @@ struct MontField {
     static constexpr bool kSmallMul = false;  // no mul_small (operands are in Montgomery form)
     static constexpr int kOrderSetupWaves = 6;  // order-mode setup: its LDS allows 6 workgroups per CU
     static constexpr uint32_t kGiantGroup = 1;  // k_bsgs: giant steps per branch on the lookups
+    // k_bsgs: no second giant loop for tables without overflow (c4 BSGS 25.0 -> 25.6 ms with one)
+    static constexpr bool kTightLookup = false;
     uint64_t p, ninv, one_;
 
     MPS_HD void init(uint64_t mod) {
@@ -280,6 +282,8 @@ struct F64Field {
     static constexpr bool kSmallMul = true;   // mul_small: a times a small integer
     static constexpr int kOrderSetupWaves = 7;  // order-mode setup: 72 registers, 7 waves per SIMD
     static constexpr uint32_t kGiantGroup = 2;  // k_bsgs: giant steps per branch on the lookups
+    // k_bsgs: a second giant loop for tables without overflow (c2 BSGS 1.99 -> 1.91 ms)
+    static constexpr bool kTightLookup = true;
     static constexpr uint32_t kHashMul = 0x9E3779B1u;
     double p, pinv;
     uint32_t plo;  // low word of p
@@ -474,6 +478,18 @@ MPS_HD bool bucket_needs_look(uint4 w, uint32_t f2) {
     const u16x2 m = __builtin_elementwise_min(__builtin_elementwise_min(x(w.x), x(w.y)),
                                               __builtin_elementwise_min(x(w.z), x(w.w)));
     return (umin(m.x, m.y) == 0u) | (w.w < 0xFFFF0000u);
+}
+
+// The lookup test of a table in which no bucket overflowed (every key sits in its own bucket, so
+// a full bucket ends the chain): a slot holds the raw fingerprint of f2 or that value with bit 0
+// flipped.  The second form covers a raw 0xFFFF, whose key is stored clamped to 0xFFFE; for any
+// other lookup it only adds rare false matches, which the exact walk drops.  The same instruction
+// count as bucket_needs_look less its test of slot 7.
+MPS_HD bool bucket_may_hold(uint4 w, uint32_t f2) {
+    auto x = [f2](uint32_t a) { return __builtin_bit_cast(u16x2, a ^ f2); };
+    const u16x2 m = __builtin_elementwise_min(__builtin_elementwise_min(x(w.x), x(w.y)),
+                                              __builtin_elementwise_min(x(w.z), x(w.w)));
+    return umin(m.x, m.y) <= 1u;
 }
 
 // Bit k set when slot k of the bucket holds fingerprint fp.

@@ -1490,6 +1490,7 @@ constexpr uint32_t kCycleFlag = 1u << 30;  // hit count of an entry whose baby v
 struct HitList {
     uint4 h[kMaxHits];
     uint32_t n;
+    uint32_t ovf;  // a baby step found its bucket full and went on along the chain
     unsigned long long best;
 };
 
@@ -1549,6 +1550,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
     }
     if (tid == 0) {
         hitl[0].n = 0;
+        hitl[0].ovf = 0;
         hitl[0].best = ~0ull;
     }
     __syncthreads();
@@ -1561,6 +1563,7 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
         // next entry (which reads it after this entry's end barrier)
         if (tid == 0) {
             hitl[par ^ 1u].n = 0;
+            hitl[par ^ 1u].ovf = 0;
             hitl[par ^ 1u].best = ~0ull;
         }
         BsgsEntry<F> be;
@@ -1626,7 +1629,10 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                 const uint32_t old = atomicAdd(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(Tb.cnt) + (bo >> 2)), 1u);
                 const bool placed = old < 8;
                 *reinterpret_cast<lds_u16*>((size_t)(placed ? bo + 2u * old : spare)) = (uint16_t)fp;
-                if (!placed) tab_insert(Tb, (bo + 16u) & Tb.omask, fp);
+                if (!placed) {
+                    hits.ovf = 1u;  // every writer stores the same value
+                    tab_insert(Tb, (bo + 16u) & Tb.omask, fp);
+                }
                 // a baby value with t's hash word (equal words: equal low words of the residues);
                 // one other than lane 0's first sends the prime to the host, which is exact, so a
                 // word match of two different values costs time only
@@ -1646,49 +1652,63 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
             const T Mt = thread_pow<F, WB>(f, E, E_MLO, E_MHI, F::from_bits(Ec[E_BM64]), E_BML, tid, mstep);
             const bool gfold = (be.meta >> 12) & 1u;  // the giant tables hold G * (B^m)^k
             const uint32_t rounds = be.ng / L, rem = be.ng - rounds * L;
-            for (uint32_t j = 0; j < be.s; ++j) {
-                T v = gfold ? Mt : f.mul(F::from_bits(Ec[E_G + j]), Mt);
-                auto probe = [&](uint32_t& hw, uint32_t& bo, T& vg) {
-                    hw = f.hash_word(v);
-                    bo = (hw >> 16) & Tb.omask;
-                    const uint4 w = read_bucket(Tb, bo);
-                    vg = v;
-                    v = f.mul(v, mstep);  // the next giant value while the LDS read is in flight
-                    return bucket_needs_look(w, fp_pair(hw));
-                };
-                auto look = [&](uint32_t hw, uint32_t bo, T vg, uint32_t g) {
-                    collect_hit(Tb, bo, min(hw & 0xFFFFu, 0xFFFEu), f.key(vg), g, j, hits);
-                };
-                uint32_t r = 0;
-                constexpr uint32_t G = F::kGiantGroup;
-                if constexpr (G > 1) {
-                    for (; r + G <= rounds; r += G) {  // G steps, then one branch for all of them
-                        uint32_t h[G], b[G];
-                        T vs[G];
-                        bool l[G], any = false;
+            // with F::kTightLookup, a table in which no baby step overflowed its bucket (most entries:
+            // at two keys per bucket about 6% of the tables of 512 keys have one) is looked up in the
+            // key's own bucket only, so a lane walks the chain only on a fingerprint match, not for
+            // every full bucket (a full bucket sends about one 2-step group of wavefront lookups in 8
+            // down the walk at that load)
+            auto giants = [&](auto chained) {
+                for (uint32_t j = 0; j < be.s; ++j) {
+                    T v = gfold ? Mt : f.mul(F::from_bits(Ec[E_G + j]), Mt);
+                    auto probe = [&](uint32_t& hw, uint32_t& bo, T& vg) {
+                        hw = f.hash_word(v);
+                        bo = (hw >> 16) & Tb.omask;
+                        const uint4 w = read_bucket(Tb, bo);
+                        vg = v;
+                        v = f.mul(v, mstep);  // the next giant value while the LDS read is in flight
+                        if constexpr (decltype(chained)::value)
+                            return bucket_needs_look(w, fp_pair(hw));
+                        else
+                            return bucket_may_hold(w, fp_pair(hw));
+                    };
+                    auto look = [&](uint32_t hw, uint32_t bo, T vg, uint32_t g) {
+                        collect_hit(Tb, bo, min(hw & 0xFFFFu, 0xFFFEu), f.key(vg), g, j, hits);
+                    };
+                    uint32_t r = 0;
+                    constexpr uint32_t G = F::kGiantGroup;
+                    if constexpr (G > 1) {
+                        for (; r + G <= rounds; r += G) {  // G steps, then one branch for all of them
+                            uint32_t h[G], b[G];
+                            T vs[G];
+                            bool l[G], any = false;
 #pragma unroll
-                        for (uint32_t k = 0; k < G; ++k) {
-                            l[k] = probe(h[k], b[k], vs[k]);
-                            any |= l[k];
-                        }
-                        if (any) {
+                            for (uint32_t k = 0; k < G; ++k) {
+                                l[k] = probe(h[k], b[k], vs[k]);
+                                any |= l[k];
+                            }
+                            if (any) {
 #pragma unroll
-                            for (uint32_t k = 0; k < G; ++k)
-                                if (l[k]) look(h[k], b[k], vs[k], tid + (r + k) * L);
+                                for (uint32_t k = 0; k < G; ++k)
+                                    if (l[k]) look(h[k], b[k], vs[k], tid + (r + k) * L);
+                            }
                         }
                     }
+                    for (; r < rounds; ++r) {
+                        uint32_t h, b;
+                        T vg;
+                        if (probe(h, b, vg)) look(h, b, vg, tid + r * L);
+                    }
+                    if (tid < rem) {
+                        uint32_t h, b;
+                        T vg;
+                        if (probe(h, b, vg)) look(h, b, vg, tid + rounds * L);
+                    }
                 }
-                for (; r < rounds; ++r) {
-                    uint32_t h, b;
-                    T vg;
-                    if (probe(h, b, vg)) look(h, b, vg, tid + r * L);
-                }
-                if (tid < rem) {
-                    uint32_t h, b;
-                    T vg;
-                    if (probe(h, b, vg)) look(h, b, vg, tid + rounds * L);
-                }
-            }
+            };
+            if (F::kTightLookup && hits.ovf == 0)  // the same in every lane (read after the babies' barrier)
+                giants(std::integral_constant<bool, !F::kTightLookup>{});
+            else
+                giants(std::true_type{});
             __syncthreads();
             // exact check of every match: the solution k = g*m - i, when in range, gives
             // n = nj + (k*stride + parity)*cm; the smallest over the workgroup by an LDS minimum

@@ -300,8 +300,8 @@ static void check_jacobi(int scale) {
     std::printf("jacobi: %ld symbols\n", n);
 }
 
-// The BSGS bucket test: bucket_needs_look (packed 16-bit minimum over w ^ fp_pair(hw)) against
-// slot-by-slot compares, on buckets with 0 to 8 filled slots, for lookups of stored, absent,
+// The BSGS bucket tests: bucket_needs_look and bucket_may_hold (packed 16-bit minimum over
+// w ^ fp_pair(hw)) against slot-by-slot compares, on buckets with 0 to 8 filled slots, for lookups of stored, absent,
 // 0xFFFE and raw 0xFFFF fingerprints under random high halves of the hash word; and the property
 // the kernel relies on: a lookup whose clamped fingerprint is stored in the bucket, or whose bucket
 // is full, always looks (so no stored key is missed, and a raw 0xFFFF always looks).
@@ -336,11 +336,18 @@ static void check_bucket(int scale) {
         const bool full = sl[7] != 0xFFFF;
         const bool look = bucket_needs_look(w, fp_pair(hw));
         CHECK(look == (any || full), "needs_look fill %d lo %04x: %d", fill, lo, (int)look);
+        bool near = false;  // a slot holds lo or lo ^ 1
+        for (int k = 0; k < 8; ++k) near |= (sl[k] ^ lo) <= 1u;
+        const bool may = bucket_may_hold(w, fp_pair(hw));
+        CHECK(may == near, "may_hold fill %d lo %04x: %d", fill, lo, (int)may);
         CHECK(match_mask(w, lo) == mm, "match_mask fill %d lo %04x", fill, lo);
         const uint32_t clamped = umin(lo, 0xFFFEu);
         bool stored = false;
         for (int k = 0; k < fill; ++k) stored |= sl[k] == clamped;
         if (stored || full || lo == 0xFFFF) CHECK(look, "missed lookup fill %d lo %04x", fill, lo);
+        // a table without overflow: a stored key is always found in its own bucket, also for a raw
+        // 0xFFFF (stored as 0xFFFE)
+        if (stored) CHECK(may, "missed lookup without overflow fill %d lo %04x", fill, lo);
     }
     std::printf("bucket: %ld lookups on 0-8 filled slots\n", n);
 }

@@ -10,8 +10,9 @@ checks it against 128-bit integer arithmetic and Euler's criterion:
     the order-mode moduli and up to 2^52, exact_div_odd, isqrt_ceil64;
   * jacobi32 / jacobi64 / jacobi_small / jacobi_signed against Euler's criterion (prime moduli up to
     2^62) and multiplicativity in the modulus (composite moduli);
-  * the BSGS bucket test (bucket_needs_look, match_mask, fp_pair) against slot-by-slot compares on
-    buckets of 0 to 8 fingerprints, raw 0xFFFF lookups included, and the no-miss property.
+  * the BSGS bucket tests (bucket_needs_look, bucket_may_hold, match_mask, fp_pair) against
+    slot-by-slot compares on buckets of 0 to 8 fingerprints, raw 0xFFFF lookups included, and the
+    no-miss properties.
 The GPU runs the same functions: tests/test_gpu_planted.py checks their results end to end."""
 import os
 import shutil
