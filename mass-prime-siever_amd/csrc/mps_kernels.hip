@@ -1627,17 +1627,18 @@ __global__ __launch_bounds__(64 * WB) void k_bsgs(const PassDesc* __restrict__ d
                 // full (no branch around the write); a full bucket (rare) sends the key on along the
                 // chain
                 const uint32_t old = atomicAdd(reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(Tb.cnt) + (bo >> 2)), 1u);
+                // a baby value with t's hash word (equal words: equal low words of the residues);
+                // one other than lane 0's first sends the prime to the host, which is exact, so a
+                // word match of two different values costs time only
+                nt += w == wt;
+                v = f.mul(v, bstep);  // the next baby value while the atomic is in flight
+                asm volatile("" : "+v"(v));  // (kept here: the compiler would sink it past the store)
                 const bool placed = old < 8;
                 *reinterpret_cast<lds_u16*>((size_t)(placed ? bo + 2u * old : spare)) = (uint16_t)fp;
                 if (!placed) {
                     hits.ovf = 1u;  // every writer stores the same value
                     tab_insert(Tb, (bo + 16u) & Tb.omask, fp);
                 }
-                // a baby value with t's hash word (equal words: equal low words of the residues);
-                // one other than lane 0's first sends the prime to the host, which is exact, so a
-                // word match of two different values costs time only
-                nt += w == wt;
-                v = f.mul(v, bstep);
             };
             for (uint32_t r = 0; r < rounds; ++r) baby();
             if (tid < rem) baby();
