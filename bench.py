@@ -107,7 +107,7 @@ def cpu_baseline(threads, slice_ints):
 def cpu_baseline_configs(threads, slice_ints=1 << 26):
     """The oracle's CPU path on c1 (whole) and the first slice_ints integers of c3 and c4, as
     primes/s; c4's sieving primes up to sqrt(P1) ~ 2e9 (sieve_small_primes, a one-off per item)
-    are timed on their own first."""
+    are timed on their own first.  c5 is the serving mix (cpu_baseline_mix)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     out = {}
@@ -123,7 +123,31 @@ def cpu_baseline_configs(threads, slice_ints=1 << 26):
                      "sample": f"first {hi - P0} integers ({primes} primes, {nrel} relations) in {dt:.2f} s"}
         if setup_s is not None:
             out[name]["sieving_primes_setup_s"] = setup_s
+    out["c5"] = cpu_baseline_mix(O, threads)
     return out
+
+
+def cpu_baseline_mix(O, threads, slice_ints=1 << 20):
+    """The oracle's CPU path on the serving mix (seed 0): the first slice_ints integers of each of
+    the 256 items' P ranges, the items spread over the threads (the oracle's calls release the GIL)."""
+    from concurrent.futures import ThreadPoolExecutor
+    items = load_mix(256)
+
+    def one(it):
+        lo = it["P0"]
+        hi = min(it["P1"], lo + slice_ints)
+        item = O.make_item(it["b"], it["c"], lo, hi, it["N0"], it["N1"], it["mod"], it["res"])
+        rels, primes = O.process_range(item, lo, hi, 1 << 20, 1, 1 << 18)
+        return primes, len(rels), hi - lo
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(one, items))
+    dt = time.perf_counter() - t0
+    primes, nrel, ints = (sum(r[k] for r in res) for k in range(3))
+    return {"value": primes / dt, "unit": "primes/s", "cores": threads, "kind": "port",
+            "sample": f"first {slice_ints} integers of each of the {len(items)} mix items ({ints} integers, "
+                      f"{primes} primes, {nrel} relations) in {dt:.2f} s"}
 
 
 def load_mix(n):
