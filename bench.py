@@ -31,7 +31,7 @@ PEAK_FP64_TFLOPS = 78.6          # MI355X published FP64 vector/matrix peak
 MUL_PER_MMUL = 11
 # HBM bytes per tuned k_bsgs launch on this workload: FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000 (scale factors
 # from a 1 GiB streaming copy), from profiles/k_bsgs_c2_hbm_traffic.txt
-K_BSGS_TRAFFIC_BYTES = 2.900e8
+K_BSGS_TRAFFIC_BYTES = 2.896e8
 TUNING_FILE = os.path.join(ROOT, "mass-prime-siever_amd", "tuning_mi355x.json")
 # the autotuner's item set: other items of the same tuning keys as the timed workloads
 TUNE_WORKTODO = os.path.join(ROOT, "mass-prime-siever_amd", "tune_worktodo.txt")
