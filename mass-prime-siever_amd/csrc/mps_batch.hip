@@ -429,7 +429,10 @@ mps_batch* mps_batch_create(mps_gpu* g, const mps_batch_opts* opts) {
     if (b->o.slots > 16) b->o.slots = 16;
     if (b->o.pass_odds == 0) b->o.pass_odds = 1ull << 25;
     if (b->o.pass_odds < 4096 || b->o.pass_odds > (1ull << 30)) {
-        g->err = "batch pass_odds outside [4096, 2^30]";
+        {
+            std::lock_guard<std::mutex> l(g->st_mu);
+            g->err = "batch pass_odds outside [4096, 2^30]";
+        }
         delete b;
         return nullptr;
     }
@@ -450,7 +453,10 @@ mps_batch* mps_batch_create(mps_gpu* g, const mps_batch_opts* opts) {
     b->slots.resize(b->o.slots);
     for (size_t i = 0; ok && i < b->slots.size(); ++i) ok = alloc_slot(b, b->slots[i]);
     if (!ok) {
-        g->err = b->err.empty() ? std::string("batch engine: device allocation failed") : b->err;
+        {
+            std::lock_guard<std::mutex> l(g->st_mu);
+            g->err = b->err.empty() ? std::string("batch engine: device allocation failed") : b->err;
+        }
         for (auto& x : b->slots) free_slot(x);
         if (b->up_stream) hipStreamDestroy(b->up_stream);
         hipFree(b->d_tables);

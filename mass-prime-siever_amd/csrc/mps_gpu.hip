@@ -391,8 +391,13 @@ FieldKind pass_field(const mps_gpu* g, uint64_t hi) {
 
 namespace {
 
+void set_err(mps_gpu* g, const std::string& what) {
+    std::lock_guard<std::mutex> l(g->st_mu);
+    g->err = what;
+}
+
 bool hip_fail(mps_gpu* g, hipError_t e, const char* what) {
-    g->err = std::string(what) + ": " + hipGetErrorString(e);
+    set_err(g, std::string(what) + ": " + hipGetErrorString(e));
     g->broken = true;  // sticky, like sv_fail (Gpu.hpp:982-987)
     return false;
 }
@@ -403,35 +408,58 @@ bool hip_fail(mps_gpu* g, hipError_t e, const char* what) {
         if (e_ != hipSuccess) return hip_fail(g, e_, what); \
     } while (0)
 
-void free_buffers(mps_gpu* g) {
-    hipFree(g->pb.surv);
-    hipFree(g->pb.cand);
-    hipFree(g->pb.ent);
-    hipFree(g->pb.out);
-    hipHostFree(g->h_out);
-    g->pb.surv = nullptr;
-    g->pb.cand = nullptr;
-    g->pb.ent = nullptr;
-    g->pb.out = nullptr;
-    g->h_out = nullptr;
-    g->buf_odds = 0;
+// The calling thread's pass lane, locked in lk: a free lane, else (all busy) the next in turn once
+// it frees.
+PassLane& take_lane(mps_gpu* g, std::unique_lock<std::mutex>& lk) {
+    for (int i = 0; i < kPassLanes; ++i) {
+        std::unique_lock<std::mutex> t(g->lane[i].mu, std::try_to_lock);
+        if (t.owns_lock()) {
+            lk = std::move(t);
+            return g->lane[i];
+        }
+    }
+    PassLane& L = g->lane[g->next_lane.fetch_add(1) % kPassLanes];
+    lk = std::unique_lock<std::mutex>(L.mu);
+    return L;
 }
 
-bool ensure_buffers(mps_gpu* g, uint64_t odds) {
-    if (g->buf_odds >= odds) return true;
-    free_buffers(g);
+// Every lane, taken in index order: no pass runs while the holder changes settings or times shapes.
+struct AllLanes {
+    std::unique_lock<std::mutex> l[kPassLanes];
+    explicit AllLanes(mps_gpu* g) {
+        for (int i = 0; i < kPassLanes; ++i) l[i] = std::unique_lock<std::mutex>(g->lane[i].mu);
+    }
+};
+
+void free_buffers(PassLane& L) {
+    hipFree(L.pb.surv);
+    hipFree(L.pb.cand);
+    hipFree(L.pb.ent);
+    hipFree(L.pb.out);
+    hipHostFree(L.h_out);
+    L.pb.surv = nullptr;
+    L.pb.cand = nullptr;
+    L.pb.ent = nullptr;
+    L.pb.out = nullptr;
+    L.h_out = nullptr;
+    L.buf_odds = 0;
+}
+
+bool ensure_buffers(mps_gpu* g, PassLane& L, uint64_t odds) {
+    if (L.buf_odds >= odds) return true;
+    free_buffers(L);
     const uint64_t ent_cap = std::max<uint64_t>(odds / 8, 4096);
     const uint64_t out_cap = record_cap(odds);  // records of 2 words
-    HIPCHK(hipMalloc(&g->pb.surv, odds * sizeof(uint32_t)), "surv alloc");
-    HIPCHK(hipMalloc(&g->pb.cand, odds * sizeof(uint64_t)), "cand alloc");
-    HIPCHK(hipMalloc(&g->pb.ent, ent_cap * kEntWords * sizeof(uint64_t)), "entry alloc");
-    HIPCHK(hipMalloc(&g->pb.out, 2 * out_cap * sizeof(uint64_t)), "out alloc");
-    HIPCHK(hipHostMalloc((void**)&g->h_out, 2 * out_cap * sizeof(uint64_t), hipHostMallocDefault), "pinned out");
-    g->pb.surv_cap = (uint32_t)odds;
-    g->pb.cand_cap = (uint32_t)odds;
-    g->pb.ent_cap = (uint32_t)ent_cap;
-    g->pb.out_cap = (uint32_t)out_cap;
-    g->buf_odds = odds;
+    HIPCHK(hipMalloc(&L.pb.surv, odds * sizeof(uint32_t)), "surv alloc");
+    HIPCHK(hipMalloc(&L.pb.cand, odds * sizeof(uint64_t)), "cand alloc");
+    HIPCHK(hipMalloc(&L.pb.ent, ent_cap * kEntWords * sizeof(uint64_t)), "entry alloc");
+    HIPCHK(hipMalloc(&L.pb.out, 2 * out_cap * sizeof(uint64_t)), "out alloc");
+    HIPCHK(hipHostMalloc((void**)&L.h_out, 2 * out_cap * sizeof(uint64_t), hipHostMallocDefault), "pinned out");
+    L.pb.surv_cap = (uint32_t)odds;
+    L.pb.cand_cap = (uint32_t)odds;
+    L.pb.ent_cap = (uint32_t)ent_cap;
+    L.pb.out_cap = (uint32_t)out_cap;
+    L.buf_odds = odds;
     return true;
 }
 
@@ -441,9 +469,9 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-bool read_counts(mps_gpu* g, const PassBuffers& pb) {
-    HIPCHK(hipMemcpyAsync(g->h_cnt, pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, g->stream), "count read");
-    HIPCHK(hipStreamSynchronize(g->stream), "stream sync");
+bool read_counts(mps_gpu* g, PassLane& L, const PassBuffers& pb) {
+    HIPCHK(hipMemcpyAsync(L.h_cnt, pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, L.stream), "count read");
+    HIPCHK(hipStreamSynchronize(L.stream), "stream sync");
     return true;
 }
 
@@ -452,23 +480,23 @@ bool read_counts(mps_gpu* g, const PassBuffers& pb) {
 constexpr uint32_t kRecPrefix = 1024;
 
 // Counts and the first `pre` records of the pass just enqueued into h_rec, one wait.
-bool read_counts_and_records(mps_gpu* g, const PassBuffers& pb, uint64_t* h_rec, size_t pre) {
-    HIPCHK(hipMemcpyAsync(g->h_cnt, pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, g->stream), "count read");
+bool read_counts_and_records(mps_gpu* g, PassLane& L, const PassBuffers& pb, uint64_t* h_rec, size_t pre) {
+    HIPCHK(hipMemcpyAsync(L.h_cnt, pb.cnt, sizeof(Counters), hipMemcpyDeviceToHost, L.stream), "count read");
     if (pre)
-        HIPCHK(hipMemcpyAsync(h_rec, pb.out, 2 * pre * sizeof(uint64_t), hipMemcpyDeviceToHost, g->stream),
+        HIPCHK(hipMemcpyAsync(h_rec, pb.out, 2 * pre * sizeof(uint64_t), hipMemcpyDeviceToHost, L.stream),
                "record read");
-    HIPCHK(hipStreamSynchronize(g->stream), "stream sync");
+    HIPCHK(hipStreamSynchronize(L.stream), "stream sync");
     return true;
 }
 
-// The pass's n_out records in h_out: those past the prefix read_counts_and_records brought.
-bool read_records(mps_gpu* g, uint32_t n_out) {
-    const size_t pre = std::min<size_t>(kRecPrefix, g->pb.out_cap);
+// The pass's n_out records in L.h_out: those past the prefix read_counts_and_records brought.
+bool read_records(mps_gpu* g, PassLane& L, uint32_t n_out) {
+    const size_t pre = std::min<size_t>(kRecPrefix, L.pb.out_cap);
     if (n_out <= pre) return true;
-    HIPCHK(hipMemcpyAsync(g->h_out + 2 * pre, g->pb.out + 2 * pre, 2 * (n_out - pre) * sizeof(uint64_t),
-                          hipMemcpyDeviceToHost, g->stream),
+    HIPCHK(hipMemcpyAsync(L.h_out + 2 * pre, L.pb.out + 2 * pre, 2 * (n_out - pre) * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, L.stream),
            "record read");
-    HIPCHK(hipStreamSynchronize(g->stream), "stream sync");
+    HIPCHK(hipStreamSynchronize(L.stream), "stream sync");
     return true;
 }
 
@@ -480,27 +508,39 @@ uint32_t blocks_for(const mps_gpu* g, uint64_t n, uint32_t per_block) {
     return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(b, cap));
 }
 
-// One pass through the stage chain on the buffers pb.  stop_stage: 0 after the sieve, 1 after the
-// filter, 2 after setup, 3 = full.  Leaves the final counts in g->h_cnt and, after a full pass,
-// the first rec_prefix records in h_rec.
-bool run_pass(mps_gpu* g, const PassBuffers& pb, uint64_t* h_rec, size_t rec_prefix, uint64_t plo, uint64_t pc,
-              const ItemParams& ip, FieldKind fk, const TableGeom& tg, int stop_stage) {
-    hipStream_t s = g->stream;
-    g->h_desc->lo = plo;
-    g->h_desc->nodd = pc;
-    g->h_desc->ip = ip;
-    g->h_desc->tg = tg;
-    HIPCHK(hipMemcpyAsync(g->d_csz, g->h_csz, kCszEntries * sizeof(uint2), hipMemcpyHostToDevice, s), "size table");
-    HIPCHK(hipMemcpyAsync(g->d_desc, g->h_desc, sizeof(PassDesc), hipMemcpyHostToDevice, s), "pass descriptor");
-    const PassDesc* d = g->d_desc;
+// Stage times of the pass just read back on lane L, up to stage `last` (0 sieve .. 3 BSGS).
+void add_stage_times(mps_gpu* g, PassLane& L, int last) {
+    const float t[4] = {elapsed(L.ev[0], L.ev[1]), last >= 1 ? elapsed(L.ev[1], L.ev[2]) : 0.f,
+                        last >= 2 ? elapsed(L.ev[2], L.ev[3]) : 0.f, last >= 3 ? elapsed(L.ev[3], L.ev[4]) : 0.f};
+    std::lock_guard<std::mutex> l(g->st_mu);
+    g->st.ms_sieve += t[0];
+    g->st.ms_filter += t[1];
+    g->st.ms_setup += t[2];
+    g->st.ms_bsgs += t[3];
+    if (last >= 3) g->st.n_bsgs_launches++;
+}
+
+// One pass through the stage chain on the buffers pb, on lane L's stream, descriptor and events.
+// stop_stage: 0 after the sieve, 1 after the filter, 2 after setup, 3 = full.  Leaves the final
+// counts in L.h_cnt and, after a full pass, the first rec_prefix records in h_rec.
+bool run_pass(mps_gpu* g, PassLane& L, const PassBuffers& pb, uint64_t* h_rec, size_t rec_prefix, uint64_t plo,
+              uint64_t pc, const ItemParams& ip, FieldKind fk, const TableGeom& tg, int stop_stage) {
+    hipStream_t s = L.stream;
+    L.h_desc->lo = plo;
+    L.h_desc->nodd = pc;
+    L.h_desc->ip = ip;
+    L.h_desc->tg = tg;
+    HIPCHK(hipMemcpyAsync(L.d_csz, L.h_csz, kCszEntries * sizeof(uint2), hipMemcpyHostToDevice, s), "size table");
+    HIPCHK(hipMemcpyAsync(L.d_desc, L.h_desc, sizeof(PassDesc), hipMemcpyHostToDevice, s), "pass descriptor");
+    const PassDesc* d = L.d_desc;
     launch_reset(s, pb.cnt, pb.vals == nullptr ? 0u : (uint32_t)pc);
-    HIPCHK(hipEventRecord(g->ev[0], s), "event");
+    HIPCHK(hipEventRecord(L.ev[0], s), "event");
     if (pb.vals == nullptr) launch_sieve(s, d, pc, g->tab, pb, 0);
     HIPCHK(hipGetLastError(), "sieve launch");
-    HIPCHK(hipEventRecord(g->ev[1], s), "event");
+    HIPCHK(hipEventRecord(L.ev[1], s), "event");
     if (stop_stage == 0) {
-        if (!read_counts(g, pb)) return false;
-        g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
+        if (!read_counts(g, L, pb)) return false;
+        add_stage_times(g, L, 0);
         return true;
     }
     // The filter and setup kernels read their element counts on the device, so their grids are
@@ -508,11 +548,10 @@ bool run_pass(mps_gpu* g, const PassBuffers& pb, uint64_t* h_rec, size_t rec_pre
     // count.
     launch_filter(s, d, pb, blocks_for(g, pc, 256));
     HIPCHK(hipGetLastError(), "filter launch");
-    HIPCHK(hipEventRecord(g->ev[2], s), "event");
+    HIPCHK(hipEventRecord(L.ev[2], s), "event");
     if (stop_stage == 1) {
-        if (!read_counts(g, pb)) return false;
-        g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
-        g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
+        if (!read_counts(g, L, pb)) return false;
+        add_stage_times(g, L, 1);
         return true;
     }
     // class mode sorts tiles of kSetupTile candidates, a workgroup a tile at a time; order mode runs
@@ -520,53 +559,58 @@ bool run_pass(mps_gpu* g, const PassBuffers& pb, uint64_t* h_rec, size_t rec_pre
     const uint32_t setup_grid = blocks_for(g, pc, ip.qmode == kModeClass ? kSetupTile : 256);
     launch_setup(s, fk, setup_variant(ip, fk), d, pb, setup_grid);
     HIPCHK(hipGetLastError(), "setup launch");
-    HIPCHK(hipEventRecord(g->ev[3], s), "event");
+    HIPCHK(hipEventRecord(L.ev[3], s), "event");
     if (stop_stage == 2) {
-        if (!read_counts(g, pb)) return false;
-        g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
-        g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
-        g->st.ms_setup += elapsed(g->ev[2], g->ev[3]);
+        if (!read_counts(g, L, pb)) return false;
+        add_stage_times(g, L, 2);
         return true;
     }
     // BSGS over the entries setup wrote, on a grid of several times the resident workgroups that
     // reads the entry count on the device (no read-back before it)
     launch_bsgs(s, fk, tg, d, pb, 0, bsgs_persistent_grid(fk, tg, g->num_cu));
     HIPCHK(hipGetLastError(), "bsgs launch");
-    HIPCHK(hipEventRecord(g->ev[4], s), "event");
+    HIPCHK(hipEventRecord(L.ev[4], s), "event");
     // the records' Miller-Rabin test reads the record count on the device, so it is enqueued now
     // and the counts and records come back under one wait
     launch_prime_records(s, pb);
     HIPCHK(hipGetLastError(), "record test launch");
-    if (!read_counts_and_records(g, pb, h_rec, rec_prefix)) return false;
-    g->st.ms_sieve += elapsed(g->ev[0], g->ev[1]);
-    g->st.ms_filter += elapsed(g->ev[1], g->ev[2]);
-    g->st.ms_setup += elapsed(g->ev[2], g->ev[3]);
-    g->st.ms_bsgs += elapsed(g->ev[3], g->ev[4]);
-    g->st.n_bsgs_launches++;
+    if (!read_counts_and_records(g, L, pb, h_rec, rec_prefix)) return false;
+    add_stage_times(g, L, 3);
     return true;
 }
 
-void accumulate(mps_gpu* g, uint64_t pc) {
-    const Counters& c = *g->h_cnt;
+void accumulate(mps_gpu* g, PassLane& L, const PassBuffers& pb, uint64_t pc) {
+    const Counters& c = *L.h_cnt;
+    std::lock_guard<std::mutex> l(g->st_mu);
     g->st.passes++;
     g->st.odds += pc;
     g->st.survivors += c.n_surv;
     g->st.candidates += c.n_cand;
     g->st.entries += c.n_ent;
     g->st.flagged += c.n_out;
-    g->st.flag2 += c.n_flag2 + (c.n_ent > g->pb.ent_cap ? c.n_ent - g->pb.ent_cap : 0);
+    g->st.flag2 += c.n_flag2 + (c.n_ent > pb.ent_cap ? c.n_ent - pb.ent_cap : 0);
     g->st.fermat_fail += c.n_fermat_fail;
     g->st.w_fermat += c.w_fermat;
     g->st.w_setup += c.w_setup;
     g->st.w_bsgs += c.w_bsgs;
 }
 
-// Item constants plus the class-size table, staged in pinned memory for run_pass to upload.
-bool prepare_item(mps_gpu* g, const mps_item* item, uint64_t P0, bool classes_on, Plan& pl) {
-    if (!plan_item(g, item, P0, classes_on, pl, g->err)) return false;
-    std::memcpy(g->h_csz, pl.csz.data(), pl.csz.size() * sizeof(uint2));
-    pl.ip.csz = g->d_csz;
+// Item constants plus the class-size table, staged in lane L's pinned memory for run_pass to upload.
+bool prepare_item(mps_gpu* g, PassLane& L, const mps_item* item, uint64_t P0, bool classes_on, Plan& pl) {
+    std::string why;
+    if (!plan_item(g, item, P0, classes_on, pl, why)) {
+        set_err(g, why);
+        return false;
+    }
+    std::memcpy(L.h_csz, pl.csz.data(), pl.csz.size() * sizeof(uint2));
+    pl.ip.csz = L.d_csz;
     return true;
+}
+
+void record_shape(mps_gpu* g, FieldKind fk, const TableGeom& tg) {
+    std::lock_guard<std::mutex> l(g->st_mu);
+    g->st.field = (int)fk;
+    g->st.tbits = (int)tg.nbbits;
 }
 
 
@@ -595,8 +639,17 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
     g->device = device_index;
     g->name = prop.name;
     g->num_cu = prop.multiProcessorCount;
-    bool ok = hipSetDevice(device_index) == hipSuccess && hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess;
-    for (int i = 0; ok && i < 6; ++i) ok = hipEventCreate(&g->ev[i]) == hipSuccess;
+    bool ok = hipSetDevice(device_index) == hipSuccess;
+    for (PassLane& L : g->lane) {  // streams at the default priority
+        ok = ok && hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) == hipSuccess &&
+             hipMalloc(&L.pb.cnt, sizeof(Counters)) == hipSuccess &&
+             hipMalloc(&L.d_csz, kCszEntries * sizeof(uint2)) == hipSuccess &&
+             hipMalloc(&L.d_desc, sizeof(PassDesc)) == hipSuccess &&
+             hipHostMalloc((void**)&L.h_desc, sizeof(PassDesc), hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void**)&L.h_csz, kCszEntries * sizeof(uint2), hipHostMallocDefault) == hipSuccess &&
+             hipHostMalloc((void**)&L.h_cnt, sizeof(Counters), hipHostMallocDefault) == hipSuccess;
+        for (int i = 0; ok && i < 6; ++i) ok = hipEventCreate(&L.ev[i]) == hipSuccess;
+    }
     // odd sieving primes 3 .. 65521 with 2^32 mod q and 1/q
     std::vector<SievePrime> sp;
     {
@@ -617,12 +670,6 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
     ok = ok && g->tab.n_comb == kCombPrimes &&
          hipMalloc(&g->d_sp, sp.size() * sizeof(SievePrime)) == hipSuccess &&
          hipMemcpy(g->d_sp, sp.data(), sp.size() * sizeof(SievePrime), hipMemcpyHostToDevice) == hipSuccess &&
-         hipMalloc(&g->pb.cnt, sizeof(Counters)) == hipSuccess &&
-         hipMalloc(&g->d_csz, kCszEntries * sizeof(uint2)) == hipSuccess &&
-         hipMalloc(&g->d_desc, sizeof(PassDesc)) == hipSuccess &&
-         hipHostMalloc((void**)&g->h_desc, sizeof(PassDesc), hipHostMallocDefault) == hipSuccess &&
-         hipHostMalloc((void**)&g->h_csz, kCszEntries * sizeof(uint2), hipHostMallocDefault) == hipSuccess &&
-         hipHostMalloc((void**)&g->h_cnt, sizeof(Counters), hipHostMallocDefault) == hipSuccess &&
          hipHostMalloc((void**)&g->h_stage, kStageRecords * 2 * sizeof(uint64_t), hipHostMallocDefault) == hipSuccess;
     if (ok) g->scratch = scratch_address(device_index);  // with the device current (hipSetDevice above)
     g->tab.sp = g->d_sp;
@@ -648,13 +695,13 @@ int mps_gpu_ok(const mps_gpu* g) { return g != nullptr && g->ok ? 1 : 0; }
 
 void mps_gpu_set_classes(mps_gpu* g, int on) {
     if (g == nullptr) return;
-    std::lock_guard<std::mutex> lock(g->mu);
+    AllLanes all(g);
     g->classes_on = on != 0;
 }
 
 int mps_gpu_set_class_policy(mps_gpu* g, int policy) {
     if (g == nullptr || policy < 0 || policy > 2) return MPS_ERROR;
-    std::lock_guard<std::mutex> lock(g->mu);
+    AllLanes all(g);
     g->class_policy = policy;
     return MPS_OK;
 }
@@ -666,7 +713,7 @@ int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap, 
     if (waves && waves != 1 && waves != 2 && waves != 4) return MPS_ERROR;
     if (mcap && (mcap < 256 || mcap > 4096)) return MPS_ERROR;
     if (pass_odds && (pass_odds < 4096 || pass_odds > (1ull << 31))) return MPS_ERROR;
-    std::lock_guard<std::mutex> lock(g->mu);
+    AllLanes all(g);
     g->field_mode = field;
     if (waves) g->waves = waves;
     if (mcap) g->mcap = mcap;
@@ -679,48 +726,47 @@ int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap, 
 namespace {
 
 // Device records (number, M) of [lo|1, hi): the plain per-chunk pipeline, pass by pass.
-int sieve_records(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, std::vector<uint64_t>& rec) {
+int sieve_records(mps_gpu* g, PassLane& L, uint64_t lo, uint64_t hi, const mps_item* item, std::vector<uint64_t>& rec) {
     rec.clear();
     if (!g->ok || g->broken) return MPS_ERROR;
     if (hi >= (1ull << 62) || lo <= 3) {
-        g->err = "range outside (3, 2^62)";
+        set_err(g, "range outside (3, 2^62)");
         return MPS_DECLINED;
     }
     lo |= 1;
     if (lo >= hi) return MPS_OK;
     Plan pl;
-    if (!prepare_item(g, item, lo, g->classes_on, pl)) return MPS_DECLINED;
+    if (!prepare_item(g, L, item, lo, g->classes_on, pl)) return MPS_DECLINED;
     const ItemParams& ip = pl.ip;
     const FieldKind fk = pass_field(g, hi);
     const TableGeom tg = pl.tg;
     const uint64_t count = (hi - lo + 1) / 2;
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
-    if (!ensure_buffers(g, std::min(count, g->pass_odds))) return MPS_ERROR;
-    g->pb.vals = nullptr;
-    g->pb.need_prime = 1;
-    g->pb.out_index = 0;
+    if (!ensure_buffers(g, L, std::min(count, g->pass_odds))) return MPS_ERROR;
+    L.pb.vals = nullptr;
+    L.pb.need_prime = 1;
+    L.pb.out_index = 0;
     for (uint64_t pb0 = 0; pb0 < count; pb0 += g->pass_odds) {
         const uint64_t pc = std::min(g->pass_odds, count - pb0);
         const uint64_t plo = lo + 2 * pb0;
-        if (!run_pass(g, g->pb, g->h_out, std::min<size_t>(kRecPrefix, g->pb.out_cap), plo, pc, ip, fk, tg, 3))
+        if (!run_pass(g, L, L.pb, L.h_out, std::min<size_t>(kRecPrefix, L.pb.out_cap), plo, pc, ip, fk, tg, 3))
             return MPS_ERROR;
-        accumulate(g, pc);
-        const uint32_t n_out = g->h_cnt->n_out;
-        if (n_out > g->pb.out_cap) {
-            g->err = "device records exceed the output buffer";
+        accumulate(g, L, L.pb, pc);
+        const uint32_t n_out = L.h_cnt->n_out;
+        if (n_out > L.pb.out_cap) {
+            set_err(g, "device records exceed the output buffer");
             return MPS_DECLINED;
         }
         if (n_out) {
-            if (!read_records(g, n_out)) return MPS_ERROR;
+            if (!read_records(g, L, n_out)) return MPS_ERROR;
             for (uint32_t i = 0; i < n_out; ++i) {
-                if (g->h_out[2 * i] == 0) continue;  // composite (k_prime_records)
-                rec.push_back(g->h_out[2 * i]);
-                rec.push_back(g->h_out[2 * i + 1]);
+                if (L.h_out[2 * i] == 0) continue;  // composite (k_prime_records)
+                rec.push_back(L.h_out[2 * i]);
+                rec.push_back(L.h_out[2 * i + 1]);
             }
         }
     }
-    g->st.field = (int)fk;
-    g->st.tbits = (int)tg.nbbits;
+    record_shape(g, fk, tg);
     return MPS_OK;
 }
 
@@ -732,9 +778,10 @@ int mps_gpu_sieve_filter(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* i
                          size_t out_cap, size_t* out_n) {
     if (out_n) *out_n = 0;
     if (g == nullptr || item == nullptr || out_n == nullptr) return MPS_ERROR;
-    std::lock_guard<std::mutex> lock(g->mu);
+    std::unique_lock<std::mutex> lock;
+    PassLane& L = take_lane(g, lock);
     std::vector<uint64_t> rec;
-    const int st = sieve_records(g, lo, hi, item, rec);
+    const int st = sieve_records(g, L, lo, hi, item, rec);
     if (st != MPS_OK) return st;
     const size_t n = rec.size() / 2;
     for (size_t i = 0; i < n && i < out_cap && out; ++i) out[i] = rec[2 * i];
@@ -746,9 +793,10 @@ int mps_gpu_sieve_relations(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item
                             size_t out_cap, size_t* out_n) {
     if (out_n) *out_n = 0;
     if (g == nullptr || item == nullptr || out_n == nullptr) return MPS_ERROR;
-    std::lock_guard<std::mutex> lock(g->mu);
+    std::unique_lock<std::mutex> lock;
+    PassLane& L = take_lane(g, lock);
     std::vector<uint64_t> rec;
-    const int st = sieve_records(g, lo, hi, item, rec);
+    const int st = sieve_records(g, L, lo, hi, item, rec);
     if (st != MPS_OK) return st;
     const size_t n = rec.size() / 2;
     for (size_t i = 0; i < n && i < out_cap && out; ++i) {
@@ -763,33 +811,35 @@ int mps_gpu_sieve_relations_noalloc(mps_gpu* g, uint64_t lo, uint64_t hi, const 
                                     mps_dev_relation* out, size_t out_cap, size_t* out_n) {
     if (out_n) *out_n = 0;
     if (g == nullptr || item == nullptr || out_n == nullptr) return MPS_ERROR;
-    std::lock_guard<std::mutex> lock(g->mu);
+    std::lock_guard<std::mutex> one(g->noalloc_mu);  // the scratch array serves one call at a time
+    std::unique_lock<std::mutex> lock;
+    PassLane& L = take_lane(g, lock);  // its stream, descriptor and staging; the buffers are scratch
     if (!g->ok || g->broken) return MPS_ERROR;
     if (g->scratch == nullptr) {
-        g->err = "no-allocation path: the scratch array's device address is unknown";
+        set_err(g, "no-allocation path: the scratch array's device address is unknown");
         return MPS_ERROR;
     }
     if (hi > lo && hi - lo > MPS_NOALLOC_MAX_INTS) {
-        g->err = "no-allocation path: a range of more than 2^20 integers (MPS_NOALLOC_MAX_INTS)";
+        set_err(g, "no-allocation path: a range of more than 2^20 integers (MPS_NOALLOC_MAX_INTS)");
         return MPS_ERROR;
     }
     if (hi >= (1ull << 62) || lo <= 3) {
-        g->err = "range outside (3, 2^62)";
+        set_err(g, "range outside (3, 2^62)");
         return MPS_DECLINED;
     }
     lo |= 1;
     if (lo >= hi) return MPS_OK;
     Plan pl;
-    if (!prepare_item(g, item, lo, g->classes_on, pl)) return MPS_DECLINED;
+    if (!prepare_item(g, L, item, lo, g->classes_on, pl)) return MPS_DECLINED;
     const FieldKind fk = pass_field(g, hi);
     const uint64_t count = (hi - lo + 1) / 2;  // at most kScratchOdds
     const PassBuffers pb = scratch_buffers(g->scratch);
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
-    if (!run_pass(g, pb, g->h_stage, kStageRecords, lo, count, pl.ip, fk, pl.tg, 3)) return MPS_ERROR;
-    accumulate(g, count);
-    const uint32_t n_rec = g->h_cnt->n_out;
+    if (!run_pass(g, L, pb, g->h_stage, kStageRecords, lo, count, pl.ip, fk, pl.tg, 3)) return MPS_ERROR;
+    accumulate(g, L, pb, count);
+    const uint32_t n_rec = L.h_cnt->n_out;
     if (n_rec > pb.out_cap) {
-        g->err = "device records exceed the scratch record area";
+        set_err(g, "device records exceed the scratch record area");
         return MPS_DECLINED;
     }
     // the records through the pinned staging area, kStageRecords at a time; composites (number 0,
@@ -799,8 +849,8 @@ int mps_gpu_sieve_relations_noalloc(mps_gpu* g, uint64_t lo, uint64_t hi, const 
         const uint32_t k = std::min<uint32_t>((uint32_t)kStageRecords, n_rec - off);
         if (off > 0) {  // the first slice came back with the counts
             hipError_t e = hipMemcpyAsync(g->h_stage, pb.out + 2 * (uint64_t)off, 2 * k * sizeof(uint64_t),
-                                          hipMemcpyDeviceToHost, g->stream);
-            if (e == hipSuccess) e = hipStreamSynchronize(g->stream);
+                                          hipMemcpyDeviceToHost, L.stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
             if (e != hipSuccess) {
                 hip_fail(g, e, "record read");
                 return MPS_ERROR;
@@ -815,8 +865,7 @@ int mps_gpu_sieve_relations_noalloc(mps_gpu* g, uint64_t lo, uint64_t hi, const 
             ++n;
         }
     }
-    g->st.field = (int)fk;
-    g->st.tbits = (int)pl.tg.nbbits;
+    record_shape(g, fk, pl.tg);
     *out_n = n;
     return n > out_cap ? MPS_CAPACITY : MPS_OK;
 }
@@ -825,7 +874,8 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
                    size_t todo_cap, size_t* n_todo) {
     if (n_todo) *n_todo = 0;
     if (g == nullptr || item == nullptr || n_todo == nullptr) return MPS_ERROR;
-    std::lock_guard<std::mutex> lock(g->mu);
+    std::unique_lock<std::mutex> lock;
+    PassLane& L = take_lane(g, lock);
     if (!g->ok || g->broken) return MPS_ERROR;
     if (n == 0) return MPS_OK;
     uint64_t pmax = 0, pmin = ~0ull;
@@ -834,53 +884,53 @@ int mps_gpu_filter(mps_gpu* g, const uint64_t* primes, size_t n, const mps_item*
         pmin = std::min(pmin, primes[i]);
     }
     Plan pl;
-    if (!prepare_item(g, item, pmin, false, pl)) return MPS_DECLINED;  // filter(): plain prep path
+    if (!prepare_item(g, L, item, pmin, false, pl)) return MPS_DECLINED;  // filter(): plain prep path
     const ItemParams& ip = pl.ip;
     if (pmax >= (1ull << 62)) {
-        g->err = "prime above 2^62";
+        set_err(g, "prime above 2^62");
         return MPS_DECLINED;
     }
     const FieldKind fk = pass_field(g, pmax + 1);
     const TableGeom tg = pl.tg;
     if (hipSetDevice(g->device) != hipSuccess) return MPS_ERROR;
-    if (!ensure_buffers(g, g->pass_odds)) return MPS_ERROR;
+    if (!ensure_buffers(g, L, g->pass_odds)) return MPS_ERROR;
     // every value is a prime and may take an entry, so a batch is at most the entry list: no entry
     // overflows to the host, and the flagged set does not depend on the order of the device's
     // slot reservations
-    const size_t batch = (size_t)std::min<uint64_t>(g->pb.ent_cap, 1ull << 22);
-    if (g->vals_cap < batch) {
-        hipFree(g->d_vals);
-        g->d_vals = nullptr;
-        if (hipMalloc(&g->d_vals, batch * sizeof(uint64_t)) != hipSuccess) {
+    const size_t batch = (size_t)std::min<uint64_t>(L.pb.ent_cap, 1ull << 22);
+    if (L.vals_cap < batch) {
+        hipFree(L.d_vals);
+        L.d_vals = nullptr;
+        if (hipMalloc(&L.d_vals, batch * sizeof(uint64_t)) != hipSuccess) {
             hip_fail(g, hipErrorOutOfMemory, "values alloc");
             return MPS_ERROR;
         }
-        g->vals_cap = batch;
+        L.vals_cap = batch;
     }
     std::vector<uint64_t> idx;
     for (size_t off = 0; off < n; off += batch) {
         const size_t cnt = std::min(batch, n - off);
-        if (hipMemcpyAsync(g->d_vals, primes + off, cnt * sizeof(uint64_t), hipMemcpyHostToDevice, g->stream) != hipSuccess) {
+        if (hipMemcpyAsync(L.d_vals, primes + off, cnt * sizeof(uint64_t), hipMemcpyHostToDevice, L.stream) != hipSuccess) {
             hip_fail(g, hipErrorUnknown, "values upload");
             return MPS_ERROR;
         }
-        g->pb.vals = g->d_vals;
-        g->pb.need_prime = 0;
-        g->pb.out_index = 1;
-        const bool ok = run_pass(g, g->pb, g->h_out, std::min<size_t>(kRecPrefix, g->pb.out_cap), 0, cnt, ip, fk, tg, 3);
-        g->pb.vals = nullptr;
-        g->pb.need_prime = 1;
-        g->pb.out_index = 0;
+        L.pb.vals = L.d_vals;
+        L.pb.need_prime = 0;
+        L.pb.out_index = 1;
+        const bool ok = run_pass(g, L, L.pb, L.h_out, std::min<size_t>(kRecPrefix, L.pb.out_cap), 0, cnt, ip, fk, tg, 3);
+        L.pb.vals = nullptr;
+        L.pb.need_prime = 1;
+        L.pb.out_index = 0;
         if (!ok) return MPS_ERROR;
-        accumulate(g, 0);
-        const uint32_t n_out = g->h_cnt->n_out;
-        if (n_out > g->pb.out_cap) {
-            g->err = "device records exceed the output buffer";
+        accumulate(g, L, L.pb, 0);
+        const uint32_t n_out = L.h_cnt->n_out;
+        if (n_out > L.pb.out_cap) {
+            set_err(g, "device records exceed the output buffer");
             return MPS_DECLINED;
         }
         if (n_out) {
-            if (!read_records(g, n_out)) return MPS_ERROR;
-            for (uint32_t i = 0; i < n_out; ++i) idx.push_back(off + g->h_out[2 * i]);
+            if (!read_records(g, L, n_out)) return MPS_ERROR;
+            for (uint32_t i = 0; i < n_out; ++i) idx.push_back(off + L.h_out[2 * i]);
         }
     }
     std::sort(idx.begin(), idx.end());
@@ -894,41 +944,42 @@ int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const m
                         size_t out_cap_words, size_t* out_n) {
     if (out_n) *out_n = 0;
     if (g == nullptr || item == nullptr || out_n == nullptr || stage < 0 || stage > 2) return MPS_ERROR;
-    std::lock_guard<std::mutex> lock(g->mu);
+    std::unique_lock<std::mutex> lock;
+    PassLane& L = take_lane(g, lock);
     if (!g->ok || g->broken) return MPS_ERROR;
     lo |= 1;
     if (lo <= 3 || hi >= (1ull << 62) || lo >= hi) return MPS_DECLINED;
     Plan pl;
-    if (!prepare_item(g, item, lo, g->classes_on, pl)) return MPS_DECLINED;
+    if (!prepare_item(g, L, item, lo, g->classes_on, pl)) return MPS_DECLINED;
     const ItemParams& ip = pl.ip;
     const FieldKind fk = pass_field(g, hi);
     const TableGeom tg = pl.tg;
     const uint64_t count = (hi - lo + 1) / 2;
     if (count > g->pass_odds) {
-        g->err = "debug stage covers one pass only";
+        set_err(g, "debug stage covers one pass only");
         return MPS_DECLINED;
     }
-    if (hipSetDevice(g->device) != hipSuccess || !ensure_buffers(g, count)) return MPS_ERROR;
-    g->pb.vals = nullptr;
-    g->pb.need_prime = 1;
-    g->pb.out_index = 0;
-    if (!run_pass(g, g->pb, g->h_out, std::min<size_t>(kRecPrefix, g->pb.out_cap), lo, count, ip, fk, tg, stage))
+    if (hipSetDevice(g->device) != hipSuccess || !ensure_buffers(g, L, count)) return MPS_ERROR;
+    L.pb.vals = nullptr;
+    L.pb.need_prime = 1;
+    L.pb.out_index = 0;
+    if (!run_pass(g, L, L.pb, L.h_out, std::min<size_t>(kRecPrefix, L.pb.out_cap), lo, count, ip, fk, tg, stage))
         return MPS_ERROR;
-    const Counters c = *g->h_cnt;
+    const Counters c = *L.h_cnt;
     size_t words = 0, recs = 0;
     if (stage == 0) {
-        const uint32_t n = std::min(c.n_surv, g->pb.surv_cap);
+        const uint32_t n = std::min(c.n_surv, L.pb.surv_cap);
         std::vector<uint32_t> v(n);
-        if (n && hipMemcpy(v.data(), g->pb.surv, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return MPS_ERROR;
+        if (n && hipMemcpy(v.data(), L.pb.surv, n * 4, hipMemcpyDeviceToHost) != hipSuccess) return MPS_ERROR;
         for (uint32_t i = 0; i < n; ++i) {
             if (words < out_cap_words) out[words] = lo + 2 * (uint64_t)v[i];
             ++words;
         }
         recs = n;
     } else if (stage == 1) {
-        const uint32_t n = std::min(c.n_cand, g->pb.cand_cap);
+        const uint32_t n = std::min(c.n_cand, L.pb.cand_cap);
         std::vector<uint64_t> v(n);
-        if (n && hipMemcpy(v.data(), g->pb.cand, n * 8, hipMemcpyDeviceToHost) != hipSuccess) return MPS_ERROR;
+        if (n && hipMemcpy(v.data(), L.pb.cand, n * 8, hipMemcpyDeviceToHost) != hipSuccess) return MPS_ERROR;
         for (uint32_t i = 0; i < n; ++i) {
             if (words + 1 < out_cap_words) {
                 out[words] = lo + 2 * (v[i] & 0xFFFFFFFFull);
@@ -938,9 +989,9 @@ int mps_gpu_debug_stage(mps_gpu* g, int stage, uint64_t lo, uint64_t hi, const m
         }
         recs = n;
     } else {
-        const uint32_t n = std::min(c.n_ent, g->pb.ent_cap);
+        const uint32_t n = std::min(c.n_ent, L.pb.ent_cap);
         std::vector<uint64_t> v((size_t)n * kEntWords);
-        if (n && hipMemcpy(v.data(), g->pb.ent, v.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return MPS_ERROR;
+        if (n && hipMemcpy(v.data(), L.pb.ent, v.size() * 8, hipMemcpyDeviceToHost) != hipSuccess) return MPS_ERROR;
         for (uint32_t i = 0; i < n; ++i) {
             const uint64_t* E = &v[(size_t)i * kEntWords];
             if (words + 3 < out_cap_words) {
@@ -961,30 +1012,36 @@ const char* mps_gpu_last_error(const mps_gpu* g) { return g ? g->err.c_str() : "
 
 int mps_gpu_get_stats(const mps_gpu* g, mps_gpu_stats* s) {
     if (g == nullptr || s == nullptr) return MPS_ERROR;
+    std::lock_guard<std::mutex> l(g->st_mu);
     *s = g->st;
     return MPS_OK;
 }
 
 void mps_gpu_reset_stats(mps_gpu* g) {
-    if (g) g->st = mps_gpu_stats{};
+    if (g == nullptr) return;
+    std::lock_guard<std::mutex> l(g->st_mu);
+    g->st = mps_gpu_stats{};
 }
 
 void mps_gpu_destroy(mps_gpu* g) {
     if (g == nullptr) return;
     if (g->ok) hipSetDevice(g->device);
-    free_buffers(g);
-    hipFree(g->d_vals);
+    for (PassLane& L : g->lane) {
+        if (L.stream) hipStreamSynchronize(L.stream);
+        free_buffers(L);
+        hipFree(L.d_vals);
+        hipFree(L.pb.cnt);
+        hipHostFree(L.h_cnt);
+        hipFree(L.d_csz);
+        hipHostFree(L.h_csz);
+        hipFree(L.d_desc);
+        hipHostFree(L.h_desc);
+        for (int i = 0; i < 6; ++i)
+            if (L.ev[i]) hipEventDestroy(L.ev[i]);
+        if (L.stream) hipStreamDestroy(L.stream);
+    }
     hipFree(g->d_sp);
-    hipFree(g->pb.cnt);
-    hipHostFree(g->h_cnt);
     hipHostFree(g->h_stage);
-    hipFree(g->d_csz);
-    hipHostFree(g->h_csz);
-    hipFree(g->d_desc);
-    hipHostFree(g->h_desc);
-    for (int i = 0; i < 6; ++i)
-        if (g->ev[i]) hipEventDestroy(g->ev[i]);
-    if (g->stream) hipStreamDestroy(g->stream);
     delete g;
 }
 
@@ -998,12 +1055,16 @@ bool valid_tuning(const mps_tuning& t) {
            t.policy <= 2;
 }
 
-double device_ms(const mps_gpu* g) { return g->st.ms_sieve + g->st.ms_filter + g->st.ms_setup + g->st.ms_bsgs; }
+double device_ms(const mps_gpu* g) {
+    std::lock_guard<std::mutex> l(g->st_mu);
+    return g->st.ms_sieve + g->st.ms_filter + g->st.ms_setup + g->st.ms_bsgs;
+}
 
 // One timed run of the sample under the current forced tuning: device ms, records sorted.
-int timed_sample(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item, std::vector<uint64_t>& rec, double& ms) {
+int timed_sample(mps_gpu* g, PassLane& L, uint64_t lo, uint64_t hi, const mps_item* item, std::vector<uint64_t>& rec,
+                 double& ms) {
     const double before = device_ms(g);
-    const int st = sieve_records(g, lo, hi, item, rec);
+    const int st = sieve_records(g, L, lo, hi, item, rec);
     ms = device_ms(g) - before;
     std::vector<std::pair<uint64_t, uint64_t>> pr(rec.size() / 2);
     for (size_t i = 0; i < pr.size(); ++i) pr[i] = {rec[2 * i], rec[2 * i + 1]};
@@ -1042,13 +1103,22 @@ extern "C" {
 int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sample_odds, mps_tuning* best,
                      double* best_ms, double* default_ms) {
     if (g == nullptr || item == nullptr || sample_odds == 0) return MPS_ERROR;
-    std::lock_guard<std::mutex> lock(g->mu);
+    AllLanes all(g);  // no other pass shares the device while shapes are timed
+    PassLane& L = g->lane[0];
     if (!g->ok || g->broken) return MPS_ERROR;
     const uint64_t lo = P0 | 1;
     sample_odds = std::min(sample_odds, g->pass_odds);
     const uint64_t hi = lo + 2 * sample_odds;
     if (lo <= 3 || hi >= (1ull << 62)) return MPS_DECLINED;
-    const mps_gpu_stats keep = g->st;
+    mps_gpu_stats keep;
+    {
+        std::lock_guard<std::mutex> l(g->st_mu);
+        keep = g->st;
+    }
+    auto restore_stats = [&]() {
+        std::lock_guard<std::mutex> l(g->st_mu);
+        g->st = keep;
+    };
     mps_tuning cand{};
     auto force = [&](bool on, const mps_tuning& t) {
         cand = t;
@@ -1061,12 +1131,12 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
     double ms = 0, t_default = 1e300;
     int st = MPS_OK;
     for (int rep = 0; rep < 3 && st == MPS_OK; ++rep) {  // the first run warms the kernels up
-        st = timed_sample(g, lo, hi, item, ref, ms);
+        st = timed_sample(g, L, lo, hi, item, ref, ms);
         if (rep > 0) t_default = std::min(t_default, ms);
     }
     if (st != MPS_OK) {
         force(false, mps_tuning{});
-        g->st = keep;
+        restore_stats();
         return st;
     }
     mps_tuning win{0, 0, 0, 0, 0, 0};
@@ -1079,10 +1149,10 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
         if (!plan_item(g, item, P0, g->classes_on, pl, why)) return true;  // the table would not fit
         double tbest = 1e300;
         for (int rep = 0; rep < 2 && st == MPS_OK; ++rep) {
-            st = timed_sample(g, lo, hi, item, rec, ms);
+            st = timed_sample(g, L, lo, hi, item, rec, ms);
             tbest = std::min(tbest, ms);
             if (st == MPS_OK && !records_agree(rec, ref)) {
-                g->err = "autotune: a candidate shape contradicts the default's records";
+                set_err(g, "autotune: a candidate shape contradicts the default's records");
                 st = MPS_ERROR;
             }
         }
@@ -1130,7 +1200,7 @@ int mps_gpu_autotune(mps_gpu* g, const mps_item* item, uint64_t P0, uint64_t sam
             }
     }
     force(false, mps_tuning{});
-    g->st = keep;
+    restore_stats();
     if (st != MPS_OK) return st;
     {
         std::lock_guard<std::mutex> l(g->tune_mu);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <string>
@@ -12,13 +13,35 @@
 #include "mps_gpu.h"
 #include "mps_kernels.h"
 
-// One device handle: the plain worker's stream and pass buffers, and the device-resident sieving
-// primes that the batched worker shares.
+// One pass lane of a handle: a stream with the buffers, pinned staging and events of one pass in
+// flight.  A call that runs passes holds one lane; with two lanes a second feeder thread's pass is
+// queued on the device while the first one's records come back and the host works on them, so the
+// device does not idle between the feeders' chunks.
+constexpr int kPassLanes = 2;
+struct PassLane {
+    std::mutex mu;  // held by the call that runs passes on this lane
+    hipStream_t stream = nullptr;
+    mps::PassBuffers pb{};  // pb.cnt is allocated at create, the rest by ensure_buffers
+    uint64_t buf_odds = 0;
+    uint64_t* d_vals = nullptr;  // the filter path's values
+    size_t vals_cap = 0;
+    mps::Counters* h_cnt = nullptr;
+    uint64_t* h_out = nullptr;
+    mps::PassDesc* d_desc = nullptr;
+    mps::PassDesc* h_desc = nullptr;
+    uint2* d_csz = nullptr;
+    uint2* h_csz = nullptr;
+    hipEvent_t ev[6] = {};
+};
+
+// One device handle: the plain worker's pass lanes, and the device-resident sieving primes that
+// the batched worker shares.  Settings (classes, policy, field, pass size, tuning defaults) change
+// only while the caller holds every lane.
 struct mps_gpu {
     int device = 0;
     bool ok = false;
     bool classes_on = true;
-    bool broken = false;
+    std::atomic<bool> broken{false};
     int field_mode = -1;
     uint64_t pass_odds = 1ull << 25;
     uint32_t mcap = 2048;
@@ -28,25 +51,16 @@ struct mps_gpu {
     // autotuned table shapes by item key
     mutable std::mutex tune_mu;
     std::map<uint32_t, mps_tuning> tuned;
-    std::mutex mu;
-    uint2* d_csz = nullptr;
-    uint2* h_csz = nullptr;
-    mps::PassDesc* d_desc = nullptr;
-    mps::PassDesc* h_desc = nullptr;
-    hipStream_t stream = nullptr;
+    PassLane lane[kPassLanes];
+    std::atomic<uint32_t> next_lane{0};  // the lane a call waits for when none is free
+    mutable std::mutex st_mu;  // st and err, written by calls on different lanes
+    std::mutex noalloc_mu;  // one allocation-free call at a time (they share the scratch array)
     mps::SievePrime* d_sp = nullptr;
     mps::SieveTable tab{};  // d_sp with its counts
-    mps::PassBuffers pb{};
-    uint64_t buf_odds = 0;
-    uint64_t* d_vals = nullptr;
-    size_t vals_cap = 0;
-    mps::Counters* h_cnt = nullptr;
-    uint64_t* h_out = nullptr;
     // allocation-free entry point: this device's scratch array (device address from
     // hipGetSymbolAddress at create) and a pinned record staging area of kStageRecords records
     void* scratch = nullptr;
     uint64_t* h_stage = nullptr;
-    hipEvent_t ev[6] = {};
     mps_gpu_stats st{};
     std::string err;
     std::string name;
