@@ -334,6 +334,11 @@ def run_configs(mps, g, log, rates, reps=3):
                 rels, _ = g.run_item(w, chunk_ints=CHUNK)
                 t_plain.append(time.perf_counter() - t)
             st = g.stats()
+            # the kernels alone on the device (one feeder: passes one after another) for k_bsgs's
+            # own launch time; with two feeders the passes of two chunks overlap
+            g.reset_stats()
+            rels_serial, _ = g.run_item(w, chunk_ints=CHUNK, feeder_threads=1)
+            st_serial = g.stats()
             eng = mps.Batch(g)
             t_batch, same = [], True
             for _ in range(reps):  # the first builds the slots' graphs
@@ -352,24 +357,25 @@ def run_configs(mps, g, log, rates, reps=3):
                      "primes": n, "relations": len(rels),
                      "plain_item_s": min(t_plain), "plain_primes_per_s": n / min(t_plain),
                      "batched_item_s": min(t_batch), "batched_primes_per_s": n / min(t_batch),
-                     "relations_identical": same,
+                     "relations_identical": same and rels_serial == rels,
                      "device": item_stats(st, reps),
                      "int_roofline": int_roofline(st, reps, min(t_plain), min(t_batch), rates),
-                     "kernel_roofline": kernel_roofline(st, rates)}
+                     "kernel_roofline": kernel_roofline(st_serial, rates)}
         log(f"{name}: plain {min(t_plain)*1e3:.1f} ms, batched {min(t_batch)*1e3:.1f} ms per item")
     return out
 
 
-def time_plain(g, item, steps, warmup):
-    """Wall time of `steps` runs of the item on the plain worker, after `warmup` runs."""
+def time_plain(g, item, steps, warmup, feeders=0):
+    """Wall time of `steps` runs of the item on the plain worker, after `warmup` runs; feeders = the
+    worker's feeder threads (0: the library's default, 2)."""
     for _ in range(warmup):
-        g.run_item(item, chunk_ints=CHUNK)
+        g.run_item(item, chunk_ints=CHUNK, feeder_threads=feeders)
     g.reset_stats()
     times, rels = [], None
     t_all = time.perf_counter()
     for _ in range(steps):
         t0 = time.perf_counter()
-        rels, _ = g.run_item(item, chunk_ints=CHUNK)
+        rels, _ = g.run_item(item, chunk_ints=CHUNK, feeder_threads=feeders)
         times.append(time.perf_counter() - t0)
     ms = 1000.0 * (time.perf_counter() - t_all) / steps
     return ms, times, rels, g.stats()
@@ -381,6 +387,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--feeders", type=int, default=0,
+                    help="feeder threads of the plain worker's timed legs (0: the library's default, 2; "
+                         "1 runs the passes one after another, for kernel profiles)")
     ap.add_argument("--cpu-slice", type=int, default=1 << 27)
     ap.add_argument("--mix", type=int, default=256, help="items of the serving mix (0 = skip the mix)")
     ap.add_argument("--mix-load", type=float, default=0.7, help="arrival rate / plain worker capacity")
@@ -416,7 +425,7 @@ def main():
         b, c, P0, P1, N0, N1, mod, res, classes = CONFIGS[args.profile_config]
         g.set_classes(classes)
         it = mps.work_item(b, c, P0, P1, N0, N1, mod, res)
-        ms, _, rels, st = time_plain(g, it, args.steps, args.warmup)
+        ms, _, rels, st = time_plain(g, it, args.steps, args.warmup, args.feeders)
         log(f"{args.profile_config} plain {ms:.2f} ms/item")
         print(json.dumps({"config": args.profile_config, "ms_per_item": ms, "relations": len(rels),
                           "device": item_stats(st, args.steps)}), flush=True)
@@ -461,11 +470,19 @@ def main():
             tuning_info["mix_items_with_tuned_key"] = sum(g.get_tuning(x)[1] in table_keys for x in wis_t)
     log(f"tuning: {tuning_info}")
 
-    ms_per_step, times, rels, s = time_plain(g, item, args.steps, args.warmup)
+    ms_per_step, times, rels, s = time_plain(g, item, args.steps, args.warmup, args.feeders)
     log(f"c2 plain {ms_per_step:.2f} ms/step")
+    # the same item with one feeder: its passes run one after another, each kernel alone on the
+    # device, so the HIP-event times are the kernels' own (with two feeders two chunks' passes
+    # overlap, and a launch's event time includes the other pass's kernels)
+    if args.feeders == 1:
+        ms_serial, rels_serial, s_serial = ms_per_step, rels, s
+    else:
+        ms_serial, _, rels_serial, s_serial = time_plain(g, item, args.steps, 1, feeders=1)
+        log(f"c2 plain, one feeder {ms_serial:.2f} ms/step")
 
     # roofline of the dominant kernel (k_bsgs), from HIP events on the worker's stream
-    kr = kernel_roofline(s, rates)
+    kr = kernel_roofline(s_serial, rates)
     roof = {"bound": "valu", "achieved": kr["achieved"] / 1e12, "peak": kr["peak"] / 1e12,
             "unit": "TFLOP/s" if s["field"] == 1 else "T 32x32 multiplies/s", "frac": kr["frac"],
             "traffic": K_BSGS_TRAFFIC_BYTES, "kernel": kr["kernel"], "ms_per_launch": kr["ms_per_launch"],
@@ -488,8 +505,15 @@ def main():
                    "chunk_ints": CHUNK, "primes": nprimes, "relations": len(rels),
                    "tuning": tuning_info,
                    "p50_step_ms": sorted(times)[len(times) // 2] * 1000,
-                   "stage_ms_per_step": {k: s[k] / args.steps for k in ("ms_sieve", "ms_filter", "ms_setup",
-                                                                          "ms_bsgs")},
+                   "feeders": args.feeders or 2,
+                   "serial_ms_per_step": ms_serial,
+                   "serial_relations_identical": rels_serial == rels,
+                   # per-stage HIP-event times of the one-feeder leg (stages in sequence) and of the
+                   # timed leg (two chunks' passes overlap, so its stages add up to more than a step)
+                   "stage_ms_per_step": {k: s_serial[k] / args.steps for k in ("ms_sieve", "ms_filter", "ms_setup",
+                                                                                 "ms_bsgs")},
+                   "stage_ms_overlapped_per_step": {k: s[k] / args.steps for k in ("ms_sieve", "ms_filter",
+                                                                                     "ms_setup", "ms_bsgs")},
                    "entries_per_step": s["entries"] // args.steps,
                    "survivors_per_step": s["survivors"] // args.steps,
                    "candidates_per_step": s["candidates"] // args.steps,
