@@ -4,8 +4,12 @@
  * This is the drop-in boundary: it replaces the reference's GpuSiever chunk interface
  * (Gpu.hpp:1040-1074) and the OpenCL/CUDA host workers behind it (Gpu.hpp:654-1036,
  * Cuda.hpp:195-455).  Plain C types only: no exceptions, no C++ or torch types cross it.
- * A handle may be used from several threads; calls on one handle are serialised, as the
- * reference serialises its feeder threads with a mutex (Gpu.hpp:1059, :1065).
+ * A handle may be used from several threads.  It has two pass lanes (a HIP stream with its own
+ * buffers each): up to two calls on one handle run at once, so that a second feeder thread's
+ * chunk runs on the device while the first one's records come back, where the reference
+ * serialises its feeder threads with a mutex (Gpu.hpp:1059, :1065); further calls wait for a lane.
+ * The settings calls (mps_gpu_set_classes, mps_gpu_set_class_policy, mps_gpu_configure) and
+ * mps_gpu_autotune wait until no call runs on the handle.  Results do not depend on the overlap.
  */
 #ifndef MPS_GPU_H
 #define MPS_GPU_H
@@ -97,8 +101,8 @@ int mps_gpu_sieve_relations(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item
  * pass on a scratch array of about 46 MB that the library keeps per GPU for the life of the
  * process (set up at mps_gpu_create); the pass's stage counters, survivors, candidates, BSGS
  * entries and records live there.  Results are those of mps_gpu_sieve_relations on the same
- * range.  Calls on one handle are serialised; calls through different handles on the same GPU
- * share its scratch array and must not overlap. */
+ * range.  Allocation-free calls on one handle are serialised; allocation-free calls through
+ * different handles on the same GPU share its scratch array and must not overlap. */
 #define MPS_NOALLOC_MAX_INTS (1ull << 20)
 int mps_gpu_sieve_relations_noalloc(mps_gpu* g, uint64_t lo, uint64_t hi, const mps_item* item,
                                     mps_dev_relation* out, size_t out_cap, size_t* out_n);
