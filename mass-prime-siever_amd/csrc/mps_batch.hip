@@ -336,16 +336,6 @@ std::string chunk_name(const Task& t) {
     return buf;
 }
 
-// The CPUs this process may run on (at least 1).
-int host_cpus() {
-    static const int n = [] {
-        cpu_set_t set;
-        CPU_ZERO(&set);
-        return sched_getaffinity(0, sizeof(set), &set) == 0 ? std::max(1, CPU_COUNT(&set)) : 4;
-    }();
-    return n;
-}
-
 // One chunk [lo, hi) of an item on slot s: returns the chunk's status.
 int run_chunk(mps_batch* b, Slot& s, const Task& t, std::vector<mps_relation>& rels, uint64_t& prime_flagged,
               double& dev_s, double& host_s) {

@@ -1,6 +1,8 @@
 // Host runtime: the GPU branch of process_chunk (Worker.hpp:195-303) driven by feeder threads
 // (run_worker_pool, Worker.hpp:305-367), and the results.json object of an item
 // (apply_relations Worker.hpp:527-559, append_results_json Results.hpp:72-140).
+#include <sched.h>
+
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -115,13 +117,24 @@ int finish_records_parallel(const mps_work_item* it, uint64_t lo, uint64_t hi, m
     return st;
 }
 
+int host_cpus() {
+    static const int n = [] {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        return sched_getaffinity(0, sizeof(set), &set) == 0 ? std::max(1, CPU_COUNT(&set)) : 4;
+    }();
+    return n;
+}
+
 }  // namespace mps
 
 namespace {
 
-// One chunk on the plain worker: device pass(es), then finish_records.
+// One chunk on the plain worker: device pass(es), then finish_records on `threads` threads when the
+// chunk has many records (thousands, for small b and c over long exponent ranges, each needing
+// q = ord_p(b) from a factorisation of p - 1).
 void process_chunk(mps_gpu* g, const mps_work_item* it, uint64_t lo, uint64_t hi, std::vector<mps_dev_relation>& buf,
-                   ChunkResult& r) {
+                   int threads, ChunkResult& r) {
     const mps_item di = {it->b, it->c, it->N0, it->N1, it->cls_mod, it->cls_res};
     if (!(hi < (1ull << 62) && lo > 3)) {  // gpu_range_ok (Worker.hpp:209)
         r.status = MPS_DECLINED;
@@ -141,7 +154,8 @@ void process_chunk(mps_gpu* g, const mps_work_item* it, uint64_t lo, uint64_t hi
         return;
     }
     r.flagged += n;
-    if (mps::finish_records(it, lo, hi, buf.data(), n, r.rels, &r.prime_flagged) != MPS_OK) r.status = MPS_ERROR;
+    if (mps::finish_records_parallel(it, lo, hi, buf.data(), n, r.rels, &r.prime_flagged, threads, 512) != MPS_OK)
+        r.status = MPS_ERROR;
     r.chk_s += now_s() - t1;
 }
 
@@ -182,6 +196,8 @@ int mps_run_item(mps_gpu* g, const mps_work_item* item, const mps_run_opts* opts
     if (g == nullptr || item == nullptr || nrel == nullptr || !mps_gpu_ok(g)) return MPS_ERROR;
     const uint64_t chunk = (opts && opts->chunk_ints) ? opts->chunk_ints : (1ull << 26);
     const int feeders = (opts && opts->feeder_threads > 0) ? opts->feeder_threads : 2;
+    // the host threads a feeder may use for one chunk's records: its share of the CPUs, at most 8
+    const int finish_threads = std::min(8, std::max(1, mps::host_cpus() / feeders));
     std::vector<std::pair<uint64_t, uint64_t>> chunks;  // enqueue_prime_chunks (Worker.hpp:176-193)
     for (uint64_t c = item->P0; c < item->P1; c += std::min(chunk, item->P1 - c))
         chunks.emplace_back(c, c + std::min(chunk, item->P1 - c));
@@ -202,7 +218,7 @@ int mps_run_item(mps_gpu* g, const mps_work_item* item, const mps_run_opts* opts
                 break;
             }
             ChunkResult r;
-            process_chunk(g, item, chunks[k].first, chunks[k].second, buf, r);
+            process_chunk(g, item, chunks[k].first, chunks[k].second, buf, finish_threads, r);
             if (on_chunk && r.status == MPS_OK) on_chunk(ctx, id, chunks[k].first, chunks[k].second, r.rels.data(), r.rels.size());
             mine.rels.insert(mine.rels.end(), r.rels.begin(), r.rels.end());
             mine.dev_s += r.dev_s;

@@ -21,4 +21,7 @@ int finish_records_parallel(const mps_work_item* it, uint64_t lo, uint64_t hi, m
                             std::vector<mps_relation>& rels, uint64_t* prime_flagged, int threads,
                             size_t min_records);
 
+// The CPUs this process may run on (at least 1).
+int host_cpus();
+
 }  // namespace mps
