@@ -433,6 +433,72 @@ MPS_HD void fpow2(const F& f, typename F::T a, typename F::T b, E e, typename F:
     }
 }
 
+// ---------------------------------------------------------------- small bases, Montgomery field
+// b and t = -c of an item are small integers; their powers modulo p > 2^50 are the bulk of the
+// order-mode setup.  Left to right, w bits at a time, a power of a small base s costs w squarings
+// and one product by the small integer s^digit per window, against a squaring and a full product
+// per bit right to left.
+
+// a * s mod p for a < p < 2^62 and 1 <= s < 2^24, pinv = 1.0 / p (a Montgomery form times a plain
+// integer is the Montgomery form of the product).  q = the double a * s * pinv truncated: its
+// relative error is a few 2^-53 on a value below 2^24, so q is floor(a * s / p) or one off, and
+// a * s - q * p, from the low 64 bits of both products, lies in (-p, 2p); one correction.
+MPS_HD uint64_t mont_mul_small(uint64_t a, uint32_t s, uint64_t p, double pinv) {
+    const uint32_t q = (uint32_t)((double)a * (double)s * pinv);
+    uint64_t r = a * (uint64_t)s - (uint64_t)q * p;
+    if ((int64_t)r < 0)
+        r += p;
+    else if (r >= p)
+        r -= p;
+    return r;
+}
+
+// s^d for a w-bit digit d from sp[i] = s^(2^i), i < w (the caller keeps s^(2^w - 1) < 2^24).
+MPS_HD uint32_t small_digit_pow(const uint32_t* sp, uint32_t w, uint32_t d) {
+    uint32_t r = 1;
+    for (uint32_t i = 0; i < w; ++i) r *= ((d >> i) & 1u) ? sp[i] : 1u;
+    return r;
+}
+
+// The Montgomery forms of sa^e and sb^e (sa, sb given by their window powers, w bits per window,
+// 1 <= w <= 4), left to right: the first window's digit, then per window w squarings of each and
+// one mont_mul_small each.  The two chains are independent (they overlap in the pipeline).
+MPS_HD void mont_pow2_small(const MontField& f, const uint32_t* sa, const uint32_t* sb, uint32_t w, uint64_t e,
+                            double pinv, uint64_t& ra, uint64_t& rb) {
+    ra = f.one();
+    rb = f.one();
+    if (e == 0) return;
+    const uint32_t mask = (1u << w) - 1u;
+    int k = (int)((63u - (uint32_t)__builtin_clzll(e)) / w * w);
+    for (bool first = true; k >= 0; k -= (int)w, first = false) {
+        if (!first)
+            for (uint32_t i = 0; i < w; ++i) {
+                ra = f.mul(ra, ra);
+                rb = f.mul(rb, rb);
+            }
+        const uint32_t d = (uint32_t)(e >> k) & mask;
+        ra = mont_mul_small(ra, small_digit_pow(sa, w, d), f.p, pinv);
+        rb = mont_mul_small(rb, small_digit_pow(sb, w, d), f.p, pinv);
+    }
+}
+
+// mont_pow2_small for one base.
+MPS_HD uint64_t mont_pow_small(const MontField& f, const uint32_t* s, uint32_t w, uint64_t e, double pinv) {
+    uint64_t r = f.one();
+    if (e == 0) return r;
+    const uint32_t mask = (1u << w) - 1u;
+    int k = (int)((63u - (uint32_t)__builtin_clzll(e)) / w * w);
+    for (bool first = true; k >= 0; k -= (int)w, first = false) {
+        if (!first)
+            for (uint32_t i = 0; i < w; ++i) r = f.mul(r, r);
+        r = mont_mul_small(r, small_digit_pow(s, w, (uint32_t)(e >> k) & mask), f.p, pinv);
+    }
+    return r;
+}
+
+// -x mod p (x < p).
+MPS_HD uint64_t mont_neg(const MontField& f, uint64_t x) { return x ? f.p - x : 0; }
+
 template <class F>
 MPS_HD bool fequal(const F& f, typename F::T a, typename F::T b) {
     return f.key(a) == f.key(b);

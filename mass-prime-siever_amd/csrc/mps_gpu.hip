@@ -289,6 +289,34 @@ bool item_params(const mps_item* it, bool classes_on, int class_policy, uint32_t
         ip.ext_end = e1;
         ip.ext_kmin = kExtDefaultKmin;
         ip.ext_fields = ext_bound == 0 ? 1u << FIELD_MONT : (1u << FIELD_MONT) | (1u << FIELD_F64);
+        // small bases: the widest window (at most 4 bits) whose digit powers of |b| and |c| stay
+        // below 2^24
+        const uint64_t ab = it->b < 0 ? 0ull - (uint64_t)it->b : (uint64_t)it->b;
+        const uint64_t ac = it->c < 0 ? 0ull - (uint64_t)it->c : (uint64_t)it->c;
+        const uint64_t smax = std::max(ab, ac);
+        ip.smw = 0;
+        if (ab != 0 && ac != 0 && smax < (1ull << 24))
+            for (uint32_t w = 4; w >= 1 && ip.smw == 0; --w) {
+                uint64_t x = 1;
+                bool fits = true;
+                for (uint32_t k = 0; k + 1 < (1u << w) && fits; ++k) fits = (x *= smax) < (1ull << 24);
+                if (fits) ip.smw = w;
+            }
+        ip.bneg = it->b < 0;
+        ip.tneg = it->c > 0;
+        for (uint32_t i = 0; i < 4; ++i) {
+            ip.bsp[i] = 0;
+            ip.tsp[i] = 0;
+        }
+        for (uint32_t i = 0; i < ip.smw; ++i) {
+            uint64_t xb = ab, xc = ac;
+            for (uint32_t k = 0; k < i; ++k) {
+                xb *= xb;
+                xc *= xc;
+            }
+            ip.bsp[i] = (uint32_t)xb;
+            ip.tsp[i] = (uint32_t)xc;
+        }
         // an entry's class has at most kcount members: m <= sqrt(kcount) in whole rounds, <= mcap
         uint64_t m = 1;
         while (m * m < ip.kcount) ++m;

@@ -157,6 +157,11 @@ struct ItemParams {
     uint32_t ext_first, ext_end;
     uint64_t ext_kmin;
     uint32_t ext_fields;
+    // order mode in the Montgomery field, small bases (mont_pow2_small): window bits smw (0: off)
+    // and the window powers |b|^(2^i), |c|^(2^i) (i < smw); b^e = (-1)^e |b|^e when bneg, and
+    // t^e = (-c)^e = (-1)^e |c|^e when tneg (c > 0)
+    uint32_t smw, bneg, tneg;
+    uint32_t bsp[4], tsp[4];
 };
 
 // BSGS table geometry of a pass: 2^nbbits buckets of 8 slots; a slot holds a fingerprint of

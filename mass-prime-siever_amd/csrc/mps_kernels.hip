@@ -589,8 +589,14 @@ struct ExtTest {
 };
 
 
+// The sign of a small base's power: (-1)^e times the power of |s| when the base is negative.
+__device__ __forceinline__ uint64_t mont_sign(const MontField& f, uint64_t x, uint64_t e, uint32_t neg) {
+    return (neg && (e & 1)) ? mont_neg(f, x) : x;
+}
+
 // b^e and t^e (t = -c) modulo p in the field f: with the FP64 field, b and -c as small integers
-// when their products with residues stay exact (fpow2_small), else from their field forms.
+// when their products with residues stay exact (fpow2_small); with the Montgomery field, |b| and
+// |c| by windows when they are small (mont_pow2_small, ip.smw); else from their field forms.
 template <class F>
 __device__ __forceinline__ void pow_b_t(const F& f, uint64_t p, const ItemParams& ip, uint64_t e, typename F::T& Be,
                                         typename F::T& Te) {
@@ -599,6 +605,13 @@ __device__ __forceinline__ void pow_b_t(const F& f, uint64_t p, const ItemParams
         const double sb = (double)bmod, sc = -(double)ip.c, pd = (double)p;
         if (sb * pd < 0x1p52 && fabs(sc) * pd < 0x1p52) {
             fpow2_small(f, sb, sc, e, Be, Te);
+            return;
+        }
+    } else {
+        if (ip.smw != 0) {
+            mont_pow2_small(f, ip.bsp, ip.tsp, ip.smw, e, 1.0 / (double)p, Be, Te);
+            Be = mont_sign(f, Be, e, ip.bneg);
+            Te = mont_sign(f, Te, e, ip.tneg);
             return;
         }
     }
@@ -663,6 +676,12 @@ __device__ __forceinline__ typename F::T pow_b_or_t(const F& f, uint64_t p, cons
     if constexpr (F::kSmallMul) {
         const double sb = (double)bmod, sc = -(double)ip.c, pd = (double)p;
         if (sb * pd < 0x1p52 && fabs(sc) * pd < 0x1p52) return fpow_small(f, tbase ? sc : sb, e);
+    } else {
+        if (ip.smw != 0) {
+            uint32_t sp[4];
+            for (uint32_t i = 0; i < 4; ++i) sp[i] = tbase ? ip.tsp[i] : ip.bsp[i];
+            return mont_sign(f, mont_pow_small(f, sp, ip.smw, e, 1.0 / (double)p), e, tbase ? ip.tneg : ip.bneg);
+        }
     }
     const typename F::T rr = f.r2();
     return fpow(f, f.from(tbase ? tv : bmod, rr), e);
@@ -883,7 +902,16 @@ __device__ __forceinline__ void order_entry(const OrderRound<F>& R, uint32_t cc,
     E[E_MG] = mg;
     E[E_SSRC] = 1u | ((uint64_t)(info & 0xFFFFFFu) << 8) | ((uint64_t)idx << 32);
     T B, G1;
-    fpow_2e(f, bm, oL, onst, B, G1);
+    bool small = false;
+    if constexpr (!F::kSmallMul) {
+        if (ip.smw != 0) {  // b^L and b^n_start by windows of the small base |b|
+            const double pinv = 1.0 / (double)pc;
+            B = mont_sign(f, mont_pow_small(f, ip.bsp, ip.smw, oL, pinv), oL, ip.bneg);
+            G1 = mont_sign(f, mont_pow_small(f, ip.bsp, ip.smw, onst, pinv), onst, ip.bneg);
+            small = true;
+        }
+    }
+    if (!small) fpow_2e(f, bm, oL, onst, B, G1);
     w_setup += 2 * bitlen64(oL) + 2 * bitlen64(onst);
     E[E_G] = F::to_bits(G1);
     E[E_NB] = onst;

@@ -398,6 +398,59 @@ static void check_ext(int scale) {
     std::printf("medium primes: %u primes %u..%u, %ld checks\n", kExtPrimes, T.e[0].r, prev, n);
 }
 
+// Small bases in the Montgomery field: mont_mul_small against 128-bit a * s mod p (p odd up to
+// 2^62, s up to 2^24 - 1, a at 0, p - 1 and random), and mont_pow2_small / mont_pow_small against
+// fpow of the Montgomery form (right to left, full products) for windows of 1 to 4 bits with the
+// largest bases each width allows (the item constants' rule: s^(2^w - 1) < 2^24), exponents 0, 1,
+// 2^k and random up to 2^62; mont_neg against p - x.
+static void check_mont_small(int scale) {
+    const int n_mod = 300 * scale;
+    long n = 0;
+    for (int i = 0; i < n_mod; ++i) {
+        const uint64_t p = (i % 4 == 0) ? ((1ull << 62) - 57) : ((rnd_below(1ull << (51 + rnd_below(11))) | 1) | (1ull << 50));
+        MontField f;
+        f.init(p);
+        const double pinv = 1.0 / (double)p;
+        for (int k = 0; k < 64; ++k) {
+            const uint64_t a = k == 0 ? 0 : k == 1 ? p - 1 : rnd_below(p);
+            const uint32_t s = k == 2 ? (1u << 24) - 1 : k == 3 ? 1u : (uint32_t)(1 + rnd_below((1u << (1 + rnd_below(24))) - 1));
+            const uint64_t ref = (uint64_t)((u128)a * s % p);
+            CHECK(mont_mul_small(a, s, p, pinv) == ref, "mont_mul_small p %llu a %llu s %u", (unsigned long long)p,
+                  (unsigned long long)a, s);
+            CHECK(mont_neg(f, a) == (a ? p - a : 0), "mont_neg");
+            ++n;
+        }
+        static const uint32_t smax[5] = {0, (1u << 24) - 1, 255, 10, 3};  // largest |s| per window width
+        for (uint32_t w = 1; w <= 4; ++w) {
+            for (int k = 0; k < 6; ++k) {
+                const uint32_t sa = k == 0 ? smax[w] : 1 + (uint32_t)rnd_below(smax[w]);
+                const uint32_t sb = k == 1 ? 1u : 1 + (uint32_t)rnd_below(smax[w]);
+                uint32_t pa[4] = {0, 0, 0, 0}, pb[4] = {0, 0, 0, 0};
+                for (uint32_t j = 0; j < w; ++j) {
+                    uint64_t xa = sa, xb = sb;
+                    for (uint32_t q = 0; q < j; ++q) {
+                        xa *= xa;
+                        xb *= xb;
+                    }
+                    pa[j] = (uint32_t)xa;
+                    pb[j] = (uint32_t)xb;
+                }
+                const uint64_t e = k == 2 ? 0 : k == 3 ? 1 : k == 4 ? 1ull << rnd_below(62) : rnd_below(1ull << 62);
+                const uint64_t ma = f.from(sa % p, f.r2()), mb = f.from(sb % p, f.r2());
+                const uint64_t ra = fpow(f, ma, e), rb = fpow(f, mb, e);
+                uint64_t ga, gb;
+                mont_pow2_small(f, pa, pb, w, e, pinv, ga, gb);
+                CHECK(ga == ra && gb == rb, "mont_pow2_small p %llu w %u s %u %u e %llu", (unsigned long long)p, w, sa,
+                      sb, (unsigned long long)e);
+                CHECK(mont_pow_small(f, pa, w, e, pinv) == ra, "mont_pow_small p %llu w %u s %u e %llu",
+                      (unsigned long long)p, w, sa, (unsigned long long)e);
+                ++n;
+            }
+        }
+    }
+    std::printf("mont small bases: %ld cases\n", n);
+}
+
 int main(int argc, char** argv) {
     const int scale = argc > 1 ? std::atoi(argv[1]) : 1;
     check_mont(scale);
@@ -407,6 +460,7 @@ int main(int argc, char** argv) {
     check_jacobi(scale);
     check_bucket(scale);
     check_ext(scale);
+    check_mont_small(scale);
     std::printf("ok: %ld checks\n", g_checks);
     return 0;
 }

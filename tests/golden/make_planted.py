@@ -276,9 +276,36 @@ def pseudoprime_items():
     return items
 
 
+# Small b and c in the Montgomery field (p > 2^50): primes p of b^n + c found by factoring b^n + c
+# (sympy.factorint, n up to 75), so that b and t = -c stay small integers, as in c4's (3, -2), and
+# the order-mode setup takes its small-base powers (windows of 4 bits for |b|, |c| <= 3, 3 bits up
+# to 10).  c > 0 makes t negative.
+SMALL_BASE_CASES = [
+    (3, -2, 37, 450283905890997361),
+    (3, -2, 40, 11246684050931479),
+    (3, 2, 36, 150094635296999123),
+    (3, 2, 41, 1387597351233433),
+    (2, 3, 61, 461168601842738791),
+    (7, -5, 36, 2947742064732341939),
+]
+
+
+def small_base_items():
+    items = []
+    for k, (b, c, n_pl, p) in enumerate(SMALL_BASE_CASES):
+        assert sympy.isprime(p) and (pow(b, n_pl, p) + c) % p == 0 and p > (1 << 50)
+        mod, res = (6, n_pl % 6) if k % 2 else (1, 0)
+        P0, P1 = window(p, ("start", "mid", "end")[k % 3])
+        M, o = smallest_solution(p, b, n_pl, 0, 10**7, mod, res)
+        items.append({"tag": f"mont-small/{b},{c},n{n_pl}", "item": [b, c, P0, P1, 0, 10**7, mod, res], "p": p,
+                      "n_planted": n_pl, "relation": [p, M, o] if M is not None else None, "fields": fields_of(p)})
+    return items
+
+
 def main():
     rng = random.Random(20261204)
-    items = planted_constant_items() + large_base_items(rng) + smooth_items(rng) + pseudoprime_items()
+    items = (planted_constant_items() + large_base_items(rng) + smooth_items(rng) + pseudoprime_items() +
+             small_base_items())
     with open(OUT, "w") as f:
         json.dump({"window": W, "items": items}, f, indent=1)
         f.write("\n")
