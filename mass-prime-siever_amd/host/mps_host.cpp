@@ -196,11 +196,23 @@ int mps_run_item(mps_gpu* g, const mps_work_item* item, const mps_run_opts* opts
     if (g == nullptr || item == nullptr || nrel == nullptr || !mps_gpu_ok(g)) return MPS_ERROR;
     const uint64_t chunk = (opts && opts->chunk_ints) ? opts->chunk_ints : (1ull << 26);
     const int feeders = (opts && opts->feeder_threads > 0) ? opts->feeder_threads : 2;
-    // the host threads a feeder may use for one chunk's records: its share of the CPUs, at most 8
-    const int finish_threads = std::min(8, std::max(1, mps::host_cpus() / feeders));
     std::vector<std::pair<uint64_t, uint64_t>> chunks;  // enqueue_prime_chunks (Worker.hpp:176-193)
     for (uint64_t c = item->P0; c < item->P1; c += std::min(chunk, item->P1 - c))
         chunks.emplace_back(c, c + std::min(chunk, item->P1 - c));
+    // an item of fewer chunks than feeders is split evenly among them (chunks of at least 2^20
+    // integers), so that every feeder's pass lane has work; the relations do not depend on where
+    // the chunks end
+    const uint64_t span = item->P1 > item->P0 ? item->P1 - item->P0 : 0;
+    constexpr uint64_t kMinSplit = 1ull << 20;
+    if (chunks.size() < (size_t)feeders && span >= 2 * kMinSplit) {
+        const uint64_t parts = std::min<uint64_t>((uint64_t)feeders, span / kMinSplit);
+        chunks.clear();
+        for (uint64_t k = 0; k < parts; ++k)
+            chunks.emplace_back(item->P0 + (uint64_t)((unsigned __int128)span * k / parts),
+                                item->P0 + (uint64_t)((unsigned __int128)span * (k + 1) / parts));
+    }
+    // the host threads a feeder may use for one chunk's records: its share of the CPUs, at most 8
+    const int finish_threads = std::min(8, std::max(1, mps::host_cpus() / feeders));
     std::atomic<size_t> next{0};
     std::mutex mu;
     ChunkResult total;

@@ -381,3 +381,25 @@ def test_chunks_of_many_passes(gpu, mps, oracle, pass_odds):
             assert rels == want, it
     finally:
         gpu.configure(-1, 1 << 25)
+
+
+SPLIT_ITEMS = [
+    (2, -25, 10**9, 10**9 + 2**22, 3_600_000, 5_000_000, 2, 1),  # relations at small p
+    (10, 111, 10**12, 10**12 + 3 * 2**20 + 5, 0, 10**7, 6, 4),  # c2's item, a short odd-sized window
+]
+
+
+@pytest.mark.parametrize("it", SPLIT_ITEMS, ids=["c1-like", "c2-like"])
+def test_small_item_split_among_feeders(gpu, mps, oracle, it):
+    """An item of fewer 2^26-integer chunks than feeder threads is split evenly among the feeders
+    (each on its own pass lane): the relations equal those of one feeder running the single chunk,
+    and the oracle's."""
+    print(f"split item {it}", flush=True)
+    w = mps.work_item(*it)
+    rels2, st2 = gpu.run_item(w)  # two feeders: two chunks
+    rels1, st1 = gpu.run_item(w, feeder_threads=1)  # one chunk
+    assert st2.declined == 0 and st1.declined == 0
+    assert st2.chunks == 2 and st1.chunks == 1
+    want, _ = oracle.process_range(oracle.make_item(*it), it[2], it[3], 1 << 18, 8)
+    assert rels2 == rels1 == want
+    assert it[0] != 2 or len(want) > 0
