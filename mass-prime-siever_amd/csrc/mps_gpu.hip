@@ -439,16 +439,27 @@ bool hip_fail(mps_gpu* g, hipError_t e, const char* what) {
 // The calling thread's pass lane, locked in lk: a free lane, else (all busy) the next in turn once
 // it frees.
 PassLane& take_lane(mps_gpu* g, std::unique_lock<std::mutex>& lk) {
-    for (int i = 0; i < kPassLanes; ++i) {
+    // g->lanes changes only under every lane's lock; read before a lane is held, it may be stale,
+    // which is harmless: lane 0 always serves, and a retired lane's buffers are allocated again
+    // by ensure_buffers if a call still takes it
+    const int n = std::max(1, std::min(g->lanes.load(), kPassLanes));
+    for (int i = 0; i < n; ++i) {
         std::unique_lock<std::mutex> t(g->lane[i].mu, std::try_to_lock);
         if (t.owns_lock()) {
             lk = std::move(t);
             return g->lane[i];
         }
     }
-    PassLane& L = g->lane[g->next_lane.fetch_add(1) % kPassLanes];
+    PassLane& L = g->lane[g->next_lane.fetch_add(1) % (uint32_t)n];
     lk = std::unique_lock<std::mutex>(L.mu);
     return L;
+}
+
+// Device bytes of one lane's pass buffers for `odds` odd numbers per pass (ensure_buffers).
+uint64_t lane_bytes(uint64_t odds) {
+    const uint64_t ent_cap = std::max<uint64_t>(odds / 8, 4096);
+    return odds * (sizeof(uint32_t) + sizeof(uint64_t)) + ent_cap * kEntWords * sizeof(uint64_t) +
+           2 * record_cap(odds) * sizeof(uint64_t);
 }
 
 // Every lane, taken in index order: no pass runs while the holder changes settings or times shapes.
@@ -667,6 +678,8 @@ mps_gpu* mps_gpu_create(int device_index, char* banner, size_t banner_cap) {
     g->device = device_index;
     g->name = prop.name;
     g->num_cu = prop.multiProcessorCount;
+    g->mem_bytes = prop.totalGlobalMem;
+    g->lanes = 2 * lane_bytes(g->pass_odds) <= g->mem_bytes / 4 * 3 ? kPassLanes : 1;
     bool ok = hipSetDevice(device_index) == hipSuccess;
     for (PassLane& L : g->lane) {  // streams at the default priority
         ok = ok && hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking) == hipSuccess &&
@@ -745,7 +758,15 @@ int mps_gpu_configure(mps_gpu* g, int field, uint64_t pass_odds, uint32_t mcap, 
     g->field_mode = field;
     if (waves) g->waves = waves;
     if (mcap) g->mcap = mcap;
-    if (pass_odds) g->pass_odds = pass_odds;
+    if (pass_odds) {
+        g->pass_odds = pass_odds;
+        // a second lane only while two passes' buffers take at most 3/4 of the device's memory
+        g->lanes = 2 * lane_bytes(pass_odds) <= g->mem_bytes / 4 * 3 ? kPassLanes : 1;
+        if (g->lanes.load() < kPassLanes) {
+            hipSetDevice(g->device);
+            for (int i = g->lanes.load(); i < kPassLanes; ++i) free_buffers(g->lane[i]);
+        }
+    }
     return MPS_OK;
 }
 

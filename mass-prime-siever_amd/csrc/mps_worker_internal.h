@@ -52,6 +52,8 @@ struct mps_gpu {
     mutable std::mutex tune_mu;
     std::map<uint32_t, mps_tuning> tuned;
     PassLane lane[kPassLanes];
+    std::atomic<int> lanes{kPassLanes};  // lanes in use: one when two passes' buffers would not fit the device
+    size_t mem_bytes = 0;    // the device's memory
     std::atomic<uint32_t> next_lane{0};  // the lane a call waits for when none is free
     mutable std::mutex st_mu;  // st and err, written by calls on different lanes
     std::mutex noalloc_mu;  // one allocation-free call at a time (they share the scratch array)
