@@ -513,10 +513,14 @@ __device__ __forceinline__ uint64_t mulmod_qo(uint64_t a, uint64_t b, const Item
 // one search bound, where a wavefront of candidates ran the step of every prime any of its lanes
 // had, for as long as its costliest lane needed.
 constexpr uint32_t kOrderRound = 256;
-// Pair bins j * 6 + min(exponent, 5): a wavefront of pairs shares the prime and the search bound.
-// (Splitting the bins further by the bit length of h, the exponent of the pair's powers, measured
-// slower: c2 setup 1.69 -> 1.83 ms.)
-constexpr uint32_t kOrderBins = 6 * kOrderMaxPrimes;
+// Pair bins (bitlen(g) / 6, at most 3) * 96 + j * 6 + min(exponent, 5): a wavefront of pairs shares
+// the prime and the search bound, and candidates with like g (so like power exponents h = g / r^a)
+// come together, which the per-candidate class of g allows without a per-pair cost (a per-pair
+// exact bit length of h measured slower: c2 setup 1.69 -> 1.83 ms).  On c2's pairs the lanes'
+// loops run 44% of the wavefront's longest instead of 39%; c4 setup 19.7 -> 19.3 ms per item, and
+// with two feeders c4 45.1 -> 42.4 ms (the setups of the two lanes then overlap each other less).
+constexpr uint32_t kOrderBins = 4 * 6 * kOrderMaxPrimes;
+static_assert(kOrderBins == 6 * 64, "wavefront 0 scans the pair bins, six per lane");
 // Pair slots of a round: 8 per candidate, where a candidate averages about 2.5 (one pair per prime
 // of Qo dividing p - 1); a round with more (never seen: over 70 standard deviations out) hands
 // all its candidates to the host.
@@ -977,6 +981,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
             const uint32_t c = threadIdx.x, i = t0 + c;
             const bool live = i < n;
             uint64_t gx = 0;
+            uint32_t gcls = 0;  // bitlen(g) / 6, at most 3: the pairs' powers have exponents g / r^a
             for (uint32_t b = c; b < kOrderBins; b += 256) R.bin[b] = 0;  // more bins than threads
             if (c == 0) R.nkeep = 0;
             // phase 1: g, b^e and t^e of each candidate
@@ -986,6 +991,7 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
                 F f;
                 f.init(pc);
                 const uint64_t g = order_g(pc, ip, gx);
+                gcls = min(bitlen64(g) / 6u, 3u) * 96u;
                 const uint32_t tz = __builtin_ctzll(g);
                 const uint64_t e = exact_div_odd((pc - 1) >> tz, g >> tz);
                 T Be, Te;
@@ -1007,24 +1013,36 @@ __device__ __forceinline__ void setup_pass(const PassDesc* __restrict__ d, PassB
             if (live)
                 for (uint32_t j = 0; j < ip.nopr; ++j) {
                     const uint32_t ej = (uint32_t)(gx >> (4 * j)) & 15u;
-                    if (ej != 0) atomicAdd(&R.bin[j * 6 + min(ej, 5u)], 1u);
+                    if (ej != 0) atomicAdd(&R.bin[gcls + j * 6 + min(ej, 5u)], 1u);
                 }
             __syncthreads();
-            if (c == 0) {
-                uint32_t tot = 0;
-                for (uint32_t b = 0; b < kOrderBins; ++b) {
-                    const uint32_t v = R.bin[b];
-                    R.bin[b] = tot;
-                    tot += v;
+            if (c < 64) {  // wavefront 0: exclusive prefix sum of the bins, six per lane
+                uint32_t v[6], s = 0;
+#pragma unroll
+                for (uint32_t k = 0; k < 6; ++k) {
+                    v[k] = R.bin[6 * c + k];
+                    s += v[k];
                 }
-                R.npair = tot;
+                uint32_t incl = s;
+#pragma unroll
+                for (uint32_t o = 1; o < 64; o <<= 1) {
+                    const uint32_t u = (uint32_t)__shfl_up((int)incl, o);
+                    if (c >= o) incl += u;
+                }
+                uint32_t ex = incl - s;
+#pragma unroll
+                for (uint32_t k = 0; k < 6; ++k) {
+                    R.bin[6 * c + k] = ex;
+                    ex += v[k];
+                }
+                if (c == 63) R.npair = incl;
             }
             __syncthreads();
             const bool overflow = R.npair > kOrderPairs;  // the same in every lane
             if (live && !overflow)
                 for (uint32_t j = 0; j < ip.nopr; ++j) {
                     const uint32_t ej = (uint32_t)(gx >> (4 * j)) & 15u;
-                    if (ej != 0) R.pair[atomicAdd(&R.bin[j * 6 + min(ej, 5u)], 1u)] = (uint16_t)(c | (j << 8));
+                    if (ej != 0) R.pair[atomicAdd(&R.bin[gcls + j * 6 + min(ej, 5u)], 1u)] = (uint16_t)(c | (j << 8));
                 }
             if (live && overflow) {  // no room for the round's pairs: the host decides its candidates
                 put_record(pb, atomicAdd(&pb.cnt->n_out, 1u), pb.out_index ? (uint32_t)pb.cand[i] : R.p[c], kHostDecides);
