@@ -180,3 +180,30 @@ def test_batch_edge_items(gpu, mps, oracle, policy):
     finally:
         eng.close()
         gpu.set_class_policy(1)
+
+
+def test_batch_planted_items(gpu, mps, oracle):
+    """Every item of tests/golden/planted.json (relations planted at every magnitude, in both fields,
+    small and large b and c) through the batch engine in order mode (class policy 2), submitted
+    at once: each item's relations equal the oracle's over its P window, and a planted relation
+    is among them."""
+    import json
+    import os
+    planted = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "planted.json")))["items"]
+    gpu.set_class_policy(2)
+    eng = mps.Batch(gpu, slots=6, pass_odds=PASS_ODDS)
+    try:
+        print(f"batch: {len(planted)} planted items", flush=True)
+        ids = [eng.submit(mps.work_item(*it["item"])) for it in planted]
+        eng.wait_all()
+        assert all(d["status"] == mps.MPS_OK for d in eng.poll())
+        for i, it in zip(ids, planted):
+            b, c, P0, P1, N0, N1, mod, res = it["item"]
+            want, _ = oracle.process_range(oracle.make_item(*it["item"]), P0, P1, 1 << 17, 16)
+            got = eng.relations(i)
+            assert got == want, it["tag"]
+            if it["relation"] is not None:
+                assert tuple(it["relation"]) in got, it["tag"]
+    finally:
+        eng.close()
+        gpu.set_class_policy(1)
